@@ -1,0 +1,793 @@
+// C ABI of the Stage-2 engine (include/cgbdsm.h).  Each entry point names the
+// reference member function it replaces; all device work of a query runs on
+// that query's own HIP stream, and no entry point exits the process.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "engine.h"
+#include "kernels.h"
+
+using namespace cgd;
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+};
+
+struct cgbdsm_ctx {
+    int device = 0;
+    uint64_t budget = 0, used = 0;
+    int num_cus = 256;
+};
+
+struct cgbdsm_query {
+    cgbdsm_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    cgbdsm_plan plan{};
+    DevPlan hplan{};
+    DevPlan* d_plan = nullptr;
+    bool have_order = false;
+    uint32_t qv = 0, qe = 0, ndir = 0, ebits = 5, ushift = 59, umask = 0;
+    uint32_t dv = 0, dv_cap = 0, vbits = 1, rbits = 1, W = 0;
+    DBuf<MRec> m_rec;
+    DBuf<URec> u_rec;
+    DBuf<uint32_t> pool;
+    Counters* d_cnt = nullptr;
+    Counters* h_cnt = nullptr;
+    DBuf<uint64_t> k_ins, k_del, k_upd, k_usort, k_alt, seg_start, caps;
+    DBuf<uint32_t> flags, seg_row, sc_val, sc_pos, rs_hist, u_vals, u_rows;
+    DBuf<uint8_t> scan_scratch;
+    DBuf<uint64_t> masks;
+    DBuf<uint32_t> front[2];
+    uint64_t n_ins = 0, n_del = 0, n_upd = 0, u_rows_upper = 0;
+    uint8_t edge_ok[kMaxQE] = {};
+    uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
+    uint32_t donate_min = 0;
+    int phase = 0;  // 0 idle, 1 staged, 2 applied, 3 U built, 4 masks, 5 matched
+    bool u_nonempty = false;
+    uint32_t last_depth = 0, last_buf = 0;
+    uint64_t last_rows = 0;
+    cgbdsm_stats st{};
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+cgbdsm_status Fail(cgbdsm_status c, const std::string& m) {
+    g_err = m;
+    return c;
+}
+
+template <class F>
+cgbdsm_status Guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return CGBDSM_OK;
+    } catch (const ApiError& e) {
+        return Fail(e.code, e.what);
+    } catch (const HipError& e) {
+        return Fail(e.e == hipErrorOutOfMemory ? CGBDSM_ERR_OOM : CGBDSM_ERR_DEVICE, e.what);
+    } catch (const std::bad_alloc&) {
+        return Fail(CGBDSM_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return Fail(CGBDSM_ERR_DEVICE, e.what());
+    }
+}
+
+[[noreturn]] void Throw(cgbdsm_status c, const std::string& m) { throw ApiError{c, m}; }
+
+void* DevAlloc(cgbdsm_ctx* ctx, size_t bytes) {
+    if (!bytes) return nullptr;
+    if (ctx->budget && ctx->used + bytes > ctx->budget)
+        Throw(CGBDSM_ERR_OOM, "device memory budget exceeded (" + std::to_string(ctx->used + bytes) + " > " +
+                                  std::to_string(ctx->budget) + " bytes)");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        Throw(CGBDSM_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    ctx->used += bytes;
+    return p;
+}
+
+void DevFree(cgbdsm_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->used -= std::min<uint64_t>(ctx->used, bytes);
+}
+
+template <class T>
+void Ensure(cgbdsm_query* q, DBuf<T>& b, size_t n, bool keep = false, size_t keep_n = 0) {
+    if (n <= b.cap) return;
+    size_t nc = std::max<size_t>({n, b.cap + b.cap / 2, 1024});
+    T* np = (T*)DevAlloc(q->ctx, nc * sizeof(T));
+    if (keep && b.p && keep_n)
+        CGD_HIP(hipMemcpyAsync(np, b.p, std::min(keep_n, b.cap) * sizeof(T), hipMemcpyDeviceToDevice, q->s));
+    if (b.p) {
+        CGD_HIP(hipStreamSynchronize(q->s));
+        DevFree(q->ctx, b.p, b.cap * sizeof(T));
+    }
+    b.p = np;
+    b.cap = nc;
+}
+
+template <class T>
+void Release(cgbdsm_query* q, DBuf<T>& b) {
+    DevFree(q->ctx, b.p, b.cap * sizeof(T));
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+Tables MakeTables(cgbdsm_query* q) {
+    Tables T;
+    T.P = q->d_plan;
+    T.m_rec = q->m_rec.p;
+    T.u_rec = q->u_rec.p;
+    T.pool = q->pool.p;
+    T.u_vals = q->u_vals.p;
+    T.masks = q->masks.p;
+    T.ndir = q->ndir;
+    T.qv = q->qv;
+    T.qe = q->qe;
+    T.dv = q->dv;
+    T.W = q->W;
+    return T;
+}
+
+void ReadCounters(cgbdsm_query* q) {
+    CGD_HIP(hipMemcpyAsync(q->h_cnt, q->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, q->s));
+    CGD_HIP(hipStreamSynchronize(q->s));
+}
+
+unsigned long long* Cnt(cgbdsm_query* q, unsigned long long Counters::*f) { return &(q->d_cnt->*f); }
+
+void SetCounter(cgbdsm_query* q, unsigned long long Counters::*f, uint64_t v) {
+    FillU64((uint64_t*)Cnt(q, f), v, 1, q->s);
+}
+
+void GrowDV(cgbdsm_query* q, uint32_t dv) {
+    if (dv > q->dv_cap) {
+        uint32_t nc = std::max<uint32_t>({dv, q->dv_cap + q->dv_cap / 2, 1024});
+        const size_t old_rows = (size_t)q->dv_cap * q->ndir, new_rows = (size_t)nc * q->ndir;
+        Ensure(q, q->m_rec, new_rows, true, old_rows);
+        Ensure(q, q->u_rec, new_rows, true, old_rows);
+        // zero the new tail (records are pairs / quads of u32)
+        FillU64((uint64_t*)(q->m_rec.p + old_rows), 0, (q->m_rec.cap - old_rows) * 2, q->s);
+        FillU64((uint64_t*)(q->u_rec.p + old_rows), 0, (q->u_rec.cap - old_rows), q->s);
+        q->dv_cap = (uint32_t)std::min<size_t>(q->m_rec.cap, q->u_rec.cap) / q->ndir;
+    }
+    if (dv > q->dv) q->dv = dv;
+    q->W = (q->dv + 63) / 64;
+    q->vbits = BitsFor(std::max<uint32_t>(q->dv, 2));
+    q->rbits = BitsFor((uint64_t)std::max<uint32_t>(q->dv, 1) * q->ndir);
+}
+
+void EnsureKeyScratch(cgbdsm_query* q, uint64_t n) {
+    n = std::max<uint64_t>(n, 1);
+    Ensure(q, q->k_alt, n);
+    Ensure(q, q->flags, n + 1);
+    Ensure(q, q->seg_row, n + 1);
+    Ensure(q, q->seg_start, n + 2);
+    Ensure(q, q->caps, n + 1);
+    Ensure(q, q->sc_val, n);
+    Ensure(q, q->sc_pos, n);
+    Ensure(q, q->rs_hist, RadixHistWords(n));
+    Ensure(q, q->scan_scratch, ScanScratchBytes(n + 2) + 64);
+}
+
+void CheckKeys(cgbdsm_query* q, const uint64_t* k, uint64_t n, uint32_t dv, const char* what) {
+    if (n && !k) Throw(CGBDSM_ERR_INVALID, std::string(what) + ": null key pointer");
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t e = k[i] >> q->ushift, u = (k[i] >> 32) & q->umask, v = (uint32_t)k[i];
+        if (e >= q->ndir || u >= dv || v >= dv)
+            Throw(CGBDSM_ERR_INVALID, std::string(what) + " key " + std::to_string(i) + " out of range (edge " +
+                                          std::to_string(e) + ", owner " + std::to_string(u) + ", target " +
+                                          std::to_string(v) + ", dv " + std::to_string(dv) + ")");
+    }
+}
+
+// Sort boundary keys `src` (n) into internal order in `dst` (dst may be any
+// buffer of n u64; k_alt is used as the ping-pong partner unless it is dst).
+void DecodeSort(cgbdsm_query* q, const uint64_t* src, uint64_t* dst, uint64_t* alt, uint64_t n) {
+    LaunchDecode(src, dst, n, q->ushift, q->umask, q->ndir, q->vbits, q->s);
+    RadixSortU64(dst, alt, n, q->vbits + q->rbits, q->rs_hist.p, q->s);
+}
+
+void GrowPool(cgbdsm_query* q, uint64_t min_cap) {
+    ReadCounters(q);
+    uint64_t used = q->h_cnt->pool_top;
+    uint64_t nc = std::max<uint64_t>({min_cap, q->pool.cap * 2, used + (64ull << 20)});
+    uint32_t* np = (uint32_t*)DevAlloc(q->ctx, nc * sizeof(uint32_t));
+    if (q->pool.p && used)
+        CGD_HIP(hipMemcpyAsync(np, q->pool.p, std::min<uint64_t>(used, q->pool.cap) * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, q->s));
+    CGD_HIP(hipStreamSynchronize(q->s));
+    DevFree(q->ctx, q->pool.p, q->pool.cap * sizeof(uint32_t));
+    q->pool.p = np;
+    q->pool.cap = nc;
+}
+
+// M <- M u keys (sorted internal keys, unique), retrying after pool growth.
+void InsertSorted(cgbdsm_query* q, const uint64_t* keys, const unsigned long long* n_segs, uint64_t n_upper) {
+    for (int attempt = 0;; attempt++) {
+        SetCounter(q, &Counters::flags, 0);
+        LaunchMergeInsert(MakeTables(q), keys, q->vbits, q->seg_row.p, q->seg_start.p, n_segs, n_upper, q->sc_val.p,
+                          q->sc_pos.p, q->pool.cap, q->d_cnt, q->s);
+        ReadCounters(q);
+        if (!(q->h_cnt->flags & kFlagPool)) return;
+        if (attempt >= 4) Throw(CGBDSM_ERR_OOM, "list pool overflow after growth");
+        GrowPool(q, q->pool.cap * 2);
+        // Lists that already merged see an empty B' and are skipped on retry.
+    }
+}
+
+void ValidatePlanTables(const cgbdsm_plan* p, bool order) {
+    auto bad = [](const std::string& m) { Throw(CGBDSM_ERR_INVALID, "plan: " + m); };
+    if (p->qv < 3 || p->qv > (uint32_t)kMaxQV) bad("qv must be 3..12");
+    if (p->qe < 2 || p->qe > (uint32_t)kMaxQE) bad("qe must be 2..42");
+    if (p->ndir != 2 * p->qe) bad("ndir != 2*qe");
+    if (p->ebits != 5 && p->ebits != 7) bad("ebits must be 5 or 7");
+    if (p->ndir > (1u << p->ebits)) Throw(CGBDSM_ERR_LIMIT, "query has more directed edges than the key format holds");
+    uint32_t seen = 0;
+    for (uint32_t a = 0; a < p->qv; a++)
+        for (uint32_t b = 0; b < p->qv; b++) {
+            uint8_t e = p->eidx[a * kMaxQV + b];
+            if (e == CGBDSM_NO_EDGE) continue;
+            if (e >= p->ndir || a == b) bad("eidx out of range");
+            if (p->dir_to_edge[e] >= p->qe) bad("dir_to_edge out of range");
+            if (p->eidx[b * kMaxQV + a] == CGBDSM_NO_EDGE) bad("eidx not symmetric");
+            seen++;
+        }
+    if (seen != p->ndir) bad("eidx does not list every directed edge");
+    for (uint32_t i = 0; i < p->qe; i++) {
+        if (p->qe_u[i] >= p->qv || p->qe_v[i] >= p->qv) bad("qe list out of range");
+        if (p->eidx[p->qe_u[i] * kMaxQV + p->qe_v[i]] == CGBDSM_NO_EDGE) bad("qe list edge missing");
+        if (p->dir_to_edge[p->eidx[p->qe_u[i] * kMaxQV + p->qe_v[i]]] != i) bad("dir_to_edge mismatch");
+    }
+    auto check_order = [&](const uint8_t* vs, const uint8_t* offs, const uint8_t* bni, int first_depth) {
+        uint32_t mask = 0;
+        for (uint32_t d = 0; d < p->qv; d++) {
+            if (vs[d] >= p->qv || (mask >> vs[d]) & 1u) bad("order is not a permutation");
+            mask |= 1u << vs[d];
+        }
+        for (uint32_t d = 0; d < p->qv; d++) {
+            if (offs[d] > offs[d + 1] || offs[d + 1] > (uint32_t)kMaxDir) bad("bni offsets");
+            if ((int)d < first_depth) continue;
+            if (offs[d] == offs[d + 1]) bad("vertex without backward neighbour");
+            for (uint32_t k = offs[d]; k < offs[d + 1]; k++) {
+                if (bni[k] >= d) bad("backward neighbour not earlier");
+                if (p->eidx[vs[bni[k]] * kMaxQV + vs[d]] == CGBDSM_NO_EDGE) bad("backward neighbour not adjacent");
+            }
+        }
+    };
+    for (uint32_t i = 0; i < p->qe; i++) {
+        check_order(p->io_vs[i], p->io_bni_offs[i], p->io_bni[i], 2);
+        if (p->io_vs[i][0] != p->qe_u[i] || p->io_vs[i][1] != p->qe_v[i]) bad("indexing order must start at the edge");
+    }
+    if (order) {
+        check_order(p->go_vs, p->go_bni_offs, p->go_bni, 1);
+        if (p->eidx[p->go_vs[0] * kMaxQV + p->go_vs[1]] == CGBDSM_NO_EDGE) bad("global order must start with an edge");
+    }
+}
+
+void UploadPlan(cgbdsm_query* q) {
+    CGD_HIP(hipMemcpyAsync(q->d_plan, &q->hplan, sizeof(DevPlan), hipMemcpyHostToDevice, q->s));
+    CGD_HIP(hipStreamSynchronize(q->s));
+}
+
+EiList ActiveEdges(cgbdsm_query* q) {
+    EiList L{};
+    for (uint32_t e = 0; e < q->qe; e++)
+        if (q->edge_ok[e]) L.ei[L.n++] = (uint8_t)e;
+    return L;
+}
+
+float Elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cgbdsm_last_error(void) { return g_err.c_str(); }
+
+const char* cgbdsm_version(void) { return "cgbdsm 0.1 gfx950 wave64 keys:ebits5|7"; }
+
+cgbdsm_status cgbdsm_ctx_create(int device, uint64_t budget_bytes, cgbdsm_ctx** out) {
+    return Guard([&] {
+        if (!out) Throw(CGBDSM_ERR_INVALID, "null out");
+        int n = 0;
+        CGD_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) Throw(CGBDSM_ERR_INVALID, "no such device");
+        CGD_HIP(hipSetDevice(device));
+        hipDeviceProp_t pr;
+        CGD_HIP(hipGetDeviceProperties(&pr, device));
+        auto* c = new cgbdsm_ctx;
+        c->device = device;
+        c->budget = budget_bytes;
+        c->num_cus = pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 256;
+        *out = c;
+    });
+}
+
+cgbdsm_status cgbdsm_ctx_destroy(cgbdsm_ctx* ctx) {
+    return Guard([&] { delete ctx; });
+}
+
+cgbdsm_status cgbdsm_query_create(cgbdsm_ctx* ctx, const cgbdsm_plan* plan, cgbdsm_query** out) {
+    return Guard([&] {
+        if (!ctx || !plan || !out) Throw(CGBDSM_ERR_INVALID, "null argument");
+        ValidatePlanTables(plan, false);
+        CGD_HIP(hipSetDevice(ctx->device));
+        auto* q = new cgbdsm_query;
+        try {
+            q->ctx = ctx;
+            q->plan = *plan;
+            q->qv = plan->qv;
+            q->qe = plan->qe;
+            q->ndir = plan->ndir;
+            q->ebits = plan->ebits;
+            q->ushift = 64 - plan->ebits;
+            q->umask = (1u << (32 - plan->ebits)) - 1u;
+            CGD_HIP(hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
+            for (auto& e : q->ev) CGD_HIP(hipEventCreate(&e));
+            q->d_plan = (DevPlan*)DevAlloc(ctx, sizeof(DevPlan));
+            q->d_cnt = (Counters*)DevAlloc(ctx, sizeof(Counters));
+            CGD_HIP(hipHostMalloc((void**)&q->h_cnt, sizeof(Counters), hipHostMallocDefault));
+            memset(q->h_cnt, 0, sizeof(Counters));
+            FillU64((uint64_t*)q->d_cnt, 0, sizeof(Counters) / 8, q->s);
+            q->hplan.p = *plan;
+            q->hplan.tag_shift = q->qe <= 32 ? 27 : 26;
+            q->hplan.v_mask = (1u << q->hplan.tag_shift) - 1u;
+            for (uint32_t e = 0; e < q->qe; e++) {
+                q->hplan.sa[e] = plan->eidx[plan->qe_u[e] * kMaxQV + plan->qe_v[e]];
+                q->hplan.sb[e] = plan->eidx[plan->qe_v[e] * kMaxQV + plan->qe_u[e]];
+                for (uint32_t v = 0; v < (uint32_t)kMaxQV; v++) q->hplan.all_ones[e][v] = 0;
+            }
+            UploadPlan(q);
+        } catch (...) {
+            cgbdsm_query_destroy(q);
+            throw;
+        }
+        *out = q;
+    });
+}
+
+cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q) return;
+        if (q->s) (void)hipStreamSynchronize(q->s);
+        Release(q, q->m_rec);
+        Release(q, q->u_rec);
+        Release(q, q->pool);
+        for (auto* b : {&q->k_ins, &q->k_del, &q->k_upd, &q->k_usort, &q->k_alt, &q->seg_start, &q->caps, &q->masks})
+            Release(q, *b);
+        for (auto* b : {&q->flags, &q->seg_row, &q->sc_val, &q->sc_pos, &q->rs_hist, &q->u_vals, &q->u_rows,
+                        &q->front[0], &q->front[1]})
+            Release(q, *b);
+        Release(q, q->scan_scratch);
+        if (q->d_plan) DevFree(q->ctx, q->d_plan, sizeof(DevPlan));
+        if (q->d_cnt) DevFree(q->ctx, q->d_cnt, sizeof(Counters));
+        if (q->h_cnt) (void)hipHostFree(q->h_cnt);
+        for (auto& e : q->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (q->s) (void)hipStreamDestroy(q->s);
+        delete q;
+    });
+}
+
+cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
+    return Guard([&] {
+        if (!q || !plan) Throw(CGBDSM_ERR_INVALID, "null argument");
+        if (plan->qv != q->qv || plan->qe != q->qe || plan->ndir != q->ndir || plan->ebits != q->ebits ||
+            memcmp(plan->eidx, q->plan.eidx, sizeof(plan->eidx)) != 0)
+            Throw(CGBDSM_ERR_INVALID, "plan does not belong to this query");
+        ValidatePlanTables(plan, true);
+        q->plan = *plan;
+        q->hplan.p = *plan;
+        for (uint32_t e = 0; e < q->qe; e++)
+            for (uint32_t v = 0; v < q->qv; v++) q->hplan.all_ones[e][v] = plan->rebuild_v[e][v] ? 0 : 1;
+        UploadPlan(q);
+        q->have_order = true;
+    });
+}
+
+cgbdsm_status cgbdsm_index_init(cgbdsm_query* q, const uint64_t* keys, uint64_t n, uint32_t dv_count,
+                                uint32_t* cardinalities, float* avg_degrees) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (dv_count > (1u << (32 - q->ebits)) || dv_count > q->hplan.v_mask)
+            Throw(CGBDSM_ERR_LIMIT, "compressed vertex count exceeds the key / row-tag width");
+        CheckKeys(q, keys, n, dv_count, "index_init");
+        GrowDV(q, std::max<uint32_t>(dv_count, 1));
+        q->dv = dv_count;
+        q->W = (q->dv + 63) / 64;
+        FillU64((uint64_t*)q->m_rec.p, 0, q->m_rec.cap * 2, q->s);
+        FillU64((uint64_t*)q->u_rec.p, 0, q->u_rec.cap, q->s);
+        FillU64((uint64_t*)q->d_cnt, 0, sizeof(Counters) / 8, q->s);
+        q->u_rows_upper = 0;
+        Ensure(q, q->k_ins, std::max<uint64_t>(n, 1));
+        EnsureKeyScratch(q, n);
+        if (n) CGD_HIP(hipMemcpyAsync(q->k_ins.p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, q->s));
+        DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, n);
+        LaunchUnique(q->k_alt.p, q->k_ins.p, n, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique), q->s);
+        LaunchSegments(q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p, q->seg_row.p, q->seg_start.p,
+                       Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+        LaunchInitCaps(q->seg_start.p, Cnt(q, &Counters::n_segs), n, q->caps.p, q->s);
+        // exclusive scan of capacities -> offsets; total into pool_top
+        // (flags hold the head positions needed by the layout kernel)
+        ScanU64(q->caps.p, q->caps.p, std::max<uint64_t>(n, 1), (uint64_t*)Cnt(q, &Counters::pool_top),
+                q->scan_scratch.p, q->s);
+        ReadCounters(q);
+        const uint64_t total = q->h_cnt->pool_top;
+        const uint64_t want = std::max<uint64_t>(total + total / 2 + (16ull << 20), 1ull << 24);
+        if (q->pool.cap < want) {
+            Release(q, q->pool);
+            q->pool.p = (uint32_t*)DevAlloc(q->ctx, want * sizeof(uint32_t));
+            q->pool.cap = want;
+        }
+        LaunchInitLayout(MakeTables(q), q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p,
+                         q->seg_row.p, q->seg_start.p, Cnt(q, &Counters::n_segs), q->caps.p, q->d_cnt, q->s);
+        ReadCounters(q);
+        for (uint32_t e = 0; e < q->ndir; e++) {
+            const uint64_t c = q->h_cnt->card[e], nd = q->h_cnt->nodes[e];
+            if (cardinalities) cardinalities[e] = (uint32_t)c;
+            if (avg_degrees) avg_degrees[e] = nd ? (float)(uint32_t)c / (float)(uint32_t)nd : 0.0f;
+        }
+        q->st.main_entries = q->h_cnt->n_unique;
+        q->phase = 0;
+    });
+}
+
+cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q, const uint64_t* ins, uint64_t n_ins, const uint64_t* del,
+                                 uint64_t n_del, const uint64_t* upd, uint64_t n_upd, const uint8_t* edge_ok,
+                                 uint32_t dv_count) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (!q->have_order) Throw(CGBDSM_ERR_INVALID, "cgbdsm_query_set_plan must run before the first batch");
+        if (!edge_ok) Throw(CGBDSM_ERR_INVALID, "null edge_ok");
+        if (dv_count < q->dv) Throw(CGBDSM_ERR_INVALID, "compressed vertex count cannot shrink");
+        if (dv_count > (1u << (32 - q->ebits)) || dv_count > q->hplan.v_mask)
+            Throw(CGBDSM_ERR_LIMIT, "compressed vertex count exceeds the key / row-tag width");
+        CheckKeys(q, ins, n_ins, dv_count, "ins");
+        CheckKeys(q, del, n_del, dv_count, "del");
+        CheckKeys(q, upd, n_upd, dv_count, "upd");
+        GrowDV(q, dv_count);
+        const uint64_t nmax = std::max<uint64_t>({n_ins, n_del, n_upd, 1});
+        Ensure(q, q->k_ins, std::max<uint64_t>(n_ins, 1));
+        Ensure(q, q->k_del, std::max<uint64_t>(n_del, 1));
+        Ensure(q, q->k_upd, std::max<uint64_t>(n_upd, 1));
+        Ensure(q, q->k_usort, std::max<uint64_t>(n_upd, 1));
+        Ensure(q, q->u_vals, std::max<uint64_t>(n_upd, 1));
+        // keep the previous batch's touched-row list readable until U is rebuilt
+        Ensure(q, q->u_rows, std::max<uint64_t>(n_upd, 1), true, q->u_rows_upper);
+        EnsureKeyScratch(q, nmax);
+        if (n_ins) CGD_HIP(hipMemcpyAsync(q->k_ins.p, ins, n_ins * 8, hipMemcpyHostToDevice, q->s));
+        if (n_del) CGD_HIP(hipMemcpyAsync(q->k_del.p, del, n_del * 8, hipMemcpyHostToDevice, q->s));
+        if (n_upd) CGD_HIP(hipMemcpyAsync(q->k_upd.p, upd, n_upd * 8, hipMemcpyHostToDevice, q->s));
+        q->n_ins = n_ins;
+        q->n_del = n_del;
+        q->n_upd = n_upd;
+        memcpy(q->edge_ok, edge_ok, q->qe);
+        q->st.last_ins = n_ins;
+        q->st.last_del = n_del;
+        q->st.last_upd = n_upd;
+        q->phase = 1;
+    });
+}
+
+cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q || q->phase != 1) Throw(CGBDSM_ERR_INVALID, "transfer_update_all needs a staged batch");
+        SetCounter(q, &Counters::bytes, 0);
+        SetCounter(q, &Counters::rows_touched, 0);
+        if (q->n_ins) {
+            DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, q->n_ins);
+            LaunchUnique(q->k_alt.p, q->k_ins.p, q->n_ins, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
+                         q->s);
+            LaunchSegments(q->k_ins.p, Cnt(q, &Counters::n_unique), q->n_ins, q->vbits, q->flags.p, q->seg_row.p,
+                           q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+            InsertSorted(q, q->k_ins.p, Cnt(q, &Counters::n_segs), q->n_ins);
+        }
+        if (q->n_del) {
+            DecodeSort(q, q->k_del.p, q->k_alt.p, q->k_del.p, q->n_del);
+            SetCounter(q, &Counters::n_unique, q->n_del);
+            LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_del, q->vbits, q->flags.p, q->seg_row.p,
+                           q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+            LaunchMergeRemove(MakeTables(q), q->k_alt.p, q->vbits, q->seg_row.p, q->seg_start.p,
+                              Cnt(q, &Counters::n_segs), q->n_del, q->d_cnt, q->s);
+        }
+        q->phase = 2;
+    });
+}
+
+cgbdsm_status cgbdsm_transfer_update_index(cgbdsm_query* q, int* nonempty) {
+    return Guard([&] {
+        if (!q || q->phase != 2) Throw(CGBDSM_ERR_INVALID, "transfer_update_index must follow transfer_update_all");
+        LaunchClearU(q->u_rec.p, q->u_rows.p, Cnt(q, &Counters::u_rows), q->u_rows_upper, q->s);
+        SetCounter(q, &Counters::u_rows, 0);
+        q->u_rows_upper = 0;
+        q->u_nonempty = q->n_upd > 0;
+        if (q->n_upd) {
+            DecodeSort(q, q->k_upd.p, q->k_usort.p, q->k_alt.p, q->n_upd);
+            SetCounter(q, &Counters::n_unique, q->n_upd);
+            LaunchSegments(q->k_usort.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
+                           q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+            LaunchBuildU(q->u_rec.p, q->k_usort.p, q->n_upd, q->vbits, q->u_vals.p, q->seg_row.p, q->seg_start.p,
+                         Cnt(q, &Counters::n_segs), q->u_rows.p, Cnt(q, &Counters::u_rows), q->s);
+            q->u_rows_upper = q->n_upd;
+        }
+        if (nonempty) *nonempty = q->u_nonempty ? 1 : 0;
+        q->phase = 3;
+    });
+}
+
+cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q || q->phase != 3) Throw(CGBDSM_ERR_INVALID, "build_local_index must follow transfer_update_index");
+        const EiList L = ActiveEdges(q);
+        Ensure(q, q->masks, std::max<uint64_t>((uint64_t)q->qe * kMaxQV * q->W, 1));
+        const Tables T = MakeTables(q);
+        LaunchMaskClear(q->masks.p, (uint64_t)kMaxQV * q->W, L, q->s);
+        EiSeedInfo info{};
+        for (uint32_t k = 0; k < L.n; k++) {
+            const uint32_t ei = L.ei[k];
+            const uint8_t a = q->plan.io_vs[ei][0], b = q->plan.io_vs[ei][1];
+            info.ei[info.n] = (uint8_t)ei;
+            info.i01[info.n] = q->plan.eidx[a * kMaxQV + b];
+            info.i10[info.n] = q->plan.eidx[b * kMaxQV + a];
+            info.vs0[info.n] = a;
+            info.vs1[info.n] = b;
+            info.n++;
+        }
+        LaunchMaskSeed(T, q->u_rows.p, Cnt(q, &Counters::u_rows), q->u_rows_upper, info, q->s);
+        for (uint32_t d = 2; d < q->qv; d++) {
+            EiList D{};
+            for (uint32_t k = 0; k < L.n; k++) {
+                const uint32_t ei = L.ei[k];
+                if (q->plan.rebuild_v[ei][q->plan.io_vs[ei][d]]) D.ei[D.n++] = (uint8_t)ei;
+            }
+            LaunchMaskDepth(T, d, D, q->s);
+        }
+        q->phase = 4;
+    });
+}
+
+cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit, uint64_t frontier_cap,
+                                      uint32_t donate_min) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        q->bfs_row_limit = bfs_row_limit;
+        q->frontier_cap_rows = frontier_cap;
+        q->donate_min = donate_min;
+    });
+}
+
+cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge) {
+    return Guard([&] {
+        if (!q || q->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
+        const EiList L = ActiveEdges(q);
+        for (auto f : {&Counters::rows_out, &Counters::row_next, &Counters::flags, &Counters::tuples})
+            SetCounter(q, f, 0);
+        FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
+        const Tables T = MakeTables(q);
+        const bool fixed_cap = q->frontier_cap_rows != 0;
+        uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : std::max<uint64_t>(q->bfs_row_limit * 4, 1u << 20);
+        Ensure(q, q->front[0], cap_rows * 2);
+        // Step 1: seeds (retry once with a buffer of the exact size on overflow).
+        uint64_t n = 0;
+        for (int attempt = 0;; attempt++) {
+            LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
+            ReadCounters(q);
+            n = q->h_cnt->rows_out;
+            if (!(q->h_cnt->flags & kFlagFrontier)) break;
+            if (attempt) Throw(CGBDSM_ERR_OVERFLOW, "seed frontier overflow");
+            Ensure(q, q->front[0], n * 2 + 64);
+            SetCounter(q, &Counters::rows_out, 0);
+            SetCounter(q, &Counters::flags, 0);
+        }
+        uint32_t depth = 2, cur = 0;
+        // Step 2: BFS levels while the frontier is small.
+        while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
+            const uint64_t want_words = (fixed_cap ? q->frontier_cap_rows : std::max<uint64_t>(n * 64, 1u << 20)) *
+                                        (depth + 1);
+            Ensure(q, q->front[cur ^ 1], want_words);
+            SetCounter(q, &Counters::rows_out, 0);
+            SetCounter(q, &Counters::flags, 0);
+            LaunchExtend(T, q->front[cur].p, n, depth, q->front[cur ^ 1].p, q->front[cur ^ 1].cap / (depth + 1),
+                         q->d_cnt, q->s);
+            ReadCounters(q);
+            if (q->h_cnt->flags & kFlagFrontier) break;  // keep this level, count it by DFS
+            n = q->h_cnt->rows_out;
+            cur ^= 1;
+            depth++;
+        }
+        // Step 5: final DFS count from the current level.
+        SetCounter(q, &Counters::row_next, 0);
+        if (n) LaunchDfs(T, q->front[cur].p, n, depth, q->d_cnt, nullptr, 0, q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+        ReadCounters(q);
+        unsigned long long total = 0;
+        for (uint32_t e = 0; e < q->qe; e++) {
+            total += q->h_cnt->per_edge[e];
+            if (per_edge) per_edge[e] = q->h_cnt->per_edge[e];
+        }
+        if (num_matches) *num_matches += total;
+        q->last_depth = depth;
+        q->last_rows = n;
+        q->last_buf = cur;
+        q->st.last_frontier_rows = n;
+        q->st.last_frontier_depth = depth;
+        q->st.last_matches = total;
+        q->phase = 5;
+    });
+}
+
+cgbdsm_status cgbdsm_match_tuples(cgbdsm_query* q, uint32_t* out, uint64_t cap, uint64_t* n_out, uint8_t* ei_out) {
+    return Guard([&] {
+        if (!q || q->phase != 5) Throw(CGBDSM_ERR_INVALID, "match_tuples must follow matching");
+        const uint64_t words = (q->qv + 1) * std::max<uint64_t>(cap, 1);
+        uint32_t* d_out = (uint32_t*)DevAlloc(q->ctx, words * 4);
+        try {
+            SetCounter(q, &Counters::row_next, 0);
+            SetCounter(q, &Counters::tuples, 0);
+            SetCounter(q, &Counters::flags, 0);
+            FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
+            if (q->last_rows)
+                LaunchDfs(MakeTables(q), q->front[q->last_buf].p, q->last_rows, q->last_depth, q->d_cnt, d_out, cap,
+                          q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+            ReadCounters(q);
+            const uint64_t n = q->h_cnt->tuples;
+            if (n_out) *n_out = n;
+            const uint64_t k = std::min(n, cap);
+            std::vector<uint32_t> h(k * (q->qv + 1));
+            if (k) CGD_HIP(hipMemcpy(h.data(), d_out, h.size() * 4, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < k; i++) {
+                if (out) memcpy(out + i * q->qv, &h[i * (q->qv + 1)], q->qv * 4);
+                if (ei_out) ei_out[i] = (uint8_t)h[i * (q->qv + 1) + q->qv];
+            }
+        } catch (...) {
+            DevFree(q->ctx, d_out, words * 4);
+            throw;
+        }
+        DevFree(q->ctx, d_out, words * 4);
+    });
+}
+
+cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q || (q->phase != 5 && q->phase != 4 && q->phase != 3))
+            Throw(CGBDSM_ERR_INVALID, "batch_update_global_index must follow transfer_update_index");
+        if (q->n_upd) {
+            LaunchUnique(q->k_usort.p, q->k_alt.p, q->n_upd, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
+                         q->s);
+            LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
+                           q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+            InsertSorted(q, q->k_alt.p, Cnt(q, &Counters::n_segs), q->n_upd);
+        }
+        q->st.batches++;
+        q->phase = 0;
+    });
+}
+
+cgbdsm_status cgbdsm_run_batch(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge, int* skipped) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        auto chk = [](cgbdsm_status s) {
+            if (s != CGBDSM_OK) Throw(s, g_err);
+        };
+        CGD_HIP(hipEventRecord(q->ev[0], q->s));
+        chk(cgbdsm_transfer_update_all(q));
+        CGD_HIP(hipEventRecord(q->ev[1], q->s));
+        int ne = 0;
+        chk(cgbdsm_transfer_update_index(q, &ne));
+        CGD_HIP(hipEventRecord(q->ev[2], q->s));
+        if (skipped) *skipped = ne ? 0 : 1;
+        if (!ne) {
+            q->phase = 0;
+            q->st.batches++;
+            CGD_HIP(hipStreamSynchronize(q->s));
+            q->st.ms_apply = Elapsed(q->ev[0], q->ev[1]);
+            q->st.ms_update_index = Elapsed(q->ev[1], q->ev[2]);
+            q->st.ms_masks = q->st.ms_match = q->st.ms_commit = 0;
+            return;
+        }
+        chk(cgbdsm_build_local_index(q));
+        CGD_HIP(hipEventRecord(q->ev[3], q->s));
+        chk(cgbdsm_matching(q, num_matches, per_edge));
+        CGD_HIP(hipEventRecord(q->ev[4], q->s));
+        chk(cgbdsm_batch_update_global_index(q));
+        CGD_HIP(hipEventRecord(q->ev[5], q->s));
+        CGD_HIP(hipStreamSynchronize(q->s));
+        q->st.ms_apply = Elapsed(q->ev[0], q->ev[1]);
+        q->st.ms_update_index = Elapsed(q->ev[1], q->ev[2]);
+        q->st.ms_masks = Elapsed(q->ev[2], q->ev[3]);
+        q->st.ms_match = Elapsed(q->ev[3], q->ev[4]);
+        q->st.ms_commit = Elapsed(q->ev[4], q->ev[5]);
+    });
+}
+
+cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        const uint64_t nrows = (uint64_t)q->dv * q->ndir;
+        uint64_t* lens = (uint64_t*)DevAlloc(q->ctx, (nrows + 1) * 8);
+        void* scr = DevAlloc(q->ctx, ScanScratchBytes(nrows + 1) + 64);
+        uint64_t* tot = (uint64_t*)DevAlloc(q->ctx, 8);
+        uint64_t total = 0;
+        std::vector<uint64_t> h;
+        try {
+            LaunchRowLens(MakeTables(q), nrows, which, lens, q->s);
+            ScanU64(lens, lens, nrows, tot, scr, q->s);
+            CGD_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, q->s));
+            CGD_HIP(hipStreamSynchronize(q->s));
+            if (total) {
+                uint64_t* d = (uint64_t*)DevAlloc(q->ctx, total * 8);
+                LaunchExport(MakeTables(q), nrows, which, lens, q->ushift, d, q->s);
+                h.resize(total);
+                CGD_HIP(hipMemcpyAsync(h.data(), d, total * 8, hipMemcpyDeviceToHost, q->s));
+                CGD_HIP(hipStreamSynchronize(q->s));
+                DevFree(q->ctx, d, total * 8);
+            }
+        } catch (...) {
+            DevFree(q->ctx, lens, (nrows + 1) * 8);
+            DevFree(q->ctx, scr, ScanScratchBytes(nrows + 1) + 64);
+            DevFree(q->ctx, tot, 8);
+            throw;
+        }
+        DevFree(q->ctx, lens, (nrows + 1) * 8);
+        DevFree(q->ctx, scr, ScanScratchBytes(nrows + 1) + 64);
+        DevFree(q->ctx, tot, 8);
+        std::sort(h.begin(), h.end());
+        if (n_out) *n_out = total;
+        if (out) memcpy(out, h.data(), std::min(cap, total) * 8);
+    });
+}
+
+cgbdsm_status cgbdsm_mask_export(cgbdsm_query* q, uint32_t ei, uint32_t qv, uint64_t* words, uint64_t nwords,
+                                 int* all_ones) {
+    return Guard([&] {
+        if (!q || ei >= q->qe || qv >= q->qv) Throw(CGBDSM_ERR_INVALID, "bad tile");
+        if (q->phase < 4) Throw(CGBDSM_ERR_INVALID, "masks are built by build_local_index");
+        const bool ones = q->hplan.all_ones[ei][qv] != 0;
+        if (all_ones) *all_ones = ones ? 1 : 0;
+        if (!ones && words && nwords) {
+            if (!q->edge_ok[ei]) Throw(CGBDSM_ERR_INVALID, "tile of an edge without batch lists is not built");
+            CGD_HIP(hipMemcpyAsync(words, q->masks.p + ((uint64_t)ei * kMaxQV + qv) * q->W,
+                                   std::min<uint64_t>(nwords, q->W) * 8, hipMemcpyDeviceToHost, q->s));
+            CGD_HIP(hipStreamSynchronize(q->s));
+        }
+    });
+}
+
+cgbdsm_status cgbdsm_stats_get(cgbdsm_query* q, cgbdsm_stats* out) {
+    return Guard([&] {
+        if (!q || !out) Throw(CGBDSM_ERR_INVALID, "null argument");
+        ReadCounters(q);
+        q->st.dv_count = q->dv;
+        q->st.pool_used = q->h_cnt->pool_top;
+        q->st.pool_cap = q->pool.cap;
+        q->st.last_rows_touched = q->h_cnt->rows_touched;
+        q->st.last_idx_bytes = q->h_cnt->bytes;
+        *out = q->st;
+    });
+}
+
+cgbdsm_status cgbdsm_sync(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        CGD_HIP(hipStreamSynchronize(q->s));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,412 @@
+// Index maintenance on gfx950: key decode, unique, segment discovery, and the
+// per-list sorted merges that keep the main index M and build the update
+// index U.  Replaces generateSizePtr (index/calig_helper_gamma.cu:223-236),
+// mergeCSROntoGraph / removeCSROromGraph (kernels/indexing.cu:296-458) and
+// the initial build of ConvertGlobalIndex (index/calig_helper_gamma.cu:238-360).
+//
+// Set semantics on insert (as index/cpu_index_mirror.cpp:155): values already
+// present are dropped before the merge, so lists never hold duplicates.
+//
+// Merge-path per list, one wavefront per list:
+//   1. dedup: each lane binary-searches its delta value in the old list; the
+//      survivors B' are compacted with a ballot and their final positions
+//      (rank in old list + rank in B') are recorded.
+//   2. if the list keeps fitting its capacity, only the tail of the old list
+//      from the first insertion point moves, back to front in 64-wide chunks
+//      (every store lands at or above the loads still pending); otherwise the
+//      merged list is written to a fresh pool block of CapFor(len) slots.
+//   3. B' values are scattered to their recorded positions.
+// Removal is an in-place ballot compaction starting at the first deleted value.
+#include "engine.h"
+#include "kernels.h"
+#include "wave.h"
+
+namespace cgd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+
+__global__ void KDecode(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint32_t ushift,
+                        uint32_t umask, uint32_t ndir, uint32_t vbits) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = in[i];
+        uint64_t e = k >> ushift, u = (k >> 32) & umask, v = (uint32_t)k;
+        out[i] = ((u * ndir + e) << vbits) | v;
+    }
+}
+
+__global__ void KMarkUnique(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void KCompactUnique(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ pos,
+                               uint64_t* __restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (i == 0 || keys[i] != keys[i - 1]) out[pos[i]] = keys[i];
+}
+
+__global__ void KMarkHeads(const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ n_dev,
+                           uint64_t n_upper, uint32_t vbits, uint32_t* __restrict__ flags) {
+    const uint64_t n = *n_dev;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_upper;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        flags[i] = (i < n && (i == 0 || (keys[i] >> vbits) != (keys[i - 1] >> vbits))) ? 1u : 0u;
+}
+
+__global__ void KWriteSegs(const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ n_dev,
+                           uint32_t vbits, const uint32_t* __restrict__ pos, const unsigned long long* __restrict__ n_segs,
+                           uint32_t* __restrict__ seg_row, uint64_t* __restrict__ seg_start) {
+    const uint64_t n = *n_dev;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i == 0 || (keys[i] >> vbits) != (keys[i - 1] >> vbits)) {
+            seg_row[pos[i]] = (uint32_t)(keys[i] >> vbits);
+            seg_start[pos[i]] = i;
+        }
+        if (i == n - 1) seg_start[*n_segs] = n;
+    }
+}
+
+__device__ __forceinline__ uint32_t LowerBoundKeys(const uint64_t* __restrict__ k, uint32_t n, uint32_t v,
+                                                   uint64_t vmask) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if ((uint32_t)(k[m] & vmask) < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// One wavefront per touched list: M[row] <- M[row] u B (B sorted, unique).
+__global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_t* __restrict__ keys, uint32_t vbits,
+                                                         const uint32_t* __restrict__ seg_row,
+                                                         const uint64_t* __restrict__ seg_start,
+                                                         const unsigned long long* __restrict__ n_segs_dev,
+                                                         uint32_t* __restrict__ sc_val, uint32_t* __restrict__ sc_pos,
+                                                         uint64_t pool_cap, Counters* __restrict__ cnt) {
+    const uint32_t lane = LaneId();
+    const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    unsigned long long bytes = 0, touched = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; s < n_segs; s += nw) {
+        const uint32_t row = seg_row[s];
+        const uint64_t bs = seg_start[s];
+        const uint32_t nb = (uint32_t)(seg_start[s + 1] - bs);
+        MRec rec = T.m_rec[row];
+        uint32_t* A = T.pool + rec.off;
+        const uint32_t na = rec.len;
+        // 1. B' = B \ A with final positions.
+        uint32_t nbp = 0;
+        for (uint32_t k = 0; k < nb; k += kWave) {
+            const uint32_t j = k + lane;
+            const bool valid = j < nb;
+            const uint32_t b = valid ? (uint32_t)(keys[bs + j] & vmask) : 0u;
+            const uint32_t p = valid ? LowerBound(A, na, b) : 0u;
+            const bool keep = valid && !(p < na && A[p] == b);
+            const uint64_t m = __ballot(keep);
+            const uint32_t r = RankBelow(m);
+            if (keep) {
+                sc_val[bs + nbp + r] = b;
+                sc_pos[bs + nbp + r] = p + nbp + r;
+            }
+            nbp += (uint32_t)__popcll(m);
+        }
+        bytes += 8ull * nb;
+        if (nbp == 0) continue;
+        WaveSync();
+        const uint32_t newlen = na + nbp;
+        const uint32_t* Bv = sc_val + bs;
+        const uint32_t* Bp = sc_pos + bs;
+        uint32_t* C = A;
+        if (newlen > rec.cap) {
+            const uint32_t ncap = CapFor(newlen);
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(&cnt->pool_top, (unsigned long long)ncap);
+            off = Uniform64(__shfl(off, 0, kWave));
+            if (off + ncap > pool_cap) {
+                if (lane == 0) atomicOr(&cnt->flags, kFlagPool);
+                continue;
+            }
+            C = T.pool + off;
+            for (uint32_t k = 0; k < na; k += kWave) {
+                const uint32_t i = k + lane;
+                if (i < na) {
+                    const uint32_t a = A[i];
+                    C[i + LowerBound(Bv, nbp, a)] = a;
+                }
+            }
+            bytes += 4ull * na + 4ull * na;
+            if (lane == 0) {
+                rec.off = off;
+                rec.cap = ncap;
+            }
+        } else {
+            // In place: only A[i0..na) moves, back to front.
+            const uint32_t i0 = Bp[0];
+            for (uint32_t hi = na; hi > i0;) {
+                const uint32_t lo = hi > i0 + kWave ? hi - kWave : i0;
+                const uint32_t i = hi - 1 - lane;
+                const bool valid = lane < hi - lo;
+                uint32_t a = 0, r = 0;
+                if (valid) {
+                    a = A[i];
+                    r = LowerBound(Bv, nbp, a);
+                }
+                WaveSync();
+                if (valid) A[i + r] = a;
+                WaveSync();
+                hi = lo;
+            }
+            bytes += 8ull * (na - i0);
+        }
+        for (uint32_t k = 0; k < nbp; k += kWave) {
+            const uint32_t j = k + lane;
+            if (j < nbp) C[Bp[j]] = Bv[j];
+        }
+        bytes += 4ull * nbp;
+        touched++;
+        if (lane == 0) {
+            rec.len = newlen;
+            T.m_rec[row] = rec;
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&cnt->bytes, bytes);
+        atomicAdd(&cnt->rows_touched, touched);
+    }
+}
+
+// One wavefront per touched list: M[row] <- M[row] \ D (D sorted).
+__global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_t* __restrict__ keys, uint32_t vbits,
+                                                         const uint32_t* __restrict__ seg_row,
+                                                         const uint64_t* __restrict__ seg_start,
+                                                         const unsigned long long* __restrict__ n_segs_dev,
+                                                         Counters* __restrict__ cnt) {
+    const uint32_t lane = LaneId();
+    const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    unsigned long long bytes = 0, touched = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; s < n_segs; s += nw) {
+        const uint32_t row = seg_row[s];
+        const uint64_t bs = seg_start[s];
+        const uint32_t nd = (uint32_t)(seg_start[s + 1] - bs);
+        const uint64_t* D = keys + bs;
+        const uint32_t na = T.m_rec[row].len;
+        uint32_t* A = T.pool + T.m_rec[row].off;
+        bytes += 8ull * nd;
+        if (na == 0) continue;
+        const uint32_t i0 = LowerBound(A, na, (uint32_t)(D[0] & vmask));
+        uint32_t w = i0;
+        for (uint32_t k = i0; k < na; k += kWave) {
+            const uint32_t i = k + lane;
+            const bool valid = i < na;
+            const uint32_t a = valid ? A[i] : 0u;
+            bool del = false;
+            if (valid) {
+                const uint32_t p = LowerBoundKeys(D, nd, a, vmask);
+                del = p < nd && (uint32_t)(D[p] & vmask) == a;
+            }
+            const bool keep = valid && !del;
+            const uint64_t m = __ballot(keep);
+            WaveSync();
+            if (keep) A[w + RankBelow(m)] = a;
+            WaveSync();
+            w += (uint32_t)__popcll(m);
+        }
+        bytes += 4ull * (na - i0) + 4ull * (w - i0);
+        if (w != na) touched++;
+        if (lane == 0) T.m_rec[row].len = w;
+    }
+    if (lane == 0) {
+        atomicAdd(&cnt->bytes, bytes);
+        atomicAdd(&cnt->rows_touched, touched);
+    }
+}
+
+__global__ void KClearU(URec* __restrict__ u_rec, const uint32_t* __restrict__ u_rows,
+                        const unsigned long long* __restrict__ n_dev) {
+    const uint64_t n = *n_dev;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        u_rec[u_rows[i]] = URec{0, 0};
+}
+
+__global__ void KBuildUValues(const uint64_t* __restrict__ keys, uint64_t n, uint32_t vbits, uint32_t* __restrict__ vals) {
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        vals[i] = (uint32_t)(keys[i] & vmask);
+}
+
+__global__ void KBuildURows(URec* __restrict__ u_rec, const uint32_t* __restrict__ seg_row,
+                            const uint64_t* __restrict__ seg_start, const unsigned long long* __restrict__ n_segs_dev,
+                            uint32_t* __restrict__ u_rows, unsigned long long* __restrict__ u_rows_count) {
+    const uint64_t n = *n_segs_dev;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t row = seg_row[s];
+        u_rec[row] = URec{(uint32_t)seg_start[s], (uint32_t)(seg_start[s + 1] - seg_start[s])};
+        u_rows[s] = row;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *u_rows_count = n;
+}
+
+// Initial build: per segment capacity (in the u64 scan buffer), then layout.
+__global__ void KInitCaps(const uint64_t* __restrict__ seg_start, const unsigned long long* __restrict__ n_segs_dev,
+                          uint64_t n_upper, uint64_t* __restrict__ caps) {
+    const uint64_t n = *n_segs_dev;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_upper; s += (uint64_t)gridDim.x * blockDim.x)
+        caps[s] = s < n ? CapFor((uint32_t)(seg_start[s + 1] - seg_start[s])) : 0;
+}
+
+__global__ void KInitRecs(Tables T, const uint32_t* __restrict__ seg_row, const uint64_t* __restrict__ seg_start,
+                          const unsigned long long* __restrict__ n_segs_dev, const uint64_t* __restrict__ offs,
+                          Counters* __restrict__ cnt) {
+    const uint64_t n = *n_segs_dev;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t row = seg_row[s];
+        const uint32_t len = (uint32_t)(seg_start[s + 1] - seg_start[s]);
+        T.m_rec[row] = MRec{offs[s], len, CapFor(len)};
+        const uint32_t e = row % T.ndir;
+        atomicAdd(&cnt->card[e], (unsigned long long)len);
+        atomicAdd(&cnt->nodes[e], 1ull);
+    }
+}
+
+__global__ void KInitValues(Tables T, const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ n_dev,
+                            uint32_t vbits, const uint32_t* __restrict__ head_pos, const uint64_t* __restrict__ seg_start,
+                            const uint64_t* __restrict__ offs) {
+    const uint64_t n = *n_dev;
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool head = i == 0 || (keys[i] >> vbits) != (keys[i - 1] >> vbits);
+        const uint64_t seg = (uint64_t)head_pos[i] + (head ? 1 : 0) - 1;
+        T.pool[offs[seg] + (i - seg_start[seg])] = (uint32_t)(keys[i] & vmask);
+    }
+}
+
+// Export helpers: M/U as (row, v) pairs in boundary key form, row order.
+__global__ void KExportM(Tables T, const uint64_t* __restrict__ row_off, uint64_t nrows, uint32_t ushift,
+                         uint64_t* __restrict__ out) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+        const MRec rec = T.m_rec[r];
+        const uint64_t e = r % T.ndir, u = r / T.ndir;
+        for (uint32_t j = 0; j < rec.len; j++)
+            out[row_off[r] + j] = (e << ushift) | (u << 32) | T.pool[rec.off + j];
+    }
+}
+
+__global__ void KRowLens(Tables T, uint64_t nrows, int which, uint64_t* __restrict__ lens) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x)
+        lens[r] = which ? T.u_rec[r].len : T.m_rec[r].len;
+}
+
+__global__ void KExportU(Tables T, const uint64_t* __restrict__ row_off, uint64_t nrows, uint32_t ushift,
+                         uint64_t* __restrict__ out) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+        const URec rec = T.u_rec[r];
+        const uint64_t e = r % T.ndir, u = r / T.ndir;
+        for (uint32_t j = 0; j < rec.len; j++)
+            out[row_off[r] + j] = (e << ushift) | (u << 32) | T.u_vals[rec.off + j];
+    }
+}
+
+unsigned GridFor(uint64_t n, unsigned per_block = 256, unsigned cap = 8192) {
+    uint64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+}  // namespace
+
+void LaunchDecode(const uint64_t* in, uint64_t* out, uint64_t n, uint32_t ushift, uint32_t umask, uint32_t ndir,
+                  uint32_t vbits, hipStream_t s) {
+    if (!n) return;
+    KDecode<<<GridFor(n), 256, 0, s>>>(in, out, n, ushift, umask, ndir, vbits);
+}
+
+void LaunchUnique(const uint64_t* in, uint64_t* out, uint64_t n, uint32_t* flags, void* scan_scratch,
+                  unsigned long long* n_out_dev, hipStream_t s) {
+    if (!n) {
+        FillU64((uint64_t*)n_out_dev, 0, 1, s);
+        return;
+    }
+    KMarkUnique<<<GridFor(n), 256, 0, s>>>(in, n, flags);
+    ScanU32(flags, flags, n, (uint64_t*)n_out_dev, scan_scratch, s);
+    KCompactUnique<<<GridFor(n), 256, 0, s>>>(in, n, flags, out);
+}
+
+void LaunchSegments(const uint64_t* keys, const unsigned long long* n_dev, uint64_t n_upper, uint32_t vbits,
+                    uint32_t* flags, uint32_t* seg_row, uint64_t* seg_start, unsigned long long* n_segs_dev,
+                    void* scan_scratch, hipStream_t s) {
+    if (!n_upper) {
+        FillU64((uint64_t*)n_segs_dev, 0, 1, s);
+        FillU64(seg_start, 0, 1, s);
+        return;
+    }
+    KMarkHeads<<<GridFor(n_upper), 256, 0, s>>>(keys, n_dev, n_upper, vbits, flags);
+    ScanU32(flags, flags, n_upper, (uint64_t*)n_segs_dev, scan_scratch, s);
+    FillU64(seg_start, 0, 1, s);  // sentinel when the key set is empty
+    KWriteSegs<<<GridFor(n_upper), 256, 0, s>>>(keys, n_dev, vbits, flags, n_segs_dev, seg_row, seg_start);
+}
+
+void LaunchMergeInsert(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint32_t* sc_val,
+                       uint32_t* sc_pos, uint64_t pool_cap, Counters* cnt, hipStream_t s) {
+    if (!n_upper) return;
+    KMergeInsert<<<GridFor(n_upper, kWavesPerBlock, 16384), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
+                                                                             n_segs, sc_val, sc_pos, pool_cap, cnt);
+}
+
+void LaunchMergeRemove(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, Counters* cnt,
+                       hipStream_t s) {
+    if (!n_upper) return;
+    KMergeRemove<<<GridFor(n_upper, kWavesPerBlock, 16384), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
+                                                                             n_segs, cnt);
+}
+
+void LaunchClearU(URec* u_rec, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,
+                  hipStream_t s) {
+    if (!n_upper) return;
+    KClearU<<<GridFor(n_upper), 256, 0, s>>>(u_rec, u_rows, n_dev);
+}
+
+void LaunchBuildU(URec* u_rec, const uint64_t* keys, uint64_t n, uint32_t vbits, uint32_t* vals,
+                  const uint32_t* seg_row, const uint64_t* seg_start, const unsigned long long* n_segs, uint32_t* u_rows,
+                  unsigned long long* u_rows_count, hipStream_t s) {
+    if (n) KBuildUValues<<<GridFor(n), 256, 0, s>>>(keys, n, vbits, vals);
+    KBuildURows<<<GridFor(n ? n : 1), 256, 0, s>>>(u_rec, seg_row, seg_start, n_segs, u_rows, u_rows_count);
+}
+
+void LaunchInitCaps(const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint64_t* caps,
+                    hipStream_t s) {
+    if (!n_upper) return;
+    KInitCaps<<<GridFor(n_upper), 256, 0, s>>>(seg_start, n_segs, n_upper, caps);
+}
+
+void LaunchInitLayout(const Tables& T, const uint64_t* keys, const unsigned long long* n_dev, uint64_t n_upper,
+                      uint32_t vbits, const uint32_t* head_pos, const uint32_t* seg_row, const uint64_t* seg_start,
+                      const unsigned long long* n_segs, const uint64_t* offs, Counters* cnt, hipStream_t s) {
+    if (!n_upper) return;
+    KInitRecs<<<GridFor(n_upper), 256, 0, s>>>(T, seg_row, seg_start, n_segs, offs, cnt);
+    KInitValues<<<GridFor(n_upper), 256, 0, s>>>(T, keys, n_dev, vbits, head_pos, seg_start, offs);
+}
+
+void LaunchRowLens(const Tables& T, uint64_t nrows, int which, uint64_t* lens, hipStream_t s) {
+    if (!nrows) return;
+    KRowLens<<<GridFor(nrows), 256, 0, s>>>(T, nrows, which, lens);
+}
+
+void LaunchExport(const Tables& T, uint64_t nrows, int which, const uint64_t* row_off, uint32_t ushift, uint64_t* out,
+                  hipStream_t s) {
+    if (!nrows) return;
+    if (which)
+        KExportU<<<GridFor(nrows), 256, 0, s>>>(T, row_off, nrows, ushift, out);
+    else
+        KExportM<<<GridFor(nrows), 256, 0, s>>>(T, row_off, nrows, ushift, out);
+}
+
+}  // namespace cgd

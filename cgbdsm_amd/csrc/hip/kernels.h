@@ -1,0 +1,61 @@
+// Host-side launchers of the gfx950 kernels (one translation unit each).
+#pragma once
+#include "engine.h"
+
+namespace cgd {
+
+// Small lists passed by value in the kernel argument segment.
+struct EiList {
+    uint32_t n;
+    uint8_t ei[kMaxQE];
+};
+struct EiSeedInfo {
+    uint32_t n;
+    uint8_t ei[kMaxQE], i01[kMaxQE], i10[kMaxQE], vs0[kMaxQE], vs1[kMaxQE];
+};
+
+// index.hip
+void LaunchDecode(const uint64_t* in, uint64_t* out, uint64_t n, uint32_t ushift, uint32_t umask, uint32_t ndir,
+                  uint32_t vbits, hipStream_t s);
+void LaunchUnique(const uint64_t* in, uint64_t* out, uint64_t n, uint32_t* flags, void* scan_scratch,
+                  unsigned long long* n_out_dev, hipStream_t s);
+void LaunchSegments(const uint64_t* keys, const unsigned long long* n_dev, uint64_t n_upper, uint32_t vbits,
+                    uint32_t* flags, uint32_t* seg_row, uint64_t* seg_start, unsigned long long* n_segs_dev,
+                    void* scan_scratch, hipStream_t s);
+void LaunchMergeInsert(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint32_t* sc_val,
+                       uint32_t* sc_pos, uint64_t pool_cap, Counters* cnt, hipStream_t s);
+void LaunchMergeRemove(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, Counters* cnt,
+                       hipStream_t s);
+void LaunchClearU(URec* u_rec, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,
+                  hipStream_t s);
+void LaunchBuildU(URec* u_rec, const uint64_t* keys, uint64_t n, uint32_t vbits, uint32_t* vals,
+                  const uint32_t* seg_row, const uint64_t* seg_start, const unsigned long long* n_segs, uint32_t* u_rows,
+                  unsigned long long* u_rows_count, hipStream_t s);
+void LaunchInitCaps(const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint64_t* caps,
+                    hipStream_t s);
+void LaunchInitLayout(const Tables& T, const uint64_t* keys, const unsigned long long* n_dev, uint64_t n_upper,
+                      uint32_t vbits, const uint32_t* head_pos, const uint32_t* seg_row, const uint64_t* seg_start,
+                      const unsigned long long* n_segs, const uint64_t* offs, Counters* cnt, hipStream_t s);
+void LaunchRowLens(const Tables& T, uint64_t nrows, int which, uint64_t* lens, hipStream_t s);
+void LaunchExport(const Tables& T, uint64_t nrows, int which, const uint64_t* row_off, uint32_t ushift, uint64_t* out,
+                  hipStream_t s);
+
+// masks.hip
+void LaunchMaskClear(uint64_t* masks, uint64_t ei_words, const EiList& L, hipStream_t s);
+void LaunchMaskSeed(const Tables& T, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,
+                    const EiSeedInfo& info, hipStream_t s);
+void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, hipStream_t s);
+
+// match.hip
+void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);
+void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out,
+                  uint64_t cap_rows, Counters* cnt, hipStream_t s);
+// Final DFS over n rows of `depth` words; counts per edge tag into cnt->per_edge.
+// tuples != nullptr: also writes full embeddings (qv words + tag word) up to tup_cap.
+void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt, uint32_t* tuples,
+               uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
+uint32_t DfsBlocksPerCU();
+
+}  // namespace cgd

@@ -1,0 +1,298 @@
+// Merged matching of all query edges on gfx950 (MatchingBitAll,
+// graph/graph_gpu.cu:1261-1480).
+//
+//   KSeed    depth-2 rows [(ei << tag_shift) | v0, v1] for every edge tag ei
+//            whose batch lists exist (writeInitialResultsAllBit,
+//            kernels/enumeration.cu:727-809).  One wavefront per 64 owners =
+//            one support-mask word; wave-wide scan + one atomic reserves rows.
+//   KExtend  one BFS level, one wavefront per row, 64 candidates per step
+//            (extendBFSAllBit with write_res, kernels/enumeration.cu:811-1155).
+//   KDfs     the final count: persistent wavefronts pull rows from a global
+//            atomic queue; each runs an iterative DFS whose per-level survivor
+//            stacks live in LDS (64 u32 per level).  At the last level the
+//            survivors are only counted (popcount of the ballot).
+//
+// Candidate rule at depth d (global order position d, query vertex u_d):
+//   list of the first backward neighbour, in its M list or (for ei's own two
+//   directions) its U list, plus its U list when that query edge's index < ei;
+//   support mask bit of (ei, u_d); injective w.r.t. positions 0..d-1; present
+//   in every other backward neighbour's list (binary search in M then U).
+#include "engine.h"
+#include "kernels.h"
+#include "wave.h"
+
+namespace cgd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWpb = kThreads / kWave;
+constexpr uint32_t kGrab = 4;  // rows taken per queue atomic
+
+struct List {
+    const uint32_t* base;
+    const uint32_t* app;
+    uint32_t bl, al;
+};
+
+__device__ __forceinline__ bool TileBit(const Tables& T, uint32_t ei, uint32_t qv, uint32_t v) {
+    if (T.P->all_ones[ei][qv]) return true;
+    return (T.masks[((uint64_t)ei * kMaxQV + qv) * T.W + (v >> 6)] >> (v & 63)) & 1ull;
+}
+
+__device__ __forceinline__ List Resolve(const Tables& T, uint32_t dir, uint32_t owner, uint32_t ei) {
+    List L;
+    const uint64_t r = (uint64_t)owner * T.ndir + dir;
+    if (dir == T.P->sa[ei] || dir == T.P->sb[ei]) {
+        const URec u = T.u_rec[r];
+        L.base = T.u_vals + u.off;
+        L.bl = u.len;
+    } else {
+        const MRec m = T.m_rec[r];
+        L.base = T.pool + m.off;
+        L.bl = m.len;
+    }
+    if (T.P->p.dir_to_edge[dir] < ei) {
+        const URec u = T.u_rec[r];
+        L.app = T.u_vals + u.off;
+        L.al = u.len;
+    } else {
+        L.app = nullptr;
+        L.al = 0;
+    }
+    return L;
+}
+
+__device__ __forceinline__ uint32_t At(const List& L, uint32_t i) { return i < L.bl ? L.base[i] : L.app[i - L.bl]; }
+
+// Candidate c for position d given positions 0..d-1 in `matched` (LDS, uniform).
+__device__ __forceinline__ bool Accept(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, uint32_t c) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t ud = p.go_vs[d];
+    if (!TileBit(T, ei, ud, c)) return false;
+    for (uint32_t i = 0; i < d; i++)
+        if (matched[i] == c) return false;
+    const uint32_t s = p.go_bni_offs[d] + 1u, e = p.go_bni_offs[d + 1];
+    for (uint32_t k = s; k < e; k++) {
+        const uint32_t bq = p.go_bni[k];
+        const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + ud];
+        const List L = Resolve(T, dir, Uniform(matched[bq]), ei);
+        if (!SortedContains(L.base, L.bl, c) && !(L.al && SortedContains(L.app, L.al, c))) return false;
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t* __restrict__ rows, uint64_t cap_rows,
+                                                  Counters* __restrict__ cnt) {
+    const uint32_t ei = EL.ei[blockIdx.y];
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t q0 = p.go_vs[0], q1 = p.go_vs[1];
+    const uint32_t idx = p.eidx[q0 * kMaxQV + q1];
+    const uint32_t lane = LaneId();
+    const uint32_t tag = ei << T.P->tag_shift;
+    const bool all0 = T.P->all_ones[ei][q0];
+    const uint64_t* tile0 = T.masks + ((uint64_t)ei * kMaxQV + q0) * T.W;
+    const uint64_t nw = (uint64_t)gridDim.x * kWpb;
+    for (uint64_t w = (uint64_t)blockIdx.x * kWpb + threadIdx.x / kWave; w < T.W; w += nw) {
+        const uint32_t v = (uint32_t)(w * 64 + lane);
+        const uint64_t word = all0 ? ~0ull : tile0[w];
+        const bool ok = v < T.dv && ((word >> lane) & 1ull);
+        if (!__ballot(ok)) continue;
+        List L{nullptr, nullptr, 0, 0};
+        uint32_t c = 0;
+        if (ok) {
+            L = Resolve(T, idx, v, ei);
+            for (uint32_t t = 0; t < L.bl + L.al; t++) c += TileBit(T, ei, q1, At(L, t)) ? 1u : 0u;
+        }
+        const uint32_t incl = WaveInclusiveScan(c);
+        const uint32_t total = __shfl(incl, 63, kWave);
+        if (total == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)total);
+        base = __shfl(base, 0, kWave);
+        if (base + total > cap_rows) {
+            if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
+            continue;
+        }
+        uint64_t pos = base + incl - c;
+        if (ok) {
+            for (uint32_t t = 0; t < L.bl + L.al; t++) {
+                const uint32_t nb = At(L, t);
+                if (TileBit(T, ei, q1, nb)) {
+                    rows[2 * pos] = tag | v;
+                    rows[2 * pos + 1] = nb;
+                    pos++;
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __restrict__ in, uint64_t n_in,
+                                                    uint32_t D, uint32_t* __restrict__ out, uint64_t cap_rows,
+                                                    Counters* __restrict__ cnt) {
+    __shared__ uint32_t s_matched[kWpb][kMaxQV];
+    const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
+    uint32_t* matched = s_matched[wv];
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t ud = p.go_vs[D];
+    const uint32_t pq = p.go_bni[p.go_bni_offs[D]];
+    const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
+    const uint64_t nw = (uint64_t)gridDim.x * kWpb;
+    for (uint64_t r = (uint64_t)blockIdx.x * kWpb + wv; r < n_in; r += nw) {
+        const uint32_t w0 = Uniform(in[r * D]);
+        const uint32_t ei = w0 >> T.P->tag_shift;
+        if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : in[r * D + lane];
+        WaveSync();
+        const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
+        const uint32_t total = L.bl + L.al;
+        for (uint32_t off = 0; off < total; off += kWave) {
+            const uint32_t i = off + lane;
+            uint32_t c = 0;
+            bool ok = false;
+            if (i < total) {
+                c = At(L, i);
+                ok = Accept(T, ei, D, matched, c);
+            }
+            const uint64_t m = __ballot(ok);
+            const uint32_t k = (uint32_t)__popcll(m);
+            if (!k) continue;
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)k);
+            base = __shfl(base, 0, kWave);
+            if (base + k > cap_rows) {
+                if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
+                continue;
+            }
+            if (ok) {
+                uint32_t* o = out + (base + RankBelow(m)) * (uint64_t)(D + 1);
+                o[0] = w0;
+                for (uint32_t j = 1; j < D; j++) o[j] = matched[j];
+                o[D] = c;
+            }
+        }
+        WaveSync();
+    }
+}
+
+struct DfsLevel {
+    uint32_t off, cnt, pos, pad;
+};
+
+template <bool kTuples>
+__global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
+                                                 uint32_t D, Counters* __restrict__ cnt, uint32_t* __restrict__ tuples,
+                                                 uint64_t tup_cap) {
+    __shared__ uint32_t s_matched[kWpb][kMaxQV];
+    __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
+    __shared__ uint32_t s_surv[kWpb][kMaxQV][kWave];
+    const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
+    uint32_t* matched = s_matched[wv];
+    DfsLevel* lvl = s_lvl[wv];
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t qv = p.qv, last = qv - 1;
+    for (;;) {
+        unsigned long long r0 = 0;
+        if (lane == 0) r0 = atomicAdd(&cnt->row_next, (unsigned long long)kGrab);
+        r0 = Uniform64(__shfl(r0, 0, kWave));
+        if (r0 >= n_rows) break;
+        const uint64_t r1 = r0 + kGrab < n_rows ? r0 + kGrab : n_rows;
+        for (uint64_t r = r0; r < r1; r++) {
+            const uint32_t w0 = Uniform(rows[r * D]);
+            const uint32_t ei = w0 >> T.P->tag_shift;
+            if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : rows[r * D + lane];
+            if (lane == 0) lvl[D] = DfsLevel{0, 0, 0, 0};
+            WaveSync();
+            unsigned long long count = 0;
+            uint32_t d = D;
+            for (;;) {
+                if (d < last) {
+                    const uint32_t pos = Uniform(lvl[d].pos);
+                    if (pos < Uniform(lvl[d].cnt)) {
+                        const uint32_t x = Uniform(s_surv[wv][d][pos]);
+                        if (lane == 0) {
+                            lvl[d].pos = pos + 1;
+                            matched[d] = x;
+                            lvl[d + 1] = DfsLevel{0, 0, 0, 0};
+                        }
+                        WaveSync();
+                        d++;
+                        continue;
+                    }
+                }
+                const uint32_t ud = p.go_vs[d];
+                const uint32_t pq = p.go_bni[p.go_bni_offs[d]];
+                const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
+                const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
+                const uint32_t total = L.bl + L.al;
+                const uint32_t off = Uniform(lvl[d].off);
+                if (off >= total) {
+                    if (d == D) break;
+                    d--;
+                    continue;
+                }
+                const uint32_t i = off + lane;
+                uint32_t c = 0;
+                bool ok = false;
+                if (i < total) {
+                    c = At(L, i);
+                    ok = Accept(T, ei, d, matched, c);
+                }
+                const uint64_t m = __ballot(ok);
+                if (d == last) {
+                    count += (unsigned long long)__popcll(m);
+                    if (kTuples && ok) {
+                        unsigned long long slot = atomicAdd(&cnt->tuples, 1ull);
+                        if (slot < tup_cap) {
+                            uint32_t* o = tuples + slot * (uint64_t)(qv + 1);
+                            for (uint32_t j = 0; j < last; j++) o[j] = matched[j];
+                            o[last] = c;
+                            o[qv] = ei;
+                        } else {
+                            atomicOr(&cnt->flags, kFlagTuples);
+                        }
+                    }
+                    if (lane == 0) lvl[d].off = off + kWave;
+                    WaveSync();
+                    continue;
+                }
+                if (ok) s_surv[wv][d][RankBelow(m)] = c;
+                if (lane == 0) lvl[d] = DfsLevel{off + kWave, (uint32_t)__popcll(m), 0, 0};
+                WaveSync();
+            }
+            if (lane == 0 && count) atomicAdd(&cnt->per_edge[ei], count);
+        }
+    }
+}
+
+unsigned Grid1(uint64_t n, unsigned per, unsigned cap) {
+    uint64_t g = (n + per - 1) / per;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+}  // namespace
+
+void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s) {
+    if (!L.n || !T.W) return;
+    KSeed<<<dim3(Grid1(T.W, kWpb, 4096), L.n), kThreads, 0, s>>>(T, L, rows, cap_rows, cnt);
+}
+
+void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out, uint64_t cap_rows,
+                  Counters* cnt, hipStream_t s) {
+    if (!n_in) return;
+    KExtend<<<Grid1(n_in, kWpb, 65535), kThreads, 0, s>>>(T, in, n_in, depth, out, cap_rows, cnt);
+}
+
+uint32_t DfsBlocksPerCU() { return 8; }
+
+void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt, uint32_t* tuples,
+               uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
+    if (!n) return;
+    const unsigned g = Grid1(n, kWpb * kGrab, grid_blocks);
+    if (tuples)
+        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, tuples, tup_cap);
+    else
+        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, nullptr, 0);
+}
+
+}  // namespace cgd

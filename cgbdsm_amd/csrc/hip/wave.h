@@ -1,0 +1,66 @@
+// wave64 helpers for gfx950 (CDNA4): lane masks, DPP-backed scans through
+// __shfl_up, and an intra-wave LDS visibility fence.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cgd {
+
+__device__ __forceinline__ uint32_t LaneId() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t LaneMaskLt(uint32_t lane) { return (1ull << lane) - 1ull; }
+
+// Number of set bits of `mask` below the calling lane (v_mbcnt).
+__device__ __forceinline__ uint32_t RankBelow(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Orders LDS and global traffic between lanes of one wavefront (lanes run in
+// lockstep, but the memory model needs the fence to publish stores).
+__device__ __forceinline__ void WaveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <class T>
+__device__ __forceinline__ T WaveInclusiveScan(T v) {
+    const uint32_t lane = LaneId();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T n = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += n;
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T WaveBroadcast(T v, int src) {
+    return __shfl(v, src, 64);
+}
+
+__device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// First index i in [0, n) with a[i] >= v (n if none).
+__device__ __forceinline__ uint32_t LowerBound(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (a[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool SortedContains(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    uint32_t p = LowerBound(a, n, v);
+    return p < n && a[p] == v;
+}
+
+}  // namespace cgd

@@ -1,0 +1,293 @@
+#include "session.h"
+
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+#include "cgbdsm_host.h"
+
+namespace cg {
+
+using Clock = std::chrono::high_resolution_clock;
+static double Ms(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+Session::Session(const std::string& qp, const std::string& dp, const std::string& up, uint32_t batch_ints,
+                 int threads, uint32_t ebits) {
+    calig_ = std::make_unique<CaLiG>(threads);
+    calig_->InputQ(qp);
+    calig_->InputG(dp);
+    calig_->InputUpdate(up, 0);
+    query_ = LoadQuery(qp);
+    Setup(batch_ints, ebits);
+}
+
+Session::Session(const SynthGraph& g, const SampledQuery& sq, uint32_t batch_ints, int threads, uint32_t ebits) {
+    calig_ = std::make_unique<CaLiG>(threads);
+    calig_->SetQuery(sq.labels, sq.edges);
+    calig_->SetData(g.labels, g.initial);
+    calig_->SetUpdates(g.stream);
+    query_ = QueryFromEdges(sq.labels, sq.edges);
+    Setup(batch_ints, ebits);
+}
+
+void Session::Setup(uint32_t batch_ints, uint32_t ebits) {
+    if (calig_->QueryVertices() != query_.vcount) throw Error("query views disagree");
+    batch_ints_ = batch_ints ? batch_ints : UINT32_MAX;
+    ebits_ = ebits;
+    uint32_t nu = calig_->UpdateSize();
+    num_batches_ = nu == 0 ? 0 : (batch_ints_ >= nu ? 1 : (nu + batch_ints_ - 1) / batch_ints_);
+    plan_ = std::make_unique<Plan>(query_);
+    plan_->GenerateIndexingOrders();
+    enc_ = std::make_unique<KeyEncoder>(*calig_, query_, ebits);
+}
+
+void Session::BuildIndex() {
+    calig_->ConstructCand();
+    calig_->StaticFilter();
+}
+
+const std::vector<uint64_t>& Session::InitialKeys() {
+    init_keys_ = enc_->InitialKeys(card_, avg_, calig_->Pool());
+    return init_keys_;
+}
+
+void Session::MakePlan(const uint32_t* card, const float* avg) { plan_->GenerateMatchingOrders(card, avg); }
+
+uint64_t Session::BatchPairs(uint32_t b) const {
+    uint64_t start = (uint64_t)b * batch_ints_;
+    uint64_t nu = calig_->UpdateSize();
+    if (start >= nu) return 0;
+    return std::min<uint64_t>(batch_ints_, nu - start) / 2;
+}
+
+const KeyBatch& Session::RunBatch(uint32_t b, S1Timing* t) {
+    uint32_t start = b * batch_ints_;
+    uint32_t count = std::min(batch_ints_, calig_->UpdateSize() - start);
+    auto t0 = Clock::now();
+    calig_->UpdateIndex(start, count);
+    auto t1 = Clock::now();
+    calig_->ConstructUpdate(start, count);
+    auto t2 = Clock::now();
+    BatchDeltas d = calig_->MergeDeltas();
+    auto t2b = Clock::now();
+    // PrepareUpdateAll and PrepareUpdateIndex are one pass here; the split of
+    // the time follows the reference's labels (ins/del vs upd work).
+    auto t3 = Clock::now();
+    cur_ = enc_->Prepare(d, calig_->Pool());
+    auto t4 = Clock::now();
+    if (t) {
+        t->update_index_ms = Ms(t0, t1);
+        t->construct_update_ms = Ms(t1, t2) + Ms(t2, t2b);
+        double prep = Ms(t3, t4);
+        double w_all = (double)(d.ins.size() + d.del.size()), w_idx = (double)d.upd.size();
+        double f = (w_all + w_idx) > 0 ? w_all / (w_all + w_idx) : 0.5;
+        t->prepare_all_ms = prep * f;
+        t->prepare_index_ms = prep * (1 - f);
+    }
+    return cur_;
+}
+
+}  // namespace cg
+
+// ----------------------------------------------------------------- C ABI
+
+struct cgh_session {
+    std::unique_ptr<cg::Session> s;
+    uint32_t next_batch = 0;
+    bool have_keys = false;
+};
+
+static thread_local std::string g_err;
+#define CGH_TRY(body)                   \
+    try {                               \
+        body;                           \
+        return 0;                       \
+    } catch (const std::exception& e) { \
+        g_err = e.what();               \
+        return 1;                       \
+    }
+
+extern "C" {
+
+const char* cgh_last_error(void) { return g_err.c_str(); }
+
+static cg::RmatSpec ToRmat(const cgh_synth_spec* sp) {
+    cg::RmatSpec r;
+    r.scale = sp->scale;
+    r.avg_degree = sp->avg_degree;
+    if (sp->a > 0) {
+        r.a = sp->a;
+        r.b = sp->b;
+        r.c = sp->c;
+    }
+    r.num_labels = sp->num_labels;
+    r.zipf_s = sp->zipf_s;
+    r.initial_frac = sp->initial_frac;
+    r.stream_edges = sp->stream_edges;
+    r.seed = sp->seed;
+    return r;
+}
+
+int cgh_generate_files(const cgh_synth_spec* spec, const char* qp, const char* dp, const char* up) {
+    CGH_TRY({
+        cg::ThreadPool pool(cg::DefaultThreads());
+        cg::SynthGraph g = cg::GenerateRmat(ToRmat(spec), &pool);
+        cg::SampledQuery q = cg::SampleQuery(g, spec->query_size, (cg::QueryKind)spec->query_kind, spec->query_seed);
+        cg::WriteQueryFile(qp, q);
+        cg::WriteDataGraph(dp, g);
+        cg::WriteStream(up, g);
+    })
+}
+
+int cgh_session_open(const char* qp, const char* dp, const char* up, uint32_t batch_ints, int threads,
+                     uint32_t ebits, cgh_session** out) {
+    CGH_TRY({
+        auto* s = new cgh_session;
+        try {
+            s->s = std::make_unique<cg::Session>(qp, dp, up, batch_ints, threads, ebits);
+            s->s->BuildIndex();
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    })
+}
+
+int cgh_session_open_synth(const cgh_synth_spec* spec, uint32_t batch_ints, int threads, uint32_t ebits,
+                           cgh_session** out) {
+    CGH_TRY({
+        auto* s = new cgh_session;
+        try {
+            cg::ThreadPool pool(threads > 0 ? threads : cg::DefaultThreads());
+            cg::SynthGraph g = cg::GenerateRmat(ToRmat(spec), &pool);
+            cg::SampledQuery q = cg::SampleQuery(g, spec->query_size, (cg::QueryKind)spec->query_kind, spec->query_seed);
+            s->s = std::make_unique<cg::Session>(g, q, batch_ints, threads, ebits);
+            s->s->BuildIndex();
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    })
+}
+
+void cgh_session_close(cgh_session* s) { delete s; }
+
+int cgh_query_tables(cgh_session* s, cgbdsm_plan* out) {
+    CGH_TRY(cg::FillQueryTables(s->s->query(), &s->s->plan(), s->s->ebits(), out))
+}
+
+int cgh_initial_keys(cgh_session* s, const uint64_t** keys, uint64_t* n, uint32_t* dv, uint32_t* card, float* avg) {
+    CGH_TRY({
+        const auto& k = s->s->InitialKeys();
+        *keys = k.data();
+        *n = k.size();
+        *dv = s->s->encoder().DV();
+        uint32_t nd = 2 * s->s->query().ecount;
+        if (card) memcpy(card, s->s->card(), nd * sizeof(uint32_t));
+        if (avg) memcpy(avg, s->s->avg(), nd * sizeof(float));
+        s->have_keys = true;
+    })
+}
+
+int cgh_make_plan(cgh_session* s, const uint32_t* card, const float* avg, cgbdsm_plan* out) {
+    CGH_TRY({
+        s->s->MakePlan(card, avg);
+        s->s->plan().Fill(out, s->s->ebits());
+    })
+}
+
+uint32_t cgh_num_batches(cgh_session* s) { return s->s->num_batches(); }
+uint32_t cgh_num_threads(cgh_session* s) { return (uint32_t)s->s->calig().Threads(); }
+
+int cgh_next_batch(cgh_session* s, cgh_batch* out) {
+    CGH_TRY({
+        if (!s->have_keys) throw cg::Error("cgh_initial_keys must run before the first batch");
+        if (s->next_batch >= s->s->num_batches()) throw cg::Error("no batches left");
+        uint32_t b = s->next_batch++;
+        cg::S1Timing t;
+        const cg::KeyBatch& kb = s->s->RunBatch(b, &t);
+        out->ins = kb.ins.data();
+        out->n_ins = kb.ins.size();
+        out->del = kb.del.data();
+        out->n_del = kb.del.size();
+        out->upd = kb.upd.data();
+        out->n_upd = kb.upd.size();
+        out->edge_ok = kb.edge_ok.data();
+        out->dv = kb.dv;
+        out->batch = b;
+        out->pairs = s->s->BatchPairs(b);
+        out->ms_update_index = t.update_index_ms;
+        out->ms_construct_update = t.construct_update_ms;
+        out->ms_prepare_all = t.prepare_all_ms;
+        out->ms_prepare_index = t.prepare_index_ms;
+    })
+}
+
+int cgh_old_ids(cgh_session* s, const uint32_t** ids, uint32_t* n) {
+    CGH_TRY({
+        *ids = s->s->encoder().OldIds().data();
+        *n = (uint32_t)s->s->encoder().OldIds().size();
+    })
+}
+
+int cgh_export_graph(cgh_session* s, uint64_t* offs, uint32_t* adj, uint64_t cap, int32_t* labels, uint32_t* n_out,
+                     uint64_t* m_out) {
+    CGH_TRY({
+        auto& c = s->s->calig();
+        uint32_t n = c.NumVertices();
+        uint64_t m = 0;
+        for (uint32_t v = 0; v < n; v++) m += c.Nei(v).size();
+        *n_out = n;
+        *m_out = m;
+        if (offs && adj && cap >= m) {
+            uint64_t p = 0;
+            for (uint32_t v = 0; v < n; v++) {
+                offs[v] = p;
+                for (int x : c.Nei(v)) adj[p++] = (uint32_t)x;
+                if (labels) labels[v] = c.Label(v);
+            }
+            offs[n] = p;
+        }
+    })
+}
+
+int cgh_export_li(cgh_session* s, uint32_t* li, uint32_t n) {
+    CGH_TRY({
+        auto& c = s->s->calig();
+        if (n != c.NumVertices()) throw cg::Error("size mismatch");
+        for (uint32_t v = 0; v < n; v++) li[v] = c.Li(v);
+    })
+}
+
+int cgh_export_cand(cgh_session* s, uint32_t v, uint32_t ui, uint32_t uj, uint32_t* out, uint32_t cap, int64_t* n_out) {
+    CGH_TRY({
+        const auto* set = s->s->calig().Cand(v, ui, uj);
+        if (!set) {
+            *n_out = -1;
+            return 0;
+        }
+        *n_out = (int64_t)set->size();
+        for (uint32_t i = 0; i < set->size() && i < cap; i++) out[i] = (uint32_t)(*set)[i];
+    })
+}
+
+int cgh_query_info(cgh_session* s, int32_t* labels, uint32_t* edges_uv, uint32_t* qv, uint32_t* qe) {
+    CGH_TRY({
+        const auto& q = s->s->query();
+        *qv = q.vcount;
+        *qe = q.ecount;
+        if (labels)
+            for (uint32_t u = 0; u < q.vcount; u++) labels[u] = q.raw_labels[u];
+        if (edges_uv)
+            for (uint32_t e = 0; e < q.ecount; e++) {
+                edges_uv[2 * e] = q.qe_list[e].first;
+                edges_uv[2 * e + 1] = q.qe_list[e].second;
+            }
+    })
+}
+
+}  // extern "C"

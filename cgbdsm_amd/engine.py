@@ -1,0 +1,162 @@
+"""Python handle on the MI355X Stage-2 engine (include/cgbdsm.h).
+
+`Engine` owns a device context; `DeviceQuery` mirrors the reference's
+GPUGraphLoader + CaLiGHelperGamma device surface
+(graph/graph_gpu.h:152-188, index/calig_helper_gamma.h:91-110):
+
+    q = eng.query(query_tables)                 # ctor + LoadQuery
+    card, avg = q.index_init(keys, dv)          # ConvertGlobalIndex
+    q.set_plan(plan)                            # LoadPlan
+    q.stage(batch)                              # staging of one batch
+    q.transfer_update_all()                     # TransferUpdateAll
+    q.transfer_update_index()                   # TransferUpdateIndex
+    q.build_local_index()                       # BuildLocalIndexBitAll
+    count, per_edge = q.matching()              # MatchingBitAll
+    q.batch_update_global_index()               # BatchUpdateGlobalIndex
+
+There is no host fallback: a missing libcgbdsm.so raises at import of the
+handle, and every non-zero status raises `CgbdsmError`.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+STATUS = {0: "OK", 1: "INVALID", 2: "OOM", 3: "DEVICE", 4: "LIMIT", 5: "OVERFLOW"}
+
+
+class CgbdsmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise CgbdsmError(rc, N.dev_lib().cgbdsm_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Engine:
+    def __init__(self, device=0, budget_bytes=0):
+        self.L = N.dev_lib()
+        self.h = C.c_void_p()
+        _check(self.L.cgbdsm_ctx_create(device, budget_bytes, C.byref(self.h)))
+
+    def query(self, tables):
+        return DeviceQuery(self, tables)
+
+    def close(self):
+        if self.h:
+            _check(self.L.cgbdsm_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceQuery:
+    def __init__(self, engine, tables):
+        self.eng, self.L = engine, engine.L
+        self.h = C.c_void_p()
+        self.tables = tables
+        self.qv, self.qe, self.ndir = tables.qv, tables.qe, tables.ndir
+        _check(self.L.cgbdsm_query_create(engine.h, C.byref(tables), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _check(self.L.cgbdsm_query_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_plan(self, plan):
+        _check(self.L.cgbdsm_query_set_plan(self.h, C.byref(plan)))
+        self.tables = plan
+
+    def index_init(self, keys, dv):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        card = np.zeros(N.MAX_DIR, dtype=np.uint32)
+        avg = np.zeros(N.MAX_DIR, dtype=np.float32)
+        _check(self.L.cgbdsm_index_init(self.h, _ptr(keys), keys.size, dv, card.ctypes.data, avg.ctypes.data))
+        return card[: self.ndir], avg[: self.ndir]
+
+    def stage(self, kb):
+        ins = np.ascontiguousarray(kb.ins, dtype=np.uint64)
+        dele = np.ascontiguousarray(kb.dele, dtype=np.uint64)
+        upd = np.ascontiguousarray(kb.upd, dtype=np.uint64)
+        eok = np.ascontiguousarray(kb.edge_ok, dtype=np.uint8)
+        _check(self.L.cgbdsm_batch_stage(self.h, _ptr(ins), ins.size, _ptr(dele), dele.size, _ptr(upd), upd.size,
+                                         eok.ctypes.data, kb.dv))
+
+    def transfer_update_all(self):
+        _check(self.L.cgbdsm_transfer_update_all(self.h))
+
+    def transfer_update_index(self):
+        ne = C.c_int()
+        _check(self.L.cgbdsm_transfer_update_index(self.h, C.byref(ne)))
+        return bool(ne.value)
+
+    def build_local_index(self):
+        _check(self.L.cgbdsm_build_local_index(self.h))
+
+    def matching(self):
+        n = C.c_ulonglong(0)
+        per = np.zeros(N.MAX_QE, dtype=np.uint64)
+        _check(self.L.cgbdsm_matching(self.h, C.byref(n), per.ctypes.data))
+        return int(n.value), per[: self.qe].copy()
+
+    def batch_update_global_index(self):
+        _check(self.L.cgbdsm_batch_update_global_index(self.h))
+
+    def run_batch(self):
+        """Whole Stage 2 of a staged batch; returns (count, per_edge, skipped)."""
+        n = C.c_ulonglong(0)
+        per = np.zeros(N.MAX_QE, dtype=np.uint64)
+        sk = C.c_int()
+        _check(self.L.cgbdsm_run_batch(self.h, C.byref(n), per.ctypes.data, C.byref(sk)))
+        return int(n.value), per[: self.qe].copy(), bool(sk.value)
+
+    def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=0):
+        _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
+
+    def export(self, which):
+        n = C.c_uint64()
+        _check(self.L.cgbdsm_index_export(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(self.L.cgbdsm_index_export(self.h, which, out.ctypes.data, n.value, C.byref(n)))
+        return out[: n.value]
+
+    def mask(self, ei, qv, dv):
+        words = (dv + 63) // 64
+        out = np.zeros(max(words, 1), dtype=np.uint64)
+        ones = C.c_int()
+        _check(self.L.cgbdsm_mask_export(self.h, ei, qv, out.ctypes.data, words, C.byref(ones)))
+        return bool(ones.value), out[:words]
+
+    def tuples(self, cap):
+        out = np.zeros(max(cap * self.qv, 1), dtype=np.uint32)
+        tei = np.zeros(max(cap, 1), dtype=np.uint8)
+        n = C.c_uint64()
+        _check(self.L.cgbdsm_match_tuples(self.h, out.ctypes.data, cap, C.byref(n), tei.ctypes.data))
+        k = min(n.value, cap)
+        return n.value, out[: k * self.qv].reshape(k, self.qv), tei[:k]
+
+    def stats(self):
+        s = N.Stats()
+        _check(self.L.cgbdsm_stats_get(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in N.Stats._fields_}
+
+    def sync(self):
+        _check(self.L.cgbdsm_sync(self.h))

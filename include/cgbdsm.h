@@ -1,0 +1,205 @@
+/*
+ * cgbdsm.h -- C ABI of the MI355X (gfx950) Stage-2 engine for continuous
+ * subgraph matching (CGBDSM's GPU stage).
+ *
+ * The host pipeline (Stage-1 CaLiG producer, planner, cgcsm_* CLI) calls the
+ * device only through these functions.  Arguments are plain pointers and sizes;
+ * every function returns a cgbdsm_status and never exits the process.
+ *
+ * Each entry point names the reference interface it replaces (file:line in the
+ * CGBDSM sources):
+ *   CaLiGHelperGamma  index/calig_helper_gamma.h:91-110
+ *   GPUGraphLoader    graph/graph_gpu.h:152-188
+ *
+ * Key format at the boundary (same as index/calig_helper_gamma.cu:264):
+ *   key = (dir_edge << (64 - ebits)) | (u << 32) | v
+ * with ebits = 5 (reference-compatible: at most 32 directed query edges,
+ * 27-bit owner ids) or ebits = 7 (wide: 84 directed edges, 25-bit owner ids).
+ */
+#ifndef CGBDSM_H
+#define CGBDSM_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time limits of the query model (utils/config.h:5-6). */
+#define CGBDSM_MAX_QV 12
+#define CGBDSM_MAX_QE 42
+#define CGBDSM_MAX_DIR (2 * CGBDSM_MAX_QE)
+#define CGBDSM_NO_EDGE 0xFFu
+
+typedef enum {
+    CGBDSM_OK = 0,
+    CGBDSM_ERR_INVALID = 1,   /* bad argument or call order               */
+    CGBDSM_ERR_OOM = 2,       /* device memory budget exceeded            */
+    CGBDSM_ERR_DEVICE = 3,    /* HIP runtime error                        */
+    CGBDSM_ERR_LIMIT = 4,     /* query or key exceeds an encoding limit   */
+    CGBDSM_ERR_OVERFLOW = 5   /* a bounded device queue or spin overflowed */
+} cgbdsm_status;
+
+/* Cartesian-product tags of the global order (graph/plan.h:43-49). */
+enum {
+    CGBDSM_CP_NONE = 0,
+    CGBDSM_CP_TREE_SINGLE = 1,
+    CGBDSM_CP_NONTREE_SINGLE = 2,
+    CGBDSM_CP_TREE_PRODUCT = 3,
+    CGBDSM_CP_NONTREE_PRODUCT = 4
+};
+
+/*
+ * Query model and plan tables: the POD form of QueryGraph (graph/graph.h:18-46),
+ * OrderPerEdge (graph/plan.h:12-18) and the Plan flags (graph/plan.h:50-59).
+ * These replace the __constant__ symbols of graph/graph_gpu.cu:35-46 and
+ * utils/globals.cu:8-16 with one per-query parameter block, so several
+ * queries can live in one process.
+ */
+typedef struct {
+    uint32_t qv;                        /* query vertices (3..12)              */
+    uint32_t qe;                        /* undirected query edges (2..42)      */
+    uint32_t ndir;                      /* directed query edges = 2*qe         */
+    uint32_t ebits;                     /* 5 (reference keys) or 7 (wide keys) */
+    uint8_t eidx[CGBDSM_MAX_QV * CGBDSM_MAX_QV];   /* [u*12+uu] -> dir edge or 0xFF */
+    uint8_t dir_to_edge[CGBDSM_MAX_DIR];           /* dir edge -> undirected edge   */
+    uint8_t qe_u[CGBDSM_MAX_QE];                   /* qe_list_[i].first  (u < uu)   */
+    uint8_t qe_v[CGBDSM_MAX_QE];                   /* qe_list_[i].second            */
+    /* global matching order (Plan::global_order_) */
+    uint8_t go_vs[CGBDSM_MAX_QV];
+    uint8_t go_bni_offs[CGBDSM_MAX_QV + 1];
+    uint8_t go_bni[CGBDSM_MAX_DIR];                /* positions, sorted by selectivity */
+    uint8_t go_cp[CGBDSM_MAX_QV];                  /* global_cartesian_product_info_   */
+    /* per-edge indexing orders (Plan::indexing_orders_) */
+    uint8_t io_vs[CGBDSM_MAX_QE][CGBDSM_MAX_QV];
+    uint8_t io_bni_offs[CGBDSM_MAX_QE][CGBDSM_MAX_QV + 1];
+    uint8_t io_bni[CGBDSM_MAX_QE][CGBDSM_MAX_DIR];
+    uint8_t rebuild_v[CGBDSM_MAX_QE][CGBDSM_MAX_QV];  /* Plan::rebuild_v_flags_ */
+} cgbdsm_plan;
+
+/* Per-batch device statistics (filled by cgbdsm_stats). */
+typedef struct {
+    uint64_t batches;
+    uint64_t dv_count;            /* compressed data vertices                   */
+    uint64_t main_entries;        /* live entries in the main index M           */
+    uint64_t pool_used;           /* u32 slots taken from the list pool         */
+    uint64_t pool_cap;
+    uint64_t last_ins, last_del, last_upd;   /* keys staged for the last batch  */
+    uint64_t last_rows_touched;   /* (edge,vertex) lists rewritten, last batch  */
+    uint64_t last_idx_bytes;      /* algorithmic index-maintenance bytes        */
+    uint64_t last_frontier_rows;  /* rows handed to the final DFS               */
+    uint64_t last_frontier_depth;
+    uint64_t last_matches;
+    uint64_t compactions;
+    double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
+} cgbdsm_stats;
+
+typedef struct cgbdsm_ctx cgbdsm_ctx;
+typedef struct cgbdsm_query cgbdsm_query;
+
+/* Most recent error text of the calling thread ("" if none). */
+const char* cgbdsm_last_error(void);
+
+/* Library build identification (gfx target, key widths). */
+const char* cgbdsm_version(void);
+
+/* Device context: one HIP device, one stream, a memory budget.
+ * Replaces cudaSetDevice + the MemPool of GPUGraphLoader's ctor
+ * (main_calig.cpp:64, graph/graph_gpu.cu:58-102). budget_bytes = 0: no cap. */
+cgbdsm_status cgbdsm_ctx_create(int device, uint64_t budget_bytes, cgbdsm_ctx** out);
+cgbdsm_status cgbdsm_ctx_destroy(cgbdsm_ctx* ctx);
+
+/* Register a query: GPUGraphLoader ctor + LoadQuery + AllocRelations
+ * (graph/graph_gpu.h:152-169, graph/graph_gpu.cu:156-197).  Only the query
+ * model fields of `plan` (qv, qe, ndir, ebits, eidx, dir_to_edge, qe_u, qe_v,
+ * io_*) are read here; the matching order arrives with cgbdsm_query_set_plan. */
+cgbdsm_status cgbdsm_query_create(cgbdsm_ctx* ctx, const cgbdsm_plan* plan, cgbdsm_query** out);
+cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q);
+
+/* Upload the global matching order, rebuild flags and cartesian tags:
+ * GPUGraphLoader::LoadPlan (graph/graph_gpu.h:159, graph/graph_gpu.cu:174-193). */
+cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan);
+
+/* Initial main index from host keys (any order, duplicates allowed):
+ * CaLiGHelperGamma::ConvertGlobalIndex device half
+ * (index/calig_helper_gamma.h:95, index/calig_helper_gamma.cu:238-360).
+ * Writes cardinalities[ndir] (entries per directed edge) and
+ * avg_degrees[ndir] (entries / non-empty lists) when non-NULL. */
+cgbdsm_status cgbdsm_index_init(cgbdsm_query* q, const uint64_t* keys, uint64_t n,
+                                uint32_t dv_count, uint32_t* cardinalities, float* avg_degrees);
+
+/* Hand one batch's host key vectors to the device (copied before return).
+ * Replaces the staging vectors el_host_stage_ / del_host_stage_ /
+ * el_update_host_stage_ and edge_ok_per_batch_ (index/calig_helper_gamma.h:60-74)
+ * plus AllocateFromMappingGPU (index/calig_helper_gamma.cu:107-130). */
+cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q,
+                                 const uint64_t* ins, uint64_t n_ins,
+                                 const uint64_t* del, uint64_t n_del,
+                                 const uint64_t* upd, uint64_t n_upd,
+                                 const uint8_t* edge_ok, uint32_t dv_count);
+
+/* M <- (M u Ins) \ Del: CaLiGHelperGamma::TransferUpdateAll
+ * (index/calig_helper_gamma.h:105, index/calig_helper_gamma.cu:485-517). */
+cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q);
+
+/* U <- sorted Upd lists: CaLiGHelperGamma::TransferUpdateIndex
+ * (index/calig_helper_gamma.h:106, index/calig_helper_gamma.cu:569-597).
+ * *nonempty = 0 when the batch has no update keys (the caller skips it,
+ * main_calig.cpp:301-302). */
+cgbdsm_status cgbdsm_transfer_update_index(cgbdsm_query* q, int* nonempty);
+
+/* Per-(query edge, query vertex) 1-bit support masks:
+ * GPUGraphLoader::BuildLocalIndexBitAll (graph/graph_gpu.h:173, graph/graph_gpu.cu:803-869). */
+cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q);
+
+/* Merged matching of all query edges: GPUGraphLoader::MatchingBitAll
+ * (graph/graph_gpu.h:180, graph/graph_gpu.cu:1261-1480).  Adds the batch's
+ * positive matches to *num_matches; per_edge[qe] (optional) receives the
+ * count attributed to each query edge (the 5-bit edge tag). */
+cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge);
+
+/* Post-match M <- M u Upd: CaLiGHelperGamma::BatchUpdateGlobalIndex
+ * (index/calig_helper_gamma.h:98, index/calig_helper_gamma.cu:539-567). */
+cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q);
+
+/* Matching knobs (no reference counterpart beyond utils/config.h:22):
+ * bfs_row_limit = MIN_NRESULTS_TO_GPU analogue (rows below which a BFS
+ * level is expanded before the final DFS); frontier_cap = capacity of the
+ * frontier buffer in rows (small values force the overflow fallback of
+ * graph/graph_gpu.cu:1323-1327); donate_min = pending candidates at which a
+ * DFS wavefront donates work to the global stealing queue (0 disables). */
+cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit,
+                                      uint64_t frontier_cap, uint32_t donate_min);
+
+/* Parity dumps.  which = 0: main index M, 1: update index U.
+ * Writes the index as sorted keys in the boundary format into out[cap];
+ * *n_out receives the total number of entries (also when cap is too small). */
+cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out);
+
+/* Support-mask words for (edge ei, query vertex qv): ceil(dv/64) u64 words,
+ * bit v%64 of word v/64.  *all_ones = 1 for tiles that are not rebuilt. */
+cgbdsm_status cgbdsm_mask_export(cgbdsm_query* q, uint32_t ei, uint32_t qv, uint64_t* words,
+                                 uint64_t nwords, int* all_ones);
+
+/* Full embeddings of the last cgbdsm_matching call, as rows of qv ids in
+ * global-order positions, ei tag stripped (small inputs only: enumerates on
+ * the device into out[cap*qv]).  *n_out = number of embeddings. */
+cgbdsm_status cgbdsm_match_tuples(cgbdsm_query* q, uint32_t* out, uint64_t cap, uint64_t* n_out,
+                                  uint8_t* ei_out);
+
+cgbdsm_status cgbdsm_stats_get(cgbdsm_query* q, cgbdsm_stats* out);
+
+/* Device-side timing of the hot path of one batch already staged
+ * (apply + update index + masks + matching + commit), returning per-stage
+ * kernel time measured with HIP events on the query's stream. */
+cgbdsm_status cgbdsm_run_batch(cgbdsm_query* q, unsigned long long* num_matches,
+                               uint64_t* per_edge, int* skipped);
+
+/* Synchronise the query's stream. */
+cgbdsm_status cgbdsm_sync(cgbdsm_query* q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CGBDSM_H */

@@ -1,0 +1,105 @@
+"""Device parity: the gfx950 engine against the CPU oracle, batch by batch,
+through the C ABI (bit-exact index, masks, counts, per-edge split, tuples)."""
+import numpy as np
+import pytest
+
+from cgbdsm_amd.engine import CgbdsmError, Engine
+from cgbdsm_amd.host import HostSession, synth_spec
+from tests import oracle_lib as O
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+# (name, scale, avg_degree, labels, zipf, query size, kind, seed, batch edges, ebits)
+STREAMS = [
+    ("cfg1-s12-tree5", 12, 8, 8, 0.0, 5, "tree", 1, None, 5),
+    ("s11-dense5-b300", 11, 8, 4, 0.0, 5, "dense", 2, 300, 5),
+    ("s11-sparse6-b250", 11, 8, 4, 0.0, 6, "sparse", 3, 250, 5),
+    ("s12-zipf-dense6-wide", 12, 12, 6, 1.2, 6, "dense", 4, 500, 7),
+    ("s10-tree7-b64", 10, 8, 3, 0.0, 7, "tree", 5, 64, 5),
+    ("s11-dense7-b400", 11, 10, 3, 0.0, 7, "dense", 6, 400, 5),
+]
+
+
+@pytest.mark.parametrize("case", STREAMS, ids=[c[0] for c in STREAMS])
+def test_stream_parity(eng, case):
+    name, scale, deg, labels, zipf, qs, kind, seed, be, ebits = case
+    print(f"case {name}", flush=True)
+    spec = synth_spec(scale, deg, labels, zipf_s=zipf, seed=seed, query_size=qs, query_kind=kind, query_seed=seed)
+    host = HostSession(spec=spec, batch_edges=be, threads=4, ebits=ebits)
+    res = parity.run_stream(host, eng, log=_log)
+    assert res["batches"] > 0
+    assert res["mismatches"] == 0
+
+
+def test_config1_full_size(eng):
+    """R-MAT scale 14, degree 8, 8 uniform labels, seed 1, 90/10 split, one
+    batch, 5-vertex random-walk BFS-tree query; count also against the
+    closed form emb(G_1) - emb(G_0)."""
+    print("case config1", flush=True)
+    spec = synth_spec(14, 8, 8, seed=1, query_size=5, query_kind="tree", query_seed=1)
+    host = HostSession(spec=spec, batch_edges=None, threads=8)
+    offs, adj, lab = host.graph()
+    ql, qe = host.query_info()
+    before = O.brute_count(offs, adj, lab, ql, qe)
+    res = parity.run_stream(host, eng, check_masks=True, log=_log)
+    offs, adj, lab = host.graph()
+    after = O.brute_count(offs, adj, lab, ql, qe)
+    assert res["mismatches"] == 0
+    assert res["total"] == after - before
+
+
+def test_tuples_parity(eng):
+    print("case tuples", flush=True)
+    spec = synth_spec(9, 6, 3, seed=7, query_size=4, query_kind="dense", query_seed=7)
+    host = HostSession(spec=spec, batch_edges=150, threads=4)
+    res = parity.run_stream(host, eng, tuples_cap=1 << 18, log=_log)
+    assert res["mismatches"] == 0
+
+
+@pytest.mark.parametrize("params", [
+    {"bfs_row_limit": 0},                          # final DFS straight from the seeds
+    {"bfs_row_limit": 1 << 40},                    # BFS down to the last level
+    {"bfs_row_limit": 1 << 40, "frontier_cap": 32},  # overflow -> DFS from the last good level
+], ids=["dfs-only", "bfs-deep", "frontier-overflow"])
+def test_matching_paths(eng, params):
+    print(f"case paths {params}", flush=True)
+    spec = synth_spec(11, 8, 4, seed=8, query_size=6, query_kind="dense", query_seed=8)
+    host = HostSession(spec=spec, batch_edges=300, threads=4)
+    res = parity.run_stream(host, eng, match_params=params, check_index=False, check_masks=False, log=_log)
+    assert res["mismatches"] == 0
+
+
+def test_rejects_out_of_range_keys(eng):
+    spec = synth_spec(8, 4, 2, seed=9, query_size=4, query_kind="tree", query_seed=9)
+    host = HostSession(spec=spec, batch_edges=None, threads=2)
+    keys, dv, card, avg = host.initial_keys()
+    q = eng.query(host.query_tables)
+    bad = np.array([(np.uint64(0) << np.uint64(59)) | (np.uint64(dv + 5) << np.uint64(32))], dtype=np.uint64)
+    with pytest.raises(CgbdsmError) as ei:
+        q.index_init(bad, dv)
+    assert ei.value.code == 1
+    q.close()
+
+
+def test_call_order_is_checked(eng):
+    spec = synth_spec(8, 4, 2, seed=10, query_size=4, query_kind="tree", query_seed=10)
+    host = HostSession(spec=spec, batch_edges=None, threads=2)
+    keys, dv, card, avg = host.initial_keys()
+    q = eng.query(host.query_tables)
+    q.index_init(keys, dv)
+    with pytest.raises(CgbdsmError):
+        q.transfer_update_all()  # nothing staged
+    q.close()
