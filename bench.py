@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Stage-2 hot-path benchmark on one MI355X.
+
+Workload: labelled R-MAT graph (a,b,c,d = .57,.19,.19,.05), scale 20,
+average degree 16, 16 vertex labels drawn Zipf(1.2), seed 1; a seeded
+shuffle keeps 90% of the edges as the initial graph and streams the rest in
+batches of 100,000 edges; one dense query from a seeded random walk (induced
+subgraph, average query degree >= 3), 6 vertices by default (7-vertex dense
+random-walk queries on this graph have more than 1e11 new embeddings per
+batch, beyond exact enumeration).
+
+A step is one batch through the device hot path: TransferUpdateAll ->
+TransferUpdateIndex -> BuildLocalIndexBitAll -> MatchingBitAll ->
+BatchUpdateGlobalIndex.  Stage 1 (CaLiG on the host) runs for every batch
+before timing and its keys are copied to HBM before the timed region
+(cgbdsm_batch_enqueue), so `value` is device throughput with resident inputs.
+
+The JSON line also carries:
+  roofline      the dominant kernel class: algorithmic bytes per launch over
+                its mean launch time (HIP events on the query's stream)
+  cpu_baseline  the CPU oracle (restated reference CPU path) timed on a
+                bounded sample of the same batches, on this host
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "update edges/s + p99 batch latency, served multi-query vs plain; exact counts"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = (len(xs) - 1) * p / 100.0
+    lo, hi = int(k), min(int(k) + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def cpu_baseline(keys, dv, plan, batches, budget_s, threads):
+    """Oracle Stage 2 (set-semantics index algebra, masks, seeding + DFS) on
+    the first batches of the stream until `budget_s` seconds of CPU work."""
+    from tests import oracle_lib as O
+
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    edges, secs, counts = 0, 0.0, []
+    for kb in batches:
+        t0 = time.perf_counter()
+        idx.set_dv(kb.dv)
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        idx.rebuild_update(kb.upd)
+        if kb.upd.size:
+            masks = idx.masks(plan)
+            cnt, _ = idx.match(plan, kb.edge_ok, masks=masks, threads=threads)
+            idx.insert(kb.upd)
+        else:
+            cnt = 0
+        secs += time.perf_counter() - t0
+        edges += kb.pairs
+        counts.append(int(cnt))
+        if secs >= budget_s:
+            break
+    return edges, secs, counts
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--avg-degree", type=float, default=16)
+    ap.add_argument("--labels", type=int, default=16)
+    ap.add_argument("--zipf", type=float, default=1.2)
+    ap.add_argument("--batch-edges", type=int, default=100_000)
+    ap.add_argument("--query-size", type=int, default=6)
+    ap.add_argument("--query-kind", default="dense", choices=["dense", "tree", "sparse"])
+    ap.add_argument("--query-seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--deadline-ms", type=float, default=60_000)
+    ap.add_argument("--donate-min", type=int, default=4, help="survivors left before a DFS wavefront donates (0: off)")
+    ap.add_argument("--bfs-row-limit", type=int, default=65536)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: {max(args.gpus, world)} GPUs requested; this build measures one GPU only "
+              "(the path would shard by frontier rows with one u64 all-reduce per batch; no multi-GPU run "
+              "is implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import HostSession, synth_spec
+
+    threads = min(16, os.cpu_count() or 1)
+    t_setup = time.perf_counter()
+    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1, initial_frac=0.9,
+                      query_size=args.query_size, query_kind=args.query_kind, query_seed=args.query_seed)
+    host = HostSession(spec=spec, batch_edges=args.batch_edges, threads=threads)
+    keys, dv, card, avg = host.initial_keys()
+    plan = host.make_plan(card, avg)
+    qlab, qedges = host.query_info()
+    log(f"graph + CaLiG ready in {time.perf_counter() - t_setup:.1f}s: {keys.size} initial index entries, "
+        f"dv {dv}, query labels {qlab.tolist()} edges {qedges.reshape(-1, 2).tolist()}, {host.num_batches} batches")
+
+    # Stage 1 for every batch (host CPU), before timing.
+    batches, s1_ms = [], []
+    for _ in range(host.num_batches):
+        kb = host.next_batch()
+        batches.append(kb)
+        s1_ms.append(sum(kb.s1_ms.values()))
+    warm = max(0, min(args.warmup, len(batches) - 1))
+    steps = max(1, min(args.steps, len(batches) - warm))
+    if steps != args.steps:
+        log(f"stream has {len(batches)} batches: timing {steps} steps after {warm} warmup")
+
+    eng = Engine(0)
+    q = eng.query(host.query_tables)
+    q.index_init(keys, dv)
+    q.set_plan(plan)
+    q.set_deadline(args.deadline_ms)
+    q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
+    for kb in batches[: warm + steps]:
+        q.enqueue(kb)
+
+    counts, lat_ms = [], []
+    for i in range(warm):
+        q.stage_next()
+        c, _, _ = q.run_batch()
+        counts.append(c)
+    q.profile(True)
+    q.kernel_times(reset=True)
+    q.sync()
+    t0 = time.perf_counter()
+    stage_ms = {"apply": 0.0, "update_index": 0.0, "masks": 0.0, "match": 0.0, "commit": 0.0}
+    for i in range(steps):
+        tb = time.perf_counter()
+        q.stage_next()
+        c, _, _ = q.run_batch()
+        lat_ms.append(1e3 * (time.perf_counter() - tb))
+        counts.append(c)
+        st = q.stats()
+        for k in stage_ms:
+            stage_ms[k] += st["ms_" + k]
+    q.sync()
+    elapsed = time.perf_counter() - t0
+    kt = q.kernel_times()
+    timed = batches[warm: warm + steps]
+    edges = sum(kb.pairs for kb in timed)
+    matches = sum(counts[warm:])
+    value = edges / elapsed
+
+    # Dominant kernel class by summed device time.
+    dom = max(kt, key=lambda k: kt[k]["ms"])
+    d = kt[dom]
+    launch_ms = d["ms"] / max(d["launches"], 1)
+    bytes_per_launch = d["bytes"] / max(d["launches"], 1)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom,
+                "launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        c_edges, c_secs, c_counts = cpu_baseline(keys, dv, plan, batches, args.cpu_seconds, threads)
+        gpu_counts = counts[: len(c_counts)]
+        cpu = {"value": round(c_edges / c_secs, 3) if c_secs > 0 else None, "unit": "update edges/s",
+               "cores": threads, "kind": "port",
+               "sample": f"first {len(c_counts)} of {len(batches)} batches of this stream (oracle Stage 2: index "
+                         f"merges, masks, seeding + DFS with {threads} threads), {c_secs:.1f}s",
+               "counts_match_gpu": gpu_counts == c_counts}
+        if gpu_counts != c_counts:
+            log(f"COUNT MISMATCH gpu {gpu_counts} cpu {c_counts}")
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "update edges/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(1e3 * elapsed / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g} "
+                               f"90/10 stream, {args.batch_edges}-edge batches, plain single query: "
+                               f"{args.query_size}-vertex {args.query_kind} random-walk query seed {args.query_seed}",
+                   "batch_edges": args.batch_edges, "query_edges": int(qedges.size // 2),
+                   "parallelism": "single GPU"},
+        "p50_batch_ms": round(percentile(lat_ms, 50), 4), "p99_batch_ms": round(percentile(lat_ms, 99), 4),
+        "matches": int(matches), "matches_per_s": round(matches / elapsed, 3),
+        "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
+        "s1_cpu_ms_per_batch": round(statistics.mean(s1_ms), 3),
+        "kernels_ms": {k: round(v["ms"], 4) for k, v in kt.items()},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,14 @@ class Stats(C.Structure):
         (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit")]
 
 
+class KTime(C.Structure):
+    """cgbdsm_ktime (include/cgbdsm.h)."""
+    _fields_ = [("ms", C.c_double), ("launches", C.c_uint64), ("bytes", C.c_uint64), ("items", C.c_uint64)]
+
+
+KERNEL_CLASSES = ["sort", "merge_insert", "merge_remove", "update_index", "masks", "seed", "extend", "dfs"]
+
+
 class SynthSpec(C.Structure):
     """cgh_synth_spec (include/cgbdsm_host.h)."""
     _fields_ = [
@@ -130,6 +138,11 @@ def dev_lib():
         L.cgbdsm_stats_get.argtypes = [P, C.POINTER(Stats)]
         L.cgbdsm_run_batch.argtypes = [P, C.POINTER(C.c_ulonglong), C.c_void_p, C.POINTER(I)]
         L.cgbdsm_sync.argtypes = [P]
+        L.cgbdsm_set_deadline_ms.argtypes = [P, C.c_double]
+        L.cgbdsm_profile.argtypes = [P, I]
+        L.cgbdsm_kernel_times.argtypes = [P, C.c_void_p, I]
+        L.cgbdsm_batch_enqueue.argtypes = [P, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U32]
+        L.cgbdsm_batch_stage_next.argtypes = [P, C.POINTER(I)]
         _dev = L
     return _dev
 
@@ -140,5 +153,6 @@ DEV_SYMBOLS = [
     "cgbdsm_batch_stage", "cgbdsm_transfer_update_all", "cgbdsm_transfer_update_index",
     "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_batch_update_global_index",
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
-    "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync",
+    "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_profile",
+    "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
 ]

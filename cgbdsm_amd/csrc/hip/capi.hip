@@ -2,9 +2,13 @@
 // reference member function it replaces; all device work of a query runs on
 // that query's own HIP stream, and no entry point exits the process.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <deque>
 #include <new>
 
 #include "engine.h"
@@ -46,13 +50,36 @@ struct cgbdsm_query {
     uint64_t n_ins = 0, n_del = 0, n_upd = 0, u_rows_upper = 0;
     uint8_t edge_ok[kMaxQE] = {};
     uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
-    uint32_t donate_min = 0;
+    uint32_t donate_min = 4;
+    double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
+    StealItem* steal_items = nullptr;
+    unsigned long long* steal_seq = nullptr;
+    uint64_t steal_cap = 1ull << 20;
+    uint32_t* h_abort = nullptr;  // coherent pinned host memory, mapped
+    uint32_t* d_abort = nullptr;
     int phase = 0;  // 0 idle, 1 staged, 2 applied, 3 U built, 4 masks, 5 matched
     bool u_nonempty = false;
     uint32_t last_depth = 0, last_buf = 0;
     uint64_t last_rows = 0;
     cgbdsm_stats st{};
     hipEvent_t ev[8] = {};
+    // kernel-class timing (events on this query's stream)
+    bool profile = false;
+    struct KPending {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<KPending> kpend;
+    std::vector<hipEvent_t> ev_pool;
+    cgbdsm_ktime kt[CGBDSM_K_COUNT] = {};
+    uint64_t batch_idx_bytes = 0;
+    struct Resident {
+        DBuf<uint64_t> ins, del, upd;
+        uint64_t n_ins = 0, n_del = 0, n_upd = 0;
+        uint8_t edge_ok[kMaxQE] = {};
+        uint32_t dv = 0;
+    };
+    std::deque<Resident> fifo;
 };
 
 namespace {
@@ -196,9 +223,16 @@ void CheckKeys(cgbdsm_query* q, const uint64_t* k, uint64_t n, uint32_t dv, cons
 
 // Sort boundary keys `src` (n) into internal order in `dst` (dst may be any
 // buffer of n u64; k_alt is used as the ping-pong partner unless it is dst).
+int KBegin(cgbdsm_query* q, int cls);
+void KEnd(cgbdsm_query* q, int idx);
+void KAdd(cgbdsm_query* q, int cls, uint64_t bytes, uint64_t items);
+
 void DecodeSort(cgbdsm_query* q, const uint64_t* src, uint64_t* dst, uint64_t* alt, uint64_t n) {
+    const int k = KBegin(q, CGBDSM_K_SORT);
     LaunchDecode(src, dst, n, q->ushift, q->umask, q->ndir, q->vbits, q->s);
     RadixSortU64(dst, alt, n, q->vbits + q->rbits, q->rs_hist.p, q->s);
+    KEnd(q, k);
+    KAdd(q, CGBDSM_K_SORT, 0, n);
 }
 
 void GrowPool(cgbdsm_query* q, uint64_t min_cap) {
@@ -219,9 +253,14 @@ void GrowPool(cgbdsm_query* q, uint64_t min_cap) {
 void InsertSorted(cgbdsm_query* q, const uint64_t* keys, const unsigned long long* n_segs, uint64_t n_upper) {
     for (int attempt = 0;; attempt++) {
         SetCounter(q, &Counters::flags, 0);
+        SetCounter(q, &Counters::bytes, 0);
+        const int k = KBegin(q, CGBDSM_K_MERGE_INSERT);
         LaunchMergeInsert(MakeTables(q), keys, q->vbits, q->seg_row.p, q->seg_start.p, n_segs, n_upper, q->sc_val.p,
                           q->sc_pos.p, q->pool.cap, q->d_cnt, q->s);
+        KEnd(q, k);
         ReadCounters(q);
+        q->batch_idx_bytes += q->h_cnt->bytes;
+        KAdd(q, CGBDSM_K_MERGE_INSERT, q->h_cnt->bytes, n_upper);
         if (!(q->h_cnt->flags & kFlagPool)) return;
         if (attempt >= 4) Throw(CGBDSM_ERR_OOM, "list pool overflow after growth");
         GrowPool(q, q->pool.cap * 2);
@@ -290,6 +329,108 @@ EiList ActiveEdges(cgbdsm_query* q) {
     return L;
 }
 
+hipEvent_t PoolEvent(cgbdsm_query* q) {
+    if (!q->ev_pool.empty()) {
+        hipEvent_t e = q->ev_pool.back();
+        q->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    CGD_HIP(hipEventCreate(&e));
+    return e;
+}
+
+int KBegin(cgbdsm_query* q, int cls) {
+    if (!q->profile) return -1;
+    cgbdsm_query::KPending k{cls, PoolEvent(q), PoolEvent(q)};
+    CGD_HIP(hipEventRecord(k.a, q->s));
+    q->kpend.push_back(k);
+    return (int)q->kpend.size() - 1;
+}
+
+void KEnd(cgbdsm_query* q, int idx) {
+    if (idx < 0) return;
+    CGD_HIP(hipEventRecord(q->kpend[idx].b, q->s));
+}
+
+void KAdd(cgbdsm_query* q, int cls, uint64_t bytes, uint64_t items) {
+    if (!q->profile) return;
+    q->kt[cls].bytes += bytes;
+    q->kt[cls].items += items;
+}
+
+void KHarvest(cgbdsm_query* q) {
+    if (q->kpend.empty()) return;
+    CGD_HIP(hipStreamSynchronize(q->s));
+    for (auto& k : q->kpend) {
+        float ms = 0;
+        CGD_HIP(hipEventElapsedTime(&ms, k.a, k.b));
+        q->kt[k.cls].ms += ms;
+        q->kt[k.cls].launches++;
+        q->ev_pool.push_back(k.a);
+        q->ev_pool.push_back(k.b);
+    }
+    q->kpend.clear();
+}
+
+StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
+    StealQueue Q;
+    Q.items = q->steal_items;
+    Q.seq = q->steal_seq;
+    Q.cap = q->steal_cap;
+    Q.donate_min = donate ? q->donate_min : 0;
+    Q.abort = q->d_abort;
+    return Q;
+}
+
+// Final DFS over the frontier; waits for it while honouring the deadline:
+// past the deadline the host raises the abort flag the kernel polls, and the
+// batch fails with CGBDSM_ERR_OVERFLOW (its count would be incomplete).
+void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
+    FillU64((uint64_t*)&q->d_cnt->q_rows.v, 0, 1, q->s);
+    FillU64((uint64_t*)&q->d_cnt->q_pending.v, n, 1, q->s);
+    FillU64((uint64_t*)&q->d_cnt->q_idle.v, 0, 1, q->s);
+    *(volatile uint32_t*)q->h_abort = 0;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    if (!n) return;
+    const int k = tuples ? -1 : KBegin(q, CGBDSM_K_DFS);
+    LaunchDfs(MakeTables(q), rows, n, depth, q->d_cnt, MakeSteal(q, tuples == nullptr), tuples, tup_cap,
+              q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+    KEnd(q, k);
+    CGD_HIP(hipGetLastError());
+    if (q->deadline_ms > 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            hipError_t e = hipStreamQuery(q->s);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) CGD_HIP(e);
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (ms > q->deadline_ms) {
+                *(volatile uint32_t*)q->h_abort = 1;
+                std::atomic_thread_fence(std::memory_order_seq_cst);
+                CGD_HIP(hipStreamSynchronize(q->s));
+                *(volatile uint32_t*)q->h_abort = 0;
+                ReadCounters(q);
+                unsigned long long part = 0;
+                for (uint32_t e = 0; e < q->qe; e++) part += q->h_cnt->per_edge[e];
+                Throw(CGBDSM_ERR_OVERFLOW, "final DFS exceeded the deadline of " + std::to_string(q->deadline_ms) +
+                                               " ms; count incomplete (partial " + std::to_string(part) +
+                                               " matches, " + std::to_string(q->h_cnt->scanned) + " candidates, " +
+                                               std::to_string(q->h_cnt->probes) + " probes, " + std::to_string(n) +
+                                               " rows at depth " + std::to_string(depth) + ", " +
+                                               std::to_string(q->h_cnt->q_rows.v) + " rows taken, " +
+                                               std::to_string(q->h_cnt->donated) + " donated; pending " +
+                                               std::to_string(q->h_cnt->q_pending.v) + ", idle " +
+                                               std::to_string(q->h_cnt->q_idle.v) + ", queue " +
+                                               std::to_string(q->h_cnt->q_head.v) + "/" +
+                                               std::to_string(q->h_cnt->q_tail.v) + ", flags " +
+                                               std::to_string(q->h_cnt->flags) + ")");
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+    }
+}
+
 float Elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, a, b);
@@ -346,6 +487,12 @@ cgbdsm_status cgbdsm_query_create(cgbdsm_ctx* ctx, const cgbdsm_plan* plan, cgbd
             q->d_cnt = (Counters*)DevAlloc(ctx, sizeof(Counters));
             CGD_HIP(hipHostMalloc((void**)&q->h_cnt, sizeof(Counters), hipHostMallocDefault));
             memset(q->h_cnt, 0, sizeof(Counters));
+            CGD_HIP(hipHostMalloc((void**)&q->h_abort, 64, hipHostMallocMapped | hipHostMallocCoherent));
+            *(volatile uint32_t*)q->h_abort = 0;
+            CGD_HIP(hipHostGetDevicePointer((void**)&q->d_abort, q->h_abort, 0));
+            q->steal_items = (StealItem*)DevAlloc(ctx, q->steal_cap * sizeof(StealItem));
+            q->steal_seq = (unsigned long long*)DevAlloc(ctx, q->steal_cap * sizeof(unsigned long long));
+            FillU64((uint64_t*)q->steal_seq, 0, q->steal_cap, q->s);
             FillU64((uint64_t*)q->d_cnt, 0, sizeof(Counters) / 8, q->s);
             q->hplan.p = *plan;
             q->hplan.tag_shift = q->qe <= 32 ? 27 : 26;
@@ -377,11 +524,24 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
                         &q->front[0], &q->front[1]})
             Release(q, *b);
         Release(q, q->scan_scratch);
+        for (auto& r : q->fifo) {
+            Release(q, r.ins);
+            Release(q, r.del);
+            Release(q, r.upd);
+        }
         if (q->d_plan) DevFree(q->ctx, q->d_plan, sizeof(DevPlan));
         if (q->d_cnt) DevFree(q->ctx, q->d_cnt, sizeof(Counters));
         if (q->h_cnt) (void)hipHostFree(q->h_cnt);
+        if (q->h_abort) (void)hipHostFree(q->h_abort);
+        if (q->steal_items) DevFree(q->ctx, q->steal_items, q->steal_cap * sizeof(StealItem));
+        if (q->steal_seq) DevFree(q->ctx, q->steal_seq, q->steal_cap * sizeof(unsigned long long));
         for (auto& e : q->ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& k : q->kpend) {
+            (void)hipEventDestroy(k.a);
+            (void)hipEventDestroy(k.b);
+        }
+        for (auto& e : q->ev_pool) (void)hipEventDestroy(e);
         if (q->s) (void)hipStreamDestroy(q->s);
         delete q;
     });
@@ -487,11 +647,81 @@ cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q, const uint64_t* ins, uint64_t 
     });
 }
 
+cgbdsm_status cgbdsm_batch_enqueue(cgbdsm_query* q, const uint64_t* ins, uint64_t n_ins, const uint64_t* del,
+                                   uint64_t n_del, const uint64_t* upd, uint64_t n_upd, const uint8_t* edge_ok,
+                                   uint32_t dv_count) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (!edge_ok) Throw(CGBDSM_ERR_INVALID, "null edge_ok");
+        const uint32_t prev = q->fifo.empty() ? q->dv : q->fifo.back().dv;
+        if (dv_count < prev) Throw(CGBDSM_ERR_INVALID, "compressed vertex count cannot shrink");
+        if (dv_count > (1u << (32 - q->ebits)) || dv_count > q->hplan.v_mask)
+            Throw(CGBDSM_ERR_LIMIT, "compressed vertex count exceeds the key / row-tag width");
+        CheckKeys(q, ins, n_ins, dv_count, "ins");
+        CheckKeys(q, del, n_del, dv_count, "del");
+        CheckKeys(q, upd, n_upd, dv_count, "upd");
+        cgbdsm_query::Resident r;
+        try {
+            Ensure(q, r.ins, std::max<uint64_t>(n_ins, 1));
+            Ensure(q, r.del, std::max<uint64_t>(n_del, 1));
+            Ensure(q, r.upd, std::max<uint64_t>(n_upd, 1));
+            if (n_ins) CGD_HIP(hipMemcpyAsync(r.ins.p, ins, n_ins * 8, hipMemcpyHostToDevice, q->s));
+            if (n_del) CGD_HIP(hipMemcpyAsync(r.del.p, del, n_del * 8, hipMemcpyHostToDevice, q->s));
+            if (n_upd) CGD_HIP(hipMemcpyAsync(r.upd.p, upd, n_upd * 8, hipMemcpyHostToDevice, q->s));
+            CGD_HIP(hipStreamSynchronize(q->s));
+        } catch (...) {
+            Release(q, r.ins);
+            Release(q, r.del);
+            Release(q, r.upd);
+            throw;
+        }
+        r.n_ins = n_ins;
+        r.n_del = n_del;
+        r.n_upd = n_upd;
+        memcpy(r.edge_ok, edge_ok, q->qe);
+        r.dv = dv_count;
+        q->fifo.push_back(r);
+    });
+}
+
+cgbdsm_status cgbdsm_batch_stage_next(cgbdsm_query* q, int* staged) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (!q->have_order) Throw(CGBDSM_ERR_INVALID, "cgbdsm_query_set_plan must run before the first batch");
+        if (staged) *staged = 0;
+        if (q->fifo.empty()) return;
+        cgbdsm_query::Resident& r = q->fifo.front();
+        GrowDV(q, r.dv);
+        const uint64_t nmax = std::max<uint64_t>({r.n_ins, r.n_del, r.n_upd, 1});
+        Ensure(q, q->k_usort, std::max<uint64_t>(r.n_upd, 1));
+        Ensure(q, q->u_vals, std::max<uint64_t>(r.n_upd, 1));
+        Ensure(q, q->u_rows, std::max<uint64_t>(r.n_upd, 1), true, q->u_rows_upper);
+        EnsureKeyScratch(q, nmax);
+        std::swap(q->k_ins, r.ins);
+        std::swap(q->k_del, r.del);
+        std::swap(q->k_upd, r.upd);
+        q->n_ins = r.n_ins;
+        q->n_del = r.n_del;
+        q->n_upd = r.n_upd;
+        memcpy(q->edge_ok, r.edge_ok, q->qe);
+        q->st.last_ins = r.n_ins;
+        q->st.last_del = r.n_del;
+        q->st.last_upd = r.n_upd;
+        Release(q, r.ins);
+        Release(q, r.del);
+        Release(q, r.upd);
+        q->fifo.pop_front();
+        q->phase = 1;
+        if (staged) *staged = 1;
+    });
+}
+
 cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
     return Guard([&] {
         if (!q || q->phase != 1) Throw(CGBDSM_ERR_INVALID, "transfer_update_all needs a staged batch");
         SetCounter(q, &Counters::bytes, 0);
         SetCounter(q, &Counters::rows_touched, 0);
+        q->batch_idx_bytes = 0;
         if (q->n_ins) {
             DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, q->n_ins);
             LaunchUnique(q->k_alt.p, q->k_ins.p, q->n_ins, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
@@ -505,9 +735,16 @@ cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
             SetCounter(q, &Counters::n_unique, q->n_del);
             LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_del, q->vbits, q->flags.p, q->seg_row.p,
                            q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+            SetCounter(q, &Counters::bytes, 0);
+            const int k = KBegin(q, CGBDSM_K_MERGE_REMOVE);
             LaunchMergeRemove(MakeTables(q), q->k_alt.p, q->vbits, q->seg_row.p, q->seg_start.p,
                               Cnt(q, &Counters::n_segs), q->n_del, q->d_cnt, q->s);
+            KEnd(q, k);
+            ReadCounters(q);
+            q->batch_idx_bytes += q->h_cnt->bytes;
+            KAdd(q, CGBDSM_K_MERGE_REMOVE, q->h_cnt->bytes, q->n_del);
         }
+        q->st.last_idx_bytes = q->batch_idx_bytes;
         q->phase = 2;
     });
 }
@@ -522,10 +759,13 @@ cgbdsm_status cgbdsm_transfer_update_index(cgbdsm_query* q, int* nonempty) {
         if (q->n_upd) {
             DecodeSort(q, q->k_upd.p, q->k_usort.p, q->k_alt.p, q->n_upd);
             SetCounter(q, &Counters::n_unique, q->n_upd);
+            const int k = KBegin(q, CGBDSM_K_UPDATE_INDEX);
             LaunchSegments(q->k_usort.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
                            q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
             LaunchBuildU(q->u_rec.p, q->k_usort.p, q->n_upd, q->vbits, q->u_vals.p, q->seg_row.p, q->seg_start.p,
                          Cnt(q, &Counters::n_segs), q->u_rows.p, Cnt(q, &Counters::u_rows), q->s);
+            KEnd(q, k);
+            KAdd(q, CGBDSM_K_UPDATE_INDEX, 12ull * q->n_upd, q->n_upd);
             q->u_rows_upper = q->n_upd;
         }
         if (nonempty) *nonempty = q->u_nonempty ? 1 : 0;
@@ -539,6 +779,7 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q) {
         const EiList L = ActiveEdges(q);
         Ensure(q, q->masks, std::max<uint64_t>((uint64_t)q->qe * kMaxQV * q->W, 1));
         const Tables T = MakeTables(q);
+        const int kt = KBegin(q, CGBDSM_K_MASKS);
         LaunchMaskClear(q->masks.p, (uint64_t)kMaxQV * q->W, L, q->s);
         EiSeedInfo info{};
         for (uint32_t k = 0; k < L.n; k++) {
@@ -560,6 +801,7 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q) {
             }
             LaunchMaskDepth(T, d, D, q->s);
         }
+        KEnd(q, kt);
         q->phase = 4;
     });
 }
@@ -574,21 +816,30 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit, u
     });
 }
 
+cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
+    return Guard([&] {
+        if (!q || ms < 0) Throw(CGBDSM_ERR_INVALID, "bad deadline");
+        q->deadline_ms = ms;
+    });
+}
+
 cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge) {
     return Guard([&] {
         if (!q || q->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
         const EiList L = ActiveEdges(q);
-        for (auto f : {&Counters::rows_out, &Counters::row_next, &Counters::flags, &Counters::tuples})
+        for (auto f : {&Counters::rows_out, &Counters::flags, &Counters::tuples})
             SetCounter(q, f, 0);
         FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
         const Tables T = MakeTables(q);
         const bool fixed_cap = q->frontier_cap_rows != 0;
-        uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : std::max<uint64_t>(q->bfs_row_limit * 4, 1u << 20);
+        uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
         Ensure(q, q->front[0], cap_rows * 2);
         // Step 1: seeds (retry once with a buffer of the exact size on overflow).
         uint64_t n = 0;
         for (int attempt = 0;; attempt++) {
+            const int ks = KBegin(q, CGBDSM_K_SEED);
             LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
+            KEnd(q, ks);
             ReadCounters(q);
             n = q->h_cnt->rows_out;
             if (!(q->h_cnt->flags & kFlagFrontier)) break;
@@ -600,23 +851,30 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
         uint32_t depth = 2, cur = 0;
         // Step 2: BFS levels while the frontier is small.
         while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
-            const uint64_t want_words = (fixed_cap ? q->frontier_cap_rows : std::max<uint64_t>(n * 64, 1u << 20)) *
-                                        (depth + 1);
+            const uint64_t want_rows =
+                fixed_cap ? q->frontier_cap_rows : std::min<uint64_t>(std::max<uint64_t>(n * 64, 1u << 20), 1ull << 28);
+            const uint64_t want_words = want_rows * (depth + 1);
             Ensure(q, q->front[cur ^ 1], want_words);
             SetCounter(q, &Counters::rows_out, 0);
             SetCounter(q, &Counters::flags, 0);
+            const int ke = KBegin(q, CGBDSM_K_EXTEND);
             LaunchExtend(T, q->front[cur].p, n, depth, q->front[cur ^ 1].p, q->front[cur ^ 1].cap / (depth + 1),
                          q->d_cnt, q->s);
+            KEnd(q, ke);
             ReadCounters(q);
+            KAdd(q, CGBDSM_K_EXTEND, 0, n);
             if (q->h_cnt->flags & kFlagFrontier) break;  // keep this level, count it by DFS
             n = q->h_cnt->rows_out;
             cur ^= 1;
             depth++;
         }
         // Step 5: final DFS count from the current level.
-        SetCounter(q, &Counters::row_next, 0);
-        if (n) LaunchDfs(T, q->front[cur].p, n, depth, q->d_cnt, nullptr, 0, q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+        for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated}) SetCounter(q, f, 0);
+        RunDfs(q, q->front[cur].p, n, depth, nullptr, 0);
         ReadCounters(q);
+        if (q->h_cnt->flags & (kFlagSteal | kFlagAborted))
+            Throw(CGBDSM_ERR_OVERFLOW, "final DFS did not complete (flags " + std::to_string(q->h_cnt->flags) + ")");
+        if (n) KAdd(q, CGBDSM_K_DFS, 4ull * (q->h_cnt->scanned + q->h_cnt->probes), n);
         unsigned long long total = 0;
         for (uint32_t e = 0; e < q->qe; e++) {
             total += q->h_cnt->per_edge[e];
@@ -639,14 +897,12 @@ cgbdsm_status cgbdsm_match_tuples(cgbdsm_query* q, uint32_t* out, uint64_t cap, 
         const uint64_t words = (q->qv + 1) * std::max<uint64_t>(cap, 1);
         uint32_t* d_out = (uint32_t*)DevAlloc(q->ctx, words * 4);
         try {
-            SetCounter(q, &Counters::row_next, 0);
             SetCounter(q, &Counters::tuples, 0);
             SetCounter(q, &Counters::flags, 0);
             FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
-            if (q->last_rows)
-                LaunchDfs(MakeTables(q), q->front[q->last_buf].p, q->last_rows, q->last_depth, q->d_cnt, d_out, cap,
-                          q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+            RunDfs(q, q->front[q->last_buf].p, q->last_rows, q->last_depth, d_out, cap);
             ReadCounters(q);
+            if (q->h_cnt->flags & kFlagTuples) Throw(CGBDSM_ERR_OVERFLOW, "tuple capacity exceeded");
             const uint64_t n = q->h_cnt->tuples;
             if (n_out) *n_out = n;
             const uint64_t k = std::min(n, cap);
@@ -714,6 +970,25 @@ cgbdsm_status cgbdsm_run_batch(cgbdsm_query* q, unsigned long long* num_matches,
         q->st.ms_masks = Elapsed(q->ev[2], q->ev[3]);
         q->st.ms_match = Elapsed(q->ev[3], q->ev[4]);
         q->st.ms_commit = Elapsed(q->ev[4], q->ev[5]);
+        KHarvest(q);
+    });
+}
+
+cgbdsm_status cgbdsm_profile(cgbdsm_query* q, int enable) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        KHarvest(q);
+        q->profile = enable != 0;
+    });
+}
+
+cgbdsm_status cgbdsm_kernel_times(cgbdsm_query* q, cgbdsm_ktime* out, int reset) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        CGD_HIP(hipStreamSynchronize(q->s));
+        KHarvest(q);
+        if (out) memcpy(out, q->kt, sizeof(q->kt));
+        if (reset) memset(q->kt, 0, sizeof(q->kt));
     });
 }
 

@@ -38,24 +38,56 @@ struct alignas(8) URec {
 };
 
 // Device counters block (one per query), zeroed by a fill kernel.
-struct Counters {
+struct alignas(128) Counters {
     unsigned long long n_unique;      // keys after unique
     unsigned long long n_segs;        // segments (touched lists)
     unsigned long long pool_top;      // next free pool slot
     unsigned long long rows_out;      // frontier rows written by the last launch
-    unsigned long long row_next;      // DFS work queue head
+    unsigned long long row_next_unused;
     unsigned long long flags;         // bit 0: pool overflow, bit 1: frontier overflow, bit 2: tuple overflow
     unsigned long long bytes;         // algorithmic bytes of the last index pass
     unsigned long long rows_touched;  // lists rewritten by the last index pass
     unsigned long long u_rows;        // touched rows of the current U
-    unsigned long long steal_head, steal_tail, pending;  // DFS donation queue
+    unsigned long long unused_q[3];
     unsigned long long per_edge[kMaxQE];
     unsigned long long card[kMaxDir], nodes[kMaxDir];
     unsigned long long tuples;
-    unsigned long long pad[5];
+    unsigned long long idle_unused;
+    unsigned long long scanned;       // candidate list entries read by the final DFS
+    unsigned long long probes;        // binary-search probes of the final DFS
+    unsigned long long donated;       // partial matches handed to the stealing queue
+    unsigned long long pad[1];
+    // Words polled by many wavefronts at once, one 128-byte L2 line each.
+    struct alignas(128) Line {
+        unsigned long long v;
+        unsigned long long pad[15];
+    };
+    Line q_head, q_tail, q_pending, q_idle, q_rows;
 };
 
-enum : unsigned long long { kFlagPool = 1, kFlagFrontier = 2, kFlagTuples = 4, kFlagSteal = 8 };
+enum : unsigned long long {
+    kFlagPool = 1,
+    kFlagFrontier = 2,
+    kFlagTuples = 4,
+    kFlagSteal = 8,
+    kFlagAborted = 16
+};
+
+// One stolen piece of work: positions 0..depth-1 of the global order are
+// fixed, and candidates [lo, hi) of position `depth` remain to be scanned.
+struct alignas(16) StealItem {
+    uint32_t depth, ei;
+    uint32_t v[CGBDSM_MAX_QV];
+    uint32_t lo, hi;
+};
+
+struct StealQueue {
+    StealItem* items;
+    unsigned long long* seq;  // seq[slot] = ticket + 1 once the item is published
+    uint64_t cap;             // slots (power of two)
+    uint32_t donate_min;      // remaining survivors needed before donating (0: never)
+    const uint32_t* abort;    // host-mapped coherent flag, non-zero asks kernels to stop
+};
 
 // Per-query parameter block in device memory (read through scalar loads).
 // Replaces the __constant__ tables of graph/graph_gpu.cu:35-46 and

@@ -54,8 +54,8 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
                   uint64_t cap_rows, Counters* cnt, hipStream_t s);
 // Final DFS over n rows of `depth` words; counts per edge tag into cnt->per_edge.
 // tuples != nullptr: also writes full embeddings (qv words + tag word) up to tup_cap.
-void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt, uint32_t* tuples,
-               uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
+void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
 uint32_t DfsBlocksPerCU();
 
 }  // namespace cgd

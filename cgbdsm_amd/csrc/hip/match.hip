@@ -176,13 +176,73 @@ __global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __
 }
 
 struct DfsLevel {
-    uint32_t off, cnt, pos, pad;
+    uint32_t off, cnt, pos, end;  // end: last candidate offset (exclusive) of this level's range
 };
 
+constexpr uint32_t kFull = 0xFFFFFFFFu;
+constexpr uint32_t kRangeMin = 1024;  // candidates a donated range must at least hold
+
+__device__ __forceinline__ bool AbortRequested(const StealQueue& Q) {
+    return __hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+}
+
+__device__ __forceinline__ unsigned long long LoadAgent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Candidate check with the binary-search probes counted (final DFS only).
+__device__ __forceinline__ bool AcceptCounted(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched,
+                                              uint32_t c, unsigned long long& probes) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t ud = p.go_vs[d];
+    if (!TileBit(T, ei, ud, c)) return false;
+    for (uint32_t i = 0; i < d; i++)
+        if (matched[i] == c) return false;
+    const uint32_t s = p.go_bni_offs[d] + 1u, e = p.go_bni_offs[d + 1];
+    for (uint32_t k = s; k < e; k++) {
+        const uint32_t bq = p.go_bni[k];
+        const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + ud];
+        const List L = Resolve(T, dir, Uniform(matched[bq]), ei);
+        bool in = false;
+        uint32_t lo = 0, hi = L.bl;
+        while (lo < hi) {
+            const uint32_t m = (lo + hi) >> 1;
+            probes++;
+            if (L.base[m] < c) lo = m + 1;
+            else hi = m;
+        }
+        if (lo < L.bl) {
+            probes++;
+            in = L.base[lo] == c;
+        }
+        if (!in && L.al) {
+            lo = 0;
+            hi = L.al;
+            while (lo < hi) {
+                const uint32_t m = (lo + hi) >> 1;
+                probes++;
+                if (L.app[m] < c) lo = m + 1;
+                else hi = m;
+            }
+            if (lo < L.al) {
+                probes++;
+                in = L.app[lo] == c;
+            }
+        }
+        if (!in) return false;
+    }
+    return true;
+}
+
+// Final DFS.  Work items are frontier rows (taken kGrab at a time from
+// row_next) and, once those run out, partial matches stolen from the global
+// queue.  A wavefront that is descending at a shallow level while others
+// idle donates the tail of its survivor list as new items; `pending` counts
+// items not yet finished, so idle wavefronts leave once it reaches zero.
 template <bool kTuples>
 __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
-                                                 uint32_t D, Counters* __restrict__ cnt, uint32_t* __restrict__ tuples,
-                                                 uint64_t tup_cap) {
+                                                 uint32_t D, Counters* __restrict__ cnt, StealQueue Q,
+                                                 uint32_t* __restrict__ tuples, uint64_t tup_cap) {
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
     __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
     __shared__ uint32_t s_surv[kWpb][kMaxQV][kWave];
@@ -191,77 +251,261 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     DfsLevel* lvl = s_lvl[wv];
     const cgbdsm_plan& p = T.P->p;
     const uint32_t qv = p.qv, last = qv - 1;
-    for (;;) {
-        unsigned long long r0 = 0;
-        if (lane == 0) r0 = atomicAdd(&cnt->row_next, (unsigned long long)kGrab);
-        r0 = Uniform64(__shfl(r0, 0, kWave));
-        if (r0 >= n_rows) break;
-        const uint64_t r1 = r0 + kGrab < n_rows ? r0 + kGrab : n_rows;
-        for (uint64_t r = r0; r < r1; r++) {
-            const uint32_t w0 = Uniform(rows[r * D]);
-            const uint32_t ei = w0 >> T.P->tag_shift;
-            if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : rows[r * D + lane];
-            if (lane == 0) lvl[D] = DfsLevel{0, 0, 0, 0};
-            WaveSync();
-            unsigned long long count = 0;
-            uint32_t d = D;
-            for (;;) {
-                if (d < last) {
-                    const uint32_t pos = Uniform(lvl[d].pos);
-                    if (pos < Uniform(lvl[d].cnt)) {
-                        const uint32_t x = Uniform(s_surv[wv][d][pos]);
-                        if (lane == 0) {
-                            lvl[d].pos = pos + 1;
-                            matched[d] = x;
-                            lvl[d + 1] = DfsLevel{0, 0, 0, 0};
-                        }
-                        WaveSync();
-                        d++;
-                        continue;
-                    }
-                }
-                const uint32_t ud = p.go_vs[d];
-                const uint32_t pq = p.go_bni[p.go_bni_offs[d]];
-                const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
-                const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
-                const uint32_t total = L.bl + L.al;
-                const uint32_t off = Uniform(lvl[d].off);
-                if (off >= total) {
-                    if (d == D) break;
-                    d--;
-                    continue;
-                }
-                const uint32_t i = off + lane;
-                uint32_t c = 0;
-                bool ok = false;
-                if (i < total) {
-                    c = At(L, i);
-                    ok = Accept(T, ei, d, matched, c);
-                }
-                const uint64_t m = __ballot(ok);
-                if (d == last) {
-                    count += (unsigned long long)__popcll(m);
-                    if (kTuples && ok) {
-                        unsigned long long slot = atomicAdd(&cnt->tuples, 1ull);
-                        if (slot < tup_cap) {
-                            uint32_t* o = tuples + slot * (uint64_t)(qv + 1);
-                            for (uint32_t j = 0; j < last; j++) o[j] = matched[j];
-                            o[last] = c;
-                            o[qv] = ei;
-                        } else {
-                            atomicOr(&cnt->flags, kFlagTuples);
-                        }
-                    }
-                    if (lane == 0) lvl[d].off = off + kWave;
-                    WaveSync();
-                    continue;
-                }
-                if (ok) s_surv[wv][d][RankBelow(m)] = c;
-                if (lane == 0) lvl[d] = DfsLevel{off + kWave, (uint32_t)__popcll(m), 0, 0};
-                WaveSync();
+    unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
+    uint64_t r_next = 0, r_end = 0;
+    bool primary = true, idle = false, stop = false;
+    uint32_t iter = 0, countdown = 64, rcountdown = 64, backoff = 1;
+    while (!stop) {
+        // ---------------- obtain a work item
+        uint32_t D0 = 0, ei = 0, lo0 = 0, hi0 = kFull;
+        bool got = false;
+        if (primary) {
+            if (r_next >= r_end) {
+                unsigned long long r0 = 0;
+                if (lane == 0) r0 = atomicAdd(&cnt->q_rows.v, (unsigned long long)kGrab);
+                r0 = Uniform64(__shfl(r0, 0, kWave));
+                r_next = r0;
+                r_end = r0 + kGrab < n_rows ? r0 + kGrab : n_rows;
+                if (r0 >= n_rows) primary = false;
             }
-            if (lane == 0 && count) atomicAdd(&cnt->per_edge[ei], count);
+            if (primary) {
+                const uint64_t r = r_next++;
+                const uint32_t w0 = Uniform(rows[r * D]);
+                ei = w0 >> T.P->tag_shift;
+                if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : rows[r * D + lane];
+                D0 = D;
+                lo0 = 0;
+                hi0 = kFull;
+                got = true;
+            }
         }
+        if (!got) {
+            unsigned long long h = 0;
+            int ok = 0;
+            if (lane == 0) {
+                h = LoadAgent(&cnt->q_head.v);
+                const unsigned long long t = LoadAgent(&cnt->q_tail.v);
+                if (h < t && atomicCAS(&cnt->q_head.v, h, h + 1) == h) ok = 1;
+            }
+            ok = __shfl(ok, 0, kWave);
+            if (ok) {
+                h = Uniform64(__shfl(h, 0, kWave));
+                const uint64_t slot = h & (Q.cap - 1);
+                if (lane == 0) {
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(&Q.seq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != h + 1) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > (1u << 24)) {
+                            atomicOr(&cnt->flags, kFlagSteal);
+                            break;
+                        }
+                    }
+                }
+                WaveSync();
+                const StealItem* it = &Q.items[slot];
+                D0 = Uniform(it->depth);
+                ei = Uniform(it->ei);
+                lo0 = Uniform(it->lo);
+                hi0 = Uniform(it->hi);
+                if (lane < D0) matched[lane] = it->v[lane];
+                got = true;
+                backoff = 1;
+                if (idle) {
+                    if (lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
+                    idle = false;
+                }
+            } else {
+                unsigned long long pend = 0;
+                if (lane == 0) pend = LoadAgent(&cnt->q_pending.v);
+                pend = __shfl(pend, 0, kWave);
+                if (pend == 0) break;
+                if (!idle) {
+                    if (lane == 0) atomicAdd(&cnt->q_idle.v, 1ull);
+                    idle = true;
+                }
+                if ((++iter & 15u) == 0) {
+                    int ab = 0;
+                    if (lane == 0) ab = AbortRequested(Q) ? 1 : 0;
+                    if (__shfl(ab, 0, kWave)) break;
+                }
+                // Exponential backoff keeps idle pollers off the queue lines.
+                for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(127);
+                backoff = backoff < 32 ? backoff * 2 : 32;
+                continue;
+            }
+        }
+        // ---------------- DFS from position D0
+        if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0};
+        WaveSync();
+        unsigned long long count = 0;
+        uint32_t d = D0;
+        for (;;) {
+            if ((++iter & 1023u) == 0) {
+                int ab = 0;
+                if (lane == 0) ab = AbortRequested(Q) ? 1 : 0;
+                if (__shfl(ab, 0, kWave)) {
+                    if (lane == 0) atomicOr(&cnt->flags, kFlagAborted);
+                    stop = true;
+                    break;
+                }
+            }
+            if (d < last) {
+                const uint32_t pos = Uniform(lvl[d].pos);
+                uint32_t ccount = Uniform(lvl[d].cnt);
+                if (pos < ccount) {
+                    // Donate the tail of this level when others are idle.
+                    if (Q.donate_min && d + 2 <= last && ccount - pos >= Q.donate_min && --countdown == 0) {
+                        countdown = 64;
+                        unsigned long long idl = 0, h = 0, t = 0;
+                        if (lane == 0) {
+                            idl = LoadAgent(&cnt->q_idle.v);
+                            if (idl) {
+                                h = LoadAgent(&cnt->q_head.v);
+                                t = LoadAgent(&cnt->q_tail.v);
+                            }
+                        }
+                        idl = __shfl(idl, 0, kWave);
+                        h = __shfl(h, 0, kWave);
+                        t = __shfl(t, 0, kWave);
+                        if (idl > 0) {
+                            uint32_t k = ccount - pos - 1;
+                            const uint32_t want = (uint32_t)(idl < 8 ? 2 * idl : 16);
+                            k = k < want ? k : want;
+                            if (k && (t - h) + k <= Q.cap / 2) {
+                                unsigned long long t0 = 0;
+                                if (lane == 0) {
+                                    atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
+                                    t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
+                                }
+                                t0 = Uniform64(__shfl(t0, 0, kWave));
+                                if (lane < k) {
+                                    const uint64_t slot = (t0 + lane) & (Q.cap - 1);
+                                    StealItem* it = &Q.items[slot];
+                                    it->depth = d + 1;
+                                    it->ei = ei;
+                                    for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
+                                    it->v[d] = s_surv[wv][d][ccount - k + lane];
+                                    it->lo = 0;
+                                    it->hi = kFull;
+                                    __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                                }
+                                donated += k;
+                                ccount -= k;
+                                if (lane == 0) lvl[d].cnt = ccount;
+                                WaveSync();
+                            }
+                        }
+                    }
+                    const uint32_t x = Uniform(s_surv[wv][d][pos]);
+                    if (lane == 0) {
+                        lvl[d].pos = pos + 1;
+                        matched[d] = x;
+                        lvl[d + 1] = DfsLevel{0, 0, 0, kFull};
+                    }
+                    WaveSync();
+                    d++;
+                    continue;
+                }
+            }
+            const uint32_t ud = p.go_vs[d];
+            const uint32_t pq = p.go_bni[p.go_bni_offs[d]];
+            const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
+            const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
+            const uint32_t total = L.bl + L.al;
+            const uint32_t off = Uniform(lvl[d].off);
+            uint32_t limit = Uniform(lvl[d].end);
+            limit = limit < total ? limit : total;
+            if (off >= limit) {
+                if (d == D0) break;
+                d--;
+                continue;
+            }
+            // Split the rest of this level's candidate range with idle wavefronts.
+            if (Q.donate_min && limit - off >= 2 * kRangeMin && --rcountdown == 0) {
+                rcountdown = 64;
+                unsigned long long idl = 0, h = 0, t = 0;
+                if (lane == 0) {
+                    idl = LoadAgent(&cnt->q_idle.v);
+                    if (idl) {
+                        h = LoadAgent(&cnt->q_head.v);
+                        t = LoadAgent(&cnt->q_tail.v);
+                    }
+                }
+                idl = __shfl(idl, 0, kWave);
+                h = __shfl(h, 0, kWave);
+                t = __shfl(t, 0, kWave);
+                const uint32_t rem = limit - off;
+                uint32_t parts = (uint32_t)(idl < 15 ? idl + 1 : 16);
+                if (parts > rem / kRangeMin) parts = rem / kRangeMin;
+                if (idl > 0 && parts >= 2 && (t - h) + parts <= Q.cap / 2) {
+                    const uint32_t piece = ((rem + parts - 1) / parts + kWave - 1) & ~(uint32_t)(kWave - 1);
+                    const uint32_t k = (rem - 1) / piece;  // donated pieces after the kept one
+                    unsigned long long t0 = 0;
+                    if (lane == 0) {
+                        atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
+                        t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
+                    }
+                    t0 = Uniform64(__shfl(t0, 0, kWave));
+                    if (lane < k) {
+                        const uint64_t slot = (t0 + lane) & (Q.cap - 1);
+                        StealItem* it = &Q.items[slot];
+                        it->depth = d;
+                        it->ei = ei;
+                        for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
+                        const uint32_t plo = off + (lane + 1) * piece;
+                        it->lo = plo;
+                        it->hi = plo + piece < limit ? plo + piece : limit;
+                        __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    donated += k;
+                    limit = off + piece;
+                    if (lane == 0) lvl[d].end = limit;
+                    WaveSync();
+                }
+            }
+            const uint32_t i = off + lane;
+            uint32_t c = 0;
+            bool ok = false;
+            if (i < limit) {
+                c = At(L, i);
+                my_scanned++;
+                ok = AcceptCounted(T, ei, d, matched, c, my_probes);
+            }
+            const uint64_t m = __ballot(ok);
+            if (d == last) {
+                count += (unsigned long long)__popcll(m);
+                if (kTuples && ok) {
+                    unsigned long long slot = atomicAdd(&cnt->tuples, 1ull);
+                    if (slot < tup_cap) {
+                        uint32_t* o = tuples + slot * (uint64_t)(qv + 1);
+                        for (uint32_t j = 0; j < last; j++) o[j] = matched[j];
+                        o[last] = c;
+                        o[qv] = ei;
+                    } else {
+                        atomicOr(&cnt->flags, kFlagTuples);
+                    }
+                }
+                if (lane == 0) lvl[d].off = off + kWave;
+                WaveSync();
+                continue;
+            }
+            if (ok) s_surv[wv][d][RankBelow(m)] = c;
+            if (lane == 0) lvl[d] = DfsLevel{off + kWave, (uint32_t)__popcll(m), 0, limit};
+            WaveSync();
+        }
+        if (lane == 0) {
+            if (count) atomicAdd(&cnt->per_edge[ei], count);
+            atomicSub(&cnt->q_pending.v, 1ull);
+        }
+    }
+    if (idle && lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
+    const unsigned long long s_tot = __shfl(WaveInclusiveScan(my_scanned), 63, kWave);
+    const unsigned long long p_tot = __shfl(WaveInclusiveScan(my_probes), 63, kWave);
+    if (lane == 0) {
+        atomicAdd(&cnt->scanned, s_tot);
+        atomicAdd(&cnt->probes, p_tot);
+        if (donated) atomicAdd(&cnt->donated, donated);
     }
 }
 
@@ -285,14 +529,14 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
 
 uint32_t DfsBlocksPerCU() { return 8; }
 
-void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt, uint32_t* tuples,
-               uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
+void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
     if (!n) return;
     const unsigned g = Grid1(n, kWpb * kGrab, grid_blocks);
     if (tuples)
-        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, tuples, tup_cap);
+        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, Q, tuples, tup_cap);
     else
-        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, nullptr, 0);
+        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, Q, nullptr, 0);
 }
 
 }  // namespace cgd

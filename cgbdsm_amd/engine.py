@@ -100,6 +100,20 @@ class DeviceQuery:
         _check(self.L.cgbdsm_batch_stage(self.h, _ptr(ins), ins.size, _ptr(dele), dele.size, _ptr(upd), upd.size,
                                          eok.ctypes.data, kb.dv))
 
+    def enqueue(self, kb):
+        """Copy one batch's keys into the device-resident FIFO."""
+        ins = np.ascontiguousarray(kb.ins, dtype=np.uint64)
+        dele = np.ascontiguousarray(kb.dele, dtype=np.uint64)
+        upd = np.ascontiguousarray(kb.upd, dtype=np.uint64)
+        eok = np.ascontiguousarray(kb.edge_ok, dtype=np.uint8)
+        _check(self.L.cgbdsm_batch_enqueue(self.h, _ptr(ins), ins.size, _ptr(dele), dele.size, _ptr(upd),
+                                           upd.size, eok.ctypes.data, kb.dv))
+
+    def stage_next(self):
+        st = C.c_int()
+        _check(self.L.cgbdsm_batch_stage_next(self.h, C.byref(st)))
+        return bool(st.value)
+
     def transfer_update_all(self):
         _check(self.L.cgbdsm_transfer_update_all(self.h))
 
@@ -128,7 +142,7 @@ class DeviceQuery:
         _check(self.L.cgbdsm_run_batch(self.h, C.byref(n), per.ctypes.data, C.byref(sk)))
         return int(n.value), per[: self.qe].copy(), bool(sk.value)
 
-    def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=0):
+    def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=4):
         _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
 
     def export(self, which):
@@ -157,6 +171,18 @@ class DeviceQuery:
         s = N.Stats()
         _check(self.L.cgbdsm_stats_get(self.h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in N.Stats._fields_}
+
+    def set_deadline(self, ms):
+        _check(self.L.cgbdsm_set_deadline_ms(self.h, float(ms)))
+
+    def profile(self, enable=True):
+        _check(self.L.cgbdsm_profile(self.h, 1 if enable else 0))
+
+    def kernel_times(self, reset=False):
+        arr = (N.KTime * len(N.KERNEL_CLASSES))()
+        _check(self.L.cgbdsm_kernel_times(self.h, arr, 1 if reset else 0))
+        return {name: {"ms": arr[i].ms, "launches": arr[i].launches, "bytes": arr[i].bytes, "items": arr[i].items}
+                for i, name in enumerate(N.KERNEL_CLASSES)}
 
     def sync(self):
         _check(self.L.cgbdsm_sync(self.h))

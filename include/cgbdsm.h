@@ -139,6 +139,18 @@ cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q,
                                  const uint64_t* upd, uint64_t n_upd,
                                  const uint8_t* edge_ok, uint32_t dv_count);
 
+/* Device-resident batch FIFO.  cgbdsm_batch_enqueue validates and copies a
+ * batch's host keys into HBM (the boundary copy); cgbdsm_batch_stage_next
+ * makes the oldest enqueued batch the staged one without any copy.  Lets the
+ * Stage-1 thread run ahead of Stage 2 (main_calig.cpp:203-289) and lets
+ * callers time Stage 2 with its inputs already resident. */
+cgbdsm_status cgbdsm_batch_enqueue(cgbdsm_query* q,
+                                   const uint64_t* ins, uint64_t n_ins,
+                                   const uint64_t* del, uint64_t n_del,
+                                   const uint64_t* upd, uint64_t n_upd,
+                                   const uint8_t* edge_ok, uint32_t dv_count);
+cgbdsm_status cgbdsm_batch_stage_next(cgbdsm_query* q, int* staged);
+
 /* M <- (M u Ins) \ Del: CaLiGHelperGamma::TransferUpdateAll
  * (index/calig_helper_gamma.h:105, index/calig_helper_gamma.cu:485-517). */
 cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q);
@@ -171,6 +183,36 @@ cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q);
  * DFS wavefront donates work to the global stealing queue (0 disables). */
 cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit,
                                       uint64_t frontier_cap, uint32_t donate_min);
+
+/* Cooperative deadline for the final DFS of cgbdsm_matching (default 120 s,
+ * 0 = none).
+ * Past the deadline the host raises a flag the kernel polls; the call then
+ * returns CGBDSM_ERR_OVERFLOW because the count is incomplete.  No reference
+ * counterpart (the reference waits unboundedly, graph/graph_gpu.cu:1457-1464). */
+cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms);
+
+/* Kernel classes timed with HIP events on the query's stream when profiling
+ * is enabled (bytes = algorithmic bytes as defined per class). */
+enum {
+    CGBDSM_K_SORT = 0,          /* key decode + LSD radix sort              */
+    CGBDSM_K_MERGE_INSERT = 1,  /* sorted set-union merge into M lists      */
+    CGBDSM_K_MERGE_REMOVE = 2,  /* sorted set-difference compaction of M    */
+    CGBDSM_K_UPDATE_INDEX = 3,  /* segment discovery + U lists              */
+    CGBDSM_K_MASKS = 4,         /* support masks, all depths                */
+    CGBDSM_K_SEED = 5,          /* depth-2 frontier                         */
+    CGBDSM_K_EXTEND = 6,        /* BFS levels                               */
+    CGBDSM_K_DFS = 7,           /* final DFS count                          */
+    CGBDSM_K_COUNT = 8
+};
+typedef struct {
+    double ms;          /* summed device time */
+    uint64_t launches;
+    uint64_t bytes;     /* algorithmic bytes (merge classes, DFS)          */
+    uint64_t items;     /* keys (sort/merge), rows (seed/extend/DFS)        */
+} cgbdsm_ktime;
+cgbdsm_status cgbdsm_profile(cgbdsm_query* q, int enable);
+/* Copies CGBDSM_K_COUNT entries (synchronises the stream); reset != 0 clears. */
+cgbdsm_status cgbdsm_kernel_times(cgbdsm_query* q, cgbdsm_ktime* out, int reset);
 
 /* Parity dumps.  which = 0: main index M, 1: update index U.
  * Writes the index as sorted keys in the boundary format into out[cap];

@@ -73,7 +73,9 @@ def test_tuples_parity(eng):
     {"bfs_row_limit": 0},                          # final DFS straight from the seeds
     {"bfs_row_limit": 1 << 40},                    # BFS down to the last level
     {"bfs_row_limit": 1 << 40, "frontier_cap": 32},  # overflow -> DFS from the last good level
-], ids=["dfs-only", "bfs-deep", "frontier-overflow"])
+    {"bfs_row_limit": 0, "donate_min": 2},         # aggressive work donation between wavefronts
+    {"bfs_row_limit": 0, "donate_min": 0},         # no donation
+], ids=["dfs-only", "bfs-deep", "frontier-overflow", "steal-eager", "steal-off"])
 def test_matching_paths(eng, params):
     print(f"case paths {params}", flush=True)
     spec = synth_spec(11, 8, 4, seed=8, query_size=6, query_kind="dense", query_seed=8)

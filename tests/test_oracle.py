@@ -1,0 +1,262 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+The reference ships no tests, fixtures or golden outputs, so every oracle
+function is pinned against mathematics that fixes its answer:
+  * brute force vs textbook closed forms (triangles, 2-paths, k-stars, cliques)
+  * anchored per-edge counts vs the closed form emb(G_b) - emb(G_{b-1})
+  * index algebra vs numpy set operations
+  * masks: pure prune (count with masks == count without) and a direct
+    recomputation of the definition
+  * full CPU path (host Stage 1 + oracle Stage 2) vs the closed form
+A deliberately planted error in the oracle's inputs must be caught.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from cgbdsm_amd.host import HostSession, synth_spec
+from tests import oracle_lib as O
+
+
+def csr(n, edges):
+    adj = [set() for _ in range(n)]
+    for a, b in edges:
+        if a != b:
+            adj[a].add(b)
+            adj[b].add(a)
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    flat = []
+    for v in range(n):
+        offs[v + 1] = offs[v] + len(adj[v])
+        flat.extend(sorted(adj[v]))
+    return offs, np.array(flat if flat else [0], dtype=np.uint32)
+
+
+def random_graph(n, m, seed):
+    rng = np.random.default_rng(seed)
+    es = set()
+    while len(es) < m:
+        a, b = rng.integers(0, n, 2)
+        if a != b:
+            es.add((min(a, b), max(a, b)))
+    return sorted(es)
+
+
+def degrees(offs):
+    return np.diff(offs).astype(np.int64)
+
+
+# ---------------------------------------------------------------- brute force
+
+def test_brute_triangle_equals_six_times_triangles():
+    n = 40
+    edges = random_graph(n, 200, 1)
+    offs, adj = csr(n, edges)
+    lab = np.zeros(n, dtype=np.int32)
+    es = set(edges)
+    tri = sum(1 for a, b, c in itertools.combinations(range(n), 3)
+              if (a, b) in es and (b, c) in es and (a, c) in es)
+    got = O.brute_count(offs, adj, lab, [0, 0, 0], [0, 1, 1, 2, 0, 2])
+    assert got == 6 * tri
+
+
+def test_brute_two_path_and_star_closed_forms():
+    n = 50
+    offs, adj = csr(n, random_graph(n, 160, 2))
+    lab = np.zeros(n, dtype=np.int32)
+    d = degrees(offs)
+    assert O.brute_count(offs, adj, lab, [0, 0, 0], [0, 1, 1, 2]) == int(np.sum(d * (d - 1)))
+    for k in (2, 3, 4):
+        q_edges = []
+        for i in range(1, k + 1):
+            q_edges += [0, i]
+        want = sum(math.perm(int(x), k) for x in d)
+        assert O.brute_count(offs, adj, lab, [0] * (k + 1), q_edges) == want
+
+
+def test_brute_clique_counts_on_complete_graph():
+    n = 9
+    offs, adj = csr(n, [(a, b) for a in range(n) for b in range(a + 1, n)])
+    lab = np.zeros(n, dtype=np.int32)
+    q4 = [a for e in itertools.combinations(range(4), 2) for a in e]
+    assert O.brute_count(offs, adj, lab, [0] * 4, q4) == math.perm(n, 4)
+
+
+def test_brute_respects_labels():
+    # labelled path a-b-c where only label pattern 1-2-1 is allowed
+    n = 30
+    offs, adj = csr(n, random_graph(n, 90, 3))
+    rng = np.random.default_rng(3)
+    lab = rng.integers(0, 3, n).astype(np.int32)
+    want = 0
+    for v in range(n):
+        if lab[v] != 2:
+            continue
+        nb = adj[int(offs[v]):int(offs[v + 1])]
+        ones = [x for x in nb if lab[x] == 1]
+        want += len(ones) * (len(ones) - 1)
+    assert O.brute_count(offs, adj, lab, [1, 2, 1], [0, 1, 1, 2]) == want
+
+
+# ---------------------------------------------------------------- anchored
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_anchored_equals_closed_form(seed):
+    n = 60
+    all_edges = random_graph(n, 300, seed)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(all_edges))
+    old = [all_edges[i] for i in perm[:240]]
+    new = [all_edges[i] for i in perm[240:]]
+    lab = rng.integers(0, 2, n).astype(np.int32)
+    qlab = [0, 1, 0, 1]
+    qe = [0, 1, 1, 2, 2, 3, 0, 2]  # triangle + pendant
+    o0, a0 = csr(n, old)
+    o1, a1 = csr(n, old + new)
+    before = O.brute_count(o0, a0, lab, qlab, qe)
+    after = O.brute_count(o1, a1, lab, qlab, qe)
+    tot, per = O.anchored_count(o1, a1, lab, qlab, qe, np.array(new))
+    assert tot == after - before
+    assert per.sum() == tot
+
+
+def test_anchored_catches_planted_error():
+    """Planting one wrong 'new' edge changes the anchored total."""
+    n = 40
+    edges = random_graph(n, 150, 7)
+    old, new = edges[:120], edges[120:]
+    lab = np.zeros(n, dtype=np.int32)
+    o1, a1 = csr(n, edges)
+    qe = [0, 1, 1, 2, 0, 2]
+    good, _ = O.anchored_count(o1, a1, lab, [0, 0, 0], qe, np.array(new))
+    bad, _ = O.anchored_count(o1, a1, lab, [0, 0, 0], qe, np.array(new[:-1] + [old[0]]))
+    o0, a0 = csr(n, old)
+    assert good == O.brute_count(o1, a1, lab, [0, 0, 0], qe) - O.brute_count(o0, a0, lab, [0, 0, 0], qe)
+    assert bad != good
+
+
+# ---------------------------------------------------------------- index algebra
+
+def _keys(rng, n, ndir, dv, ebits=5):
+    e = rng.integers(0, ndir, n).astype(np.uint64)
+    u = rng.integers(0, dv, n).astype(np.uint64)
+    v = rng.integers(0, dv, n).astype(np.uint64)
+    return (e << np.uint64(64 - ebits)) | (u << np.uint64(32)) | v
+
+
+@pytest.mark.parametrize("ebits", [5, 7])
+def test_index_algebra_matches_numpy_sets(ebits):
+    rng = np.random.default_rng(11)
+    ndir, dv = 6, 300
+    idx = O.Index(ndir, ebits)
+    idx.set_dv(dv)
+    a = _keys(rng, 3000, ndir, dv, ebits)
+    idx.init(a)
+    ref = np.unique(a)
+    assert np.array_equal(idx.export(0), ref)
+    for step in range(4):
+        ins = _keys(rng, 800, ndir, dv, ebits)
+        dele = np.concatenate([_keys(rng, 300, ndir, dv, ebits), rng.choice(ref, 200)])
+        idx.insert(ins)
+        idx.remove(dele)
+        ref = np.setdiff1d(np.union1d(ref, ins), dele)
+        assert np.array_equal(idx.export(0), ref), step
+        upd = np.concatenate([_keys(rng, 100, ndir, dv, ebits)] * 2)  # duplicates are kept in U
+        idx.rebuild_update(upd)
+        assert np.array_equal(idx.export(1), np.sort(upd))
+
+
+# ---------------------------------------------------------------- full CPU path
+
+def _cpu_stream(spec, batch_edges, check_li=False):
+    host = HostSession(spec=spec, batch_edges=batch_edges, threads=4)
+    keys, dv, card, avg = host.initial_keys()
+    plan = host.make_plan(card, avg)
+    ql, qe = host.query_info()
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    offs, adj, lab = host.graph()
+    prev = O.brute_count(offs, adj, lab, ql, qe)
+    for _ in range(host.num_batches):
+        kb = host.next_batch()
+        idx.set_dv(kb.dv)
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        idx.rebuild_update(kb.upd)
+        masks = idx.masks(plan)
+        cnt, per = idx.match(plan, kb.edge_ok, masks=masks)
+        cnt0, per0 = idx.match(plan, kb.edge_ok, masks=None)
+        idx.insert(kb.upd)
+        offs, adj, lab = host.graph()
+        cur = O.brute_count(offs, adj, lab, ql, qe)
+        assert cnt == cnt0 and np.array_equal(per, per0), "support masks must be a pure prune"
+        assert cnt == cur - prev, "count differs from emb(G_b) - emb(G_b-1)"
+        if check_li:
+            li = host.lit(len(lab))
+            fp = O.li_fixpoint(offs, adj, lab, ql, qe)
+            assert np.array_equal(li, fp), "incremental lit bits differ from the from-scratch fixpoint"
+        prev = cur
+    return host
+
+
+@pytest.mark.parametrize("kind,seed", [("tree", 1), ("dense", 2), ("sparse", 3), ("dense", 4)])
+def test_cpu_path_matches_closed_form(kind, seed):
+    spec = synth_spec(10, 8, 4, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
+    _cpu_stream(spec, 300)
+
+
+def test_lit_bits_equal_from_scratch_fixpoint():
+    spec = synth_spec(9, 8, 3, seed=5, query_size=5, query_kind="dense", query_seed=5)
+    _cpu_stream(spec, 200, check_li=True)
+
+
+def test_masks_match_definition():
+    """Recompute one depth>=2 tile from the definition in Python."""
+    spec = synth_spec(9, 8, 3, seed=6, query_size=5, query_kind="dense", query_seed=6)
+    host = HostSession(spec=spec, batch_edges=300, threads=2)
+    keys, dv, card, avg = host.initial_keys()
+    plan = host.make_plan(card, avg)
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    kb = host.next_batch()
+    idx.set_dv(kb.dv)
+    idx.insert(kb.ins)
+    idx.remove(kb.dele)
+    idx.rebuild_update(kb.upd)
+    m, ones, W = idx.masks(plan)
+    M = idx.export(0)
+    U = idx.export(1)
+
+    def lists(keys_, e, v):
+        sel = keys_[(keys_ >> np.uint64(59)) == e]
+        sel = sel[((sel >> np.uint64(32)) & np.uint64((1 << 27) - 1)) == v]
+        return [int(x) & 0xFFFFFFFF for x in sel]
+
+    def bit(ei, qv, v):
+        t = ei * 12 + qv
+        return bool(ones[t]) or bool((int(m[t * W + v // 64]) >> (v % 64)) & 1)
+
+    checked = 0
+    for ei in range(plan.qe):
+        vs = list(plan.io_vs[ei])[: plan.qv]
+        for d in range(2, plan.qv):
+            u0 = vs[d]
+            if not plan.rebuild_v[ei][u0]:
+                continue
+            for v in range(kb.dv):
+                ok = True
+                for k in range(plan.io_bni_offs[ei][d], plan.io_bni_offs[ei][d + 1]):
+                    u1 = vs[plan.io_bni[ei][k]]
+                    bwd = plan.eidx[u0 * 12 + u1]
+                    cand = lists(M, bwd, v)
+                    if plan.dir_to_edge[bwd] <= ei:
+                        cand += lists(U, bwd, v)
+                    ok = ok and any(bit(ei, u1, x) for x in cand)
+                assert ok == bit(ei, u0, v), (ei, d, v)
+                checked += 1
+            break
+    assert checked > 0
