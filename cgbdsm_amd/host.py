@@ -55,10 +55,14 @@ def _arr(ptr, n, dtype):
 class HostSession:
     """One query over one data graph and its update stream."""
 
-    def __init__(self, *, files=None, spec=None, batch_edges=None, threads=0, ebits=5):
+    def __init__(self, *, files=None, spec=None, batch_edges=None, batch_ints=None, threads=0, ebits=5):
+        """batch_edges: stream edges per batch; batch_ints: the same in ints
+        (two per edge) as the reference CLI counts them, odd values included
+        (main_calig.cpp:65-69).  Neither: one batch."""
         L = self._L = N.host_lib()
         h = C.c_void_p()
-        batch_ints = 0 if batch_edges is None else 2 * int(batch_edges)
+        if batch_ints is None:
+            batch_ints = 0 if batch_edges is None else 2 * int(batch_edges)
         if files is not None:
             q, d, u = files
             _check(L.cgh_session_open(str(q).encode(), str(d).encode(), str(u).encode(), batch_ints,
