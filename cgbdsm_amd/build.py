@@ -25,7 +25,7 @@ ARCH = os.environ.get("CGBDSM_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", f"-I{INC}"]
-CXX_FLAGS = ["-O3", "-g", "-std=c++17", "-fPIC", "-Wall", f"-I{INC}", "-pthread"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INC}", "-pthread"]
 CLI_TARGETS = ["cgcsm_merged_matching", "cgcsm_gamma", "cgcsm_merged_matching_global"]
 
 

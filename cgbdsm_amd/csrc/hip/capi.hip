@@ -26,6 +26,7 @@ struct cgbdsm_ctx {
     int device = 0;
     uint64_t budget = 0, used = 0;
     int num_cus = 256;
+    int wall_khz = 100000;  // device wall clock (wall_clock64) rate
 };
 
 struct cgbdsm_query {
@@ -380,6 +381,8 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
     Q.cap = q->steal_cap;
     Q.donate_min = donate ? q->donate_min : 0;
     Q.abort = q->d_abort;
+    Q.poll_ticks = std::max<long long>(1, q->ctx->wall_khz / 20);  // 50 us
+    Q.host_ticks = (long long)q->ctx->wall_khz * 2;              // 2 ms
     return Q;
 }
 
@@ -387,6 +390,7 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
 // past the deadline the host raises the abort flag the kernel polls, and the
 // batch fails with CGBDSM_ERR_OVERFLOW (its count would be incomplete).
 void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
+    SetCounter(q, &Counters::flags, 0);
     FillU64((uint64_t*)&q->d_cnt->q_rows.v, 0, 1, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_pending.v, n, 1, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_idle.v, 0, 1, q->s);
@@ -458,6 +462,9 @@ cgbdsm_status cgbdsm_ctx_create(int device, uint64_t budget_bytes, cgbdsm_ctx** 
         c->device = device;
         c->budget = budget_bytes;
         c->num_cus = pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 256;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            c->wall_khz = khz;
         *out = c;
     });
 }
@@ -793,15 +800,23 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q) {
             info.n++;
         }
         LaunchMaskSeed(T, q->u_rows.p, Cnt(q, &Counters::u_rows), q->u_rows_upper, info, q->s);
+        if (q->profile) SetCounter(q, &Counters::bytes, 0);
+        uint64_t tiles = 0;
         for (uint32_t d = 2; d < q->qv; d++) {
             EiList D{};
             for (uint32_t k = 0; k < L.n; k++) {
                 const uint32_t ei = L.ei[k];
                 if (q->plan.rebuild_v[ei][q->plan.io_vs[ei][d]]) D.ei[D.n++] = (uint8_t)ei;
             }
-            LaunchMaskDepth(T, d, D, q->s);
+            LaunchMaskDepth(T, d, D, q->d_cnt, q->s);
+            tiles += D.n;
         }
         KEnd(q, kt);
+        if (q->profile) {
+            // Tile writes: ceil(dv/8) bytes per rebuilt tile and per seeded tile.
+            ReadCounters(q);
+            KAdd(q, CGBDSM_K_MASKS, q->h_cnt->bytes + (tiles + 2ull * L.n) * ((q->dv + 7) / 8), tiles);
+        }
         q->phase = 4;
     });
 }

@@ -87,6 +87,8 @@ struct StealQueue {
     uint64_t cap;             // slots (power of two)
     uint32_t donate_min;      // remaining survivors needed before donating (0: never)
     const uint32_t* abort;    // host-mapped coherent flag, non-zero asks kernels to stop
+    long long poll_ticks;     // wall-clock ticks between a wave's reads of shared queue words
+    long long host_ticks;     // wall-clock ticks between a wave's reads of the host flag
 };
 
 // Per-query parameter block in device memory (read through scalar loads).

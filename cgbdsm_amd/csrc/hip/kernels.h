@@ -46,7 +46,7 @@ void LaunchExport(const Tables& T, uint64_t nrows, int which, const uint64_t* ro
 void LaunchMaskClear(uint64_t* masks, uint64_t ei_words, const EiList& L, hipStream_t s);
 void LaunchMaskSeed(const Tables& T, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,
                     const EiSeedInfo& info, hipStream_t s);
-void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, hipStream_t s);
+void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, Counters* cnt, hipStream_t s);
 
 // match.hip
 void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);

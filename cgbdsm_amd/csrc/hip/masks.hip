@@ -47,7 +47,10 @@ __global__ void KMaskSeed(Tables T, const uint32_t* __restrict__ u_rows, const u
     }
 }
 
-__global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L) {
+// Algorithmic bytes counted into cnt->bytes: 4 per list entry read and 4 per
+// (backward neighbour, vertex) list size read; the tile write is added by
+// the host (ceil(dv/8) per rebuilt tile).
+__global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L, Counters* __restrict__ cnt) {
     const uint32_t ei = L.ei[blockIdx.y];
     const cgbdsm_plan& p = T.P->p;
     const uint32_t u0 = p.io_vs[ei][d];
@@ -55,6 +58,7 @@ __global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L
     const uint32_t lane = LaneId();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
     uint64_t* out = T.masks + ((uint64_t)ei * kMaxQV + u0) * T.W;
+    unsigned long long reads = 0;
     for (uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; w < T.W; w += nw) {
         const uint32_t v = (uint32_t)(w * 64 + lane);
         bool ok = v < T.dv;
@@ -67,11 +71,14 @@ __global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L
                 const uint64_t r = (uint64_t)v * T.ndir + bwd;
                 const MRec m = T.m_rec[r];
                 const uint32_t* L0 = T.pool + m.off;
-                for (uint32_t j = 0; j < m.len && !sat; j++) sat = TileBitM(T, ei, u1, L0[j]);
+                uint32_t j = 0;
+                for (; j < m.len && !sat; j++) sat = TileBitM(T, ei, u1, L0[j]);
+                reads += 1 + j;
                 if (!sat && up_vis) {
                     const URec uu = T.u_rec[r];
                     const uint32_t* L1 = T.u_vals + uu.off;
-                    for (uint32_t j = 0; j < uu.len && !sat; j++) sat = TileBitM(T, ei, u1, L1[j]);
+                    for (j = 0; j < uu.len && !sat; j++) sat = TileBitM(T, ei, u1, L1[j]);
+                    reads += 1 + j;
                 }
             }
             ok = ok && sat;
@@ -80,6 +87,8 @@ __global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L
         const uint64_t word = __ballot(ok);
         if (lane == 0) out[w] = word;
     }
+    const unsigned long long tot = __shfl(WaveInclusiveScan(reads), 63, kWave);
+    if (lane == 0 && tot) atomicAdd(&cnt->bytes, 4ull * tot);
 }
 
 }  // namespace
@@ -97,10 +106,10 @@ void LaunchMaskSeed(const Tables& T, const uint32_t* u_rows, const unsigned long
     KMaskSeed<<<g, 256, 0, s>>>(T, u_rows, n_dev, info);
 }
 
-void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, hipStream_t s) {
+void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, Counters* cnt, hipStream_t s) {
     if (!L.n || !T.W) return;
     unsigned gx = (unsigned)std::min<uint64_t>((T.W + 3) / 4, 4096);
-    KMaskDepth<<<dim3(gx, L.n), 256, 0, s>>>(T, d, L);
+    KMaskDepth<<<dim3(gx, L.n), 256, 0, s>>>(T, d, L, cnt);
 }
 
 }  // namespace cgd

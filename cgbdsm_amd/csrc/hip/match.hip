@@ -254,7 +254,34 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
     uint64_t r_next = 0, r_end = 0;
     bool primary = true, idle = false, stop = false;
-    uint32_t iter = 0, countdown = 64, rcountdown = 64, backoff = 1;
+    uint32_t iter = 0, backoff = 1;
+    // Shared words (idle count, flags) and the host flag are read at most
+    // once per poll_ticks / host_ticks of wall clock per wave: thousands of
+    // waves polling one address would otherwise saturate the memory-side
+    // path that agent-scope loads take and stall the waves that work.
+    long long t_poll = Uniform64(wall_clock64()), t_host = t_poll;
+    unsigned long long idl_hint = 0;
+    // Returns true when the kernel is asked to stop.
+    auto poll = [&]() -> bool {
+        const long long now = (long long)Uniform64((unsigned long long)wall_clock64());
+        if (now - t_poll < Q.poll_ticks) return false;
+        t_poll = now;
+        const bool host = now - t_host >= Q.host_ticks;
+        if (host) t_host = now;
+        unsigned long long idl = 0, fl = 0;
+        int ab = 0;
+        if (lane == 0) {
+            idl = LoadAgent(&cnt->q_idle.v);
+            fl = LoadAgent(&cnt->flags);
+            if (host && AbortRequested(Q)) {
+                atomicOr(&cnt->flags, kFlagAborted);
+                ab = 1;
+            }
+        }
+        idl_hint = __shfl(idl, 0, kWave);
+        fl = __shfl(fl, 0, kWave);
+        return __shfl(ab, 0, kWave) || (fl & kFlagAborted);
+    };
     while (!stop) {
         // ---------------- obtain a work item
         uint32_t D0 = 0, ei = 0, lo0 = 0, hi0 = kFull;
@@ -293,13 +320,16 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 const uint64_t slot = h & (Q.cap - 1);
                 if (lane == 0) {
                     uint32_t spins = 0;
-                    while (__hip_atomic_load(&Q.seq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != h + 1) {
+                    // Relaxed spin, one acquire fence after: an acquire per
+                    // iteration would invalidate this CU's L1 each time.
+                    while (LoadAgent(&Q.seq[slot]) != h + 1) {
                         __builtin_amdgcn_s_sleep(2);
                         if (++spins > (1u << 24)) {
                             atomicOr(&cnt->flags, kFlagSteal);
                             break;
                         }
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 }
                 WaveSync();
                 const StealItem* it = &Q.items[slot];
@@ -323,11 +353,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                     if (lane == 0) atomicAdd(&cnt->q_idle.v, 1ull);
                     idle = true;
                 }
-                if ((++iter & 15u) == 0) {
-                    int ab = 0;
-                    if (lane == 0) ab = AbortRequested(Q) ? 1 : 0;
-                    if (__shfl(ab, 0, kWave)) break;
-                }
+                if (poll()) break;
                 // Exponential backoff keeps idle pollers off the queue lines.
                 for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(127);
                 backoff = backoff < 32 ? backoff * 2 : 32;
@@ -340,36 +366,26 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
         unsigned long long count = 0;
         uint32_t d = D0;
         for (;;) {
-            if ((++iter & 1023u) == 0) {
-                int ab = 0;
-                if (lane == 0) ab = AbortRequested(Q) ? 1 : 0;
-                if (__shfl(ab, 0, kWave)) {
-                    if (lane == 0) atomicOr(&cnt->flags, kFlagAborted);
-                    stop = true;
-                    break;
-                }
+            if ((++iter & 31u) == 0 && poll()) {
+                stop = true;
+                break;
             }
             if (d < last) {
                 const uint32_t pos = Uniform(lvl[d].pos);
                 uint32_t ccount = Uniform(lvl[d].cnt);
                 if (pos < ccount) {
                     // Donate the tail of this level when others are idle.
-                    if (Q.donate_min && d + 2 <= last && ccount - pos >= Q.donate_min && --countdown == 0) {
-                        countdown = 64;
-                        unsigned long long idl = 0, h = 0, t = 0;
+                    if (Q.donate_min && idl_hint > 0 && d + 2 <= last && ccount - pos >= Q.donate_min) {
+                        unsigned long long h = 0, t = 0;
                         if (lane == 0) {
-                            idl = LoadAgent(&cnt->q_idle.v);
-                            if (idl) {
-                                h = LoadAgent(&cnt->q_head.v);
-                                t = LoadAgent(&cnt->q_tail.v);
-                            }
+                            h = LoadAgent(&cnt->q_head.v);
+                            t = LoadAgent(&cnt->q_tail.v);
                         }
-                        idl = __shfl(idl, 0, kWave);
                         h = __shfl(h, 0, kWave);
                         t = __shfl(t, 0, kWave);
-                        if (idl > 0) {
+                        {
                             uint32_t k = ccount - pos - 1;
-                            const uint32_t want = (uint32_t)(idl < 8 ? 2 * idl : 16);
+                            const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
                             k = k < want ? k : want;
                             if (k && (t - h) + k <= Q.cap / 2) {
                                 unsigned long long t0 = 0;
@@ -392,6 +408,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                                 }
                                 donated += k;
                                 ccount -= k;
+                                idl_hint = idl_hint > k ? idl_hint - k : 0;
                                 if (lane == 0) lvl[d].cnt = ccount;
                                 WaveSync();
                             }
@@ -422,23 +439,18 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 continue;
             }
             // Split the rest of this level's candidate range with idle wavefronts.
-            if (Q.donate_min && limit - off >= 2 * kRangeMin && --rcountdown == 0) {
-                rcountdown = 64;
-                unsigned long long idl = 0, h = 0, t = 0;
+            if (Q.donate_min && idl_hint > 0 && limit - off >= 2 * kRangeMin) {
+                unsigned long long h = 0, t = 0;
                 if (lane == 0) {
-                    idl = LoadAgent(&cnt->q_idle.v);
-                    if (idl) {
-                        h = LoadAgent(&cnt->q_head.v);
-                        t = LoadAgent(&cnt->q_tail.v);
-                    }
+                    h = LoadAgent(&cnt->q_head.v);
+                    t = LoadAgent(&cnt->q_tail.v);
                 }
-                idl = __shfl(idl, 0, kWave);
                 h = __shfl(h, 0, kWave);
                 t = __shfl(t, 0, kWave);
                 const uint32_t rem = limit - off;
-                uint32_t parts = (uint32_t)(idl < 15 ? idl + 1 : 16);
+                uint32_t parts = (uint32_t)(idl_hint < 15 ? idl_hint + 1 : 16);
                 if (parts > rem / kRangeMin) parts = rem / kRangeMin;
-                if (idl > 0 && parts >= 2 && (t - h) + parts <= Q.cap / 2) {
+                if (parts >= 2 && (t - h) + parts <= Q.cap / 2) {
                     const uint32_t piece = ((rem + parts - 1) / parts + kWave - 1) & ~(uint32_t)(kWave - 1);
                     const uint32_t k = (rem - 1) / piece;  // donated pieces after the kept one
                     unsigned long long t0 = 0;
@@ -459,6 +471,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     donated += k;
+                    idl_hint = idl_hint > k ? idl_hint - k : 0;
                     limit = off + piece;
                     if (lane == 0) lvl[d].end = limit;
                     WaveSync();
