@@ -139,6 +139,7 @@ def dev_lib():
         L.cgbdsm_run_batch.argtypes = [P, C.POINTER(C.c_ulonglong), C.c_void_p, C.POINTER(I)]
         L.cgbdsm_sync.argtypes = [P]
         L.cgbdsm_set_deadline_ms.argtypes = [P, C.c_double]
+        L.cgbdsm_set_dfs_options.argtypes = [P, C.c_int, C.c_int]
         L.cgbdsm_profile.argtypes = [P, I]
         L.cgbdsm_kernel_times.argtypes = [P, C.c_void_p, I]
         L.cgbdsm_batch_enqueue.argtypes = [P, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U32]
@@ -153,6 +154,7 @@ DEV_SYMBOLS = [
     "cgbdsm_batch_stage", "cgbdsm_transfer_update_all", "cgbdsm_transfer_update_index",
     "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_batch_update_global_index",
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
-    "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_profile",
+    "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
+    "cgbdsm_profile",
     "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
 ]

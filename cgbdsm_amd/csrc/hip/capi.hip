@@ -52,6 +52,7 @@ struct cgbdsm_query {
     uint8_t edge_ok[kMaxQE] = {};
     uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
     uint32_t donate_min = 4;
+    bool dfs_shortest = true, dfs_fold = true;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;
@@ -380,6 +381,8 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
     Q.seq = q->steal_seq;
     Q.cap = q->steal_cap;
     Q.donate_min = donate ? q->donate_min : 0;
+    Q.shortest = q->dfs_shortest ? 1 : 0;
+    Q.fold = q->dfs_fold ? 1 : 0;
     Q.abort = q->d_abort;
     Q.poll_ticks = std::max<long long>(1, q->ctx->wall_khz / 20);  // 50 us
     Q.host_ticks = (long long)q->ctx->wall_khz * 2;              // 2 ms
@@ -565,6 +568,16 @@ cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
         q->hplan.p = *plan;
         for (uint32_t e = 0; e < q->qe; e++)
             for (uint32_t v = 0; v < q->qv; v++) q->hplan.all_ones[e][v] = plan->rebuild_v[e][v] ? 0 : 1;
+        // Position qv-1 can be folded into qv-2 when none of its backward
+        // neighbours is position qv-2 (its candidates then do not depend on it).
+        q->hplan.fold_last = 0;
+        if (plan->qv >= 3) {
+            const uint32_t last = plan->qv - 1;
+            bool fold = plan->go_bni_offs[last + 1] > plan->go_bni_offs[last];
+            for (uint32_t k = plan->go_bni_offs[last]; k < plan->go_bni_offs[last + 1]; k++)
+                if (plan->go_bni[k] >= last - 1) fold = false;
+            q->hplan.fold_last = fold ? 1 : 0;
+        }
         UploadPlan(q);
         q->have_order = true;
     });
@@ -765,6 +778,8 @@ cgbdsm_status cgbdsm_transfer_update_index(cgbdsm_query* q, int* nonempty) {
         q->u_nonempty = q->n_upd > 0;
         if (q->n_upd) {
             DecodeSort(q, q->k_upd.p, q->k_usort.p, q->k_alt.p, q->n_upd);
+            SetCounter(q, &Counters::u_dups, 0);
+            LaunchCountDups(q->k_usort.p, q->n_upd, q->d_cnt, q->s);
             SetCounter(q, &Counters::n_unique, q->n_upd);
             const int k = KBegin(q, CGBDSM_K_UPDATE_INDEX);
             LaunchSegments(q->k_usort.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
@@ -828,6 +843,14 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit, u
         q->bfs_row_limit = bfs_row_limit;
         q->frontier_cap_rows = frontier_cap;
         q->donate_min = donate_min;
+    });
+}
+
+cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int fold_last) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        q->dfs_shortest = shortest_list != 0;
+        q->dfs_fold = fold_last != 0;
     });
 }
 

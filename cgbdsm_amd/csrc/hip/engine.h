@@ -43,7 +43,7 @@ struct alignas(128) Counters {
     unsigned long long n_segs;        // segments (touched lists)
     unsigned long long pool_top;      // next free pool slot
     unsigned long long rows_out;      // frontier rows written by the last launch
-    unsigned long long row_next_unused;
+    unsigned long long u_dups;        // adjacent equal keys in the sorted update keys
     unsigned long long flags;         // bit 0: pool overflow, bit 1: frontier overflow, bit 2: tuple overflow
     unsigned long long bytes;         // algorithmic bytes of the last index pass
     unsigned long long rows_touched;  // lists rewritten by the last index pass
@@ -89,6 +89,8 @@ struct StealQueue {
     const uint32_t* abort;    // host-mapped coherent flag, non-zero asks kernels to stop
     long long poll_ticks;     // wall-clock ticks between a wave's reads of shared queue words
     long long host_ticks;     // wall-clock ticks between a wave's reads of the host flag
+    uint32_t shortest;        // scan the shortest backward-neighbour list when lists are sets
+    uint32_t fold;            // fold the last position into its predecessor where the plan allows
 };
 
 // Per-query parameter block in device memory (read through scalar loads).
@@ -100,6 +102,7 @@ struct DevPlan {
     uint8_t sa[kMaxQE], sb[kMaxQE];    // the two directions of edge ei
     uint32_t tag_shift;                // frontier word 0 = ei << tag_shift | v0
     uint32_t v_mask;                   // (1 << tag_shift) - 1
+    uint32_t fold_last;                // the last position's backward neighbours all precede qv-2
 };
 
 // Device pointers of one query's index, passed by value to kernels.

@@ -409,4 +409,20 @@ void LaunchExport(const Tables& T, uint64_t nrows, int which, const uint64_t* ro
         KExportM<<<GridFor(nrows), 256, 0, s>>>(T, row_off, nrows, ushift, out);
 }
 
+// Adjacent equal keys of a sorted key array into cnt->u_dups (0 means the
+// update lists are sets).
+__global__ void KCountDups(const uint64_t* __restrict__ k, uint64_t n, Counters* __restrict__ cnt) {
+    unsigned long long mine = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        mine += k[i] == k[i - 1];
+    const unsigned long long tot = __shfl(WaveInclusiveScan(mine), 63, kWave);
+    if (LaneId() == 0 && tot) atomicAdd(&cnt->u_dups, tot);
+}
+
+void LaunchCountDups(const uint64_t* k, uint64_t n, Counters* cnt, hipStream_t s) {
+    if (n < 2) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
+    KCountDups<<<g, 256, 0, s>>>(k, n, cnt);
+}
+
 }  // namespace cgd

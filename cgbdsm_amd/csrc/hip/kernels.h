@@ -58,4 +58,6 @@ void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
 uint32_t DfsBlocksPerCU();
 
+void LaunchCountDups(const uint64_t* k, uint64_t n, Counters* cnt, hipStream_t s);
+
 }  // namespace cgd

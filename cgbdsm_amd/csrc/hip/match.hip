@@ -176,10 +176,14 @@ __global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __
 }
 
 struct DfsLevel {
-    uint32_t off, cnt, pos, end;  // end: last candidate offset (exclusive) of this level's range
+    uint32_t off, cnt, pos, end;  // scan offset, survivors, next survivor, range end (exclusive)
+    uint32_t sel;                 // backward neighbour whose list is scanned (kNoSel: not chosen yet)
+    uint32_t x;                   // folded level: |X| + 1 once computed, 0 before
 };
 
 constexpr uint32_t kFull = 0xFFFFFFFFu;
+constexpr uint32_t kNoSel = 0xFFu;
+constexpr uint32_t kNoSkip = 0xFFu;
 constexpr uint32_t kRangeMin = 1024;  // candidates a donated range must at least hold
 
 __device__ __forceinline__ bool AbortRequested(const StealQueue& Q) {
@@ -190,55 +194,128 @@ __device__ __forceinline__ unsigned long long LoadAgent(const unsigned long long
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Candidate check with the binary-search probes counted (final DFS only).
-__device__ __forceinline__ bool AcceptCounted(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched,
-                                              uint32_t c, unsigned long long& probes) {
+// List of backward neighbour k of position d (owner = its matched vertex);
+// k must be the same in every lane.
+__device__ __forceinline__ List BnList(const Tables& T, uint32_t ei, uint32_t d, uint32_t k, const uint32_t* matched) {
     const cgbdsm_plan& p = T.P->p;
-    const uint32_t ud = p.go_vs[d];
-    if (!TileBit(T, ei, ud, c)) return false;
-    for (uint32_t i = 0; i < d; i++)
+    const uint32_t bq = p.go_bni[p.go_bni_offs[d] + k];
+    const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]];
+    return Resolve(T, dir, Uniform(matched[bq]), ei);
+}
+
+// Occurrences test by binary search in both parts of a list (probes counted).
+__device__ __forceinline__ bool InList(const List& L, uint32_t c, unsigned long long& probes) {
+    uint32_t lo = 0, hi = L.bl;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        probes++;
+        if (L.base[m] < c) lo = m + 1;
+        else hi = m;
+    }
+    if (lo < L.bl) {
+        probes++;
+        if (L.base[lo] == c) return true;
+    }
+    if (!L.al) return false;
+    lo = 0;
+    hi = L.al;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        probes++;
+        if (L.app[m] < c) lo = m + 1;
+        else hi = m;
+    }
+    if (lo < L.al) {
+        probes++;
+        return L.app[lo] == c;
+    }
+    return false;
+}
+
+// Candidate c for position d: support-mask bit, distinct from positions
+// 0..ninj-1, present in the list of every backward neighbour except `skip`.
+__device__ __forceinline__ bool AcceptSel(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, uint32_t c,
+                                          uint32_t skip, uint32_t ninj, unsigned long long& probes) {
+    const cgbdsm_plan& p = T.P->p;
+    if (!TileBit(T, ei, p.go_vs[d], c)) return false;
+    for (uint32_t i = 0; i < ninj; i++)
         if (matched[i] == c) return false;
-    const uint32_t s = p.go_bni_offs[d] + 1u, e = p.go_bni_offs[d + 1];
-    for (uint32_t k = s; k < e; k++) {
-        const uint32_t bq = p.go_bni[k];
-        const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + ud];
-        const List L = Resolve(T, dir, Uniform(matched[bq]), ei);
-        bool in = false;
-        uint32_t lo = 0, hi = L.bl;
-        while (lo < hi) {
-            const uint32_t m = (lo + hi) >> 1;
-            probes++;
-            if (L.base[m] < c) lo = m + 1;
-            else hi = m;
-        }
-        if (lo < L.bl) {
-            probes++;
-            in = L.base[lo] == c;
-        }
-        if (!in && L.al) {
-            lo = 0;
-            hi = L.al;
-            while (lo < hi) {
-                const uint32_t m = (lo + hi) >> 1;
-                probes++;
-                if (L.app[m] < c) lo = m + 1;
-                else hi = m;
-            }
-            if (lo < L.al) {
-                probes++;
-                in = L.app[lo] == c;
-            }
-        }
-        if (!in) return false;
+    const uint32_t nbn = p.go_bni_offs[d + 1] - p.go_bni_offs[d];
+    for (uint32_t k = 0; k < nbn; k++) {
+        if (k == skip) continue;
+        if (!InList(BnList(T, ei, d, k, matched), c, probes)) return false;
     }
     return true;
 }
 
-// Final DFS.  Work items are frontier rows (taken kGrab at a time from
-// row_next) and, once those run out, partial matches stolen from the global
-// queue.  A wavefront that is descending at a shallow level while others
-// idle donates the tail of its survivor list as new items; `pending` counts
-// items not yet finished, so idle wavefronts leave once it reaches zero.
+// Backward neighbour of position d with the shortest visible list (lowest
+// index on ties).  Scanning it instead of the first one enumerates the same
+// set when no list holds a value twice.
+__device__ __forceinline__ uint32_t ChooseList(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t nbn = p.go_bni_offs[d + 1] - p.go_bni_offs[d];
+    if (nbn <= 1) return 0;
+    const uint32_t lane = LaneId();
+    uint32_t key = 0xFFFFFFFFu;
+    if (lane < nbn) {
+        const uint32_t bq = p.go_bni[p.go_bni_offs[d] + lane];
+        const List L = Resolve(T, p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]], matched[bq], ei);
+        const uint32_t n = L.bl + L.al;
+        key = ((n < (1u << 27) ? n : (1u << 27) - 1) << 4) | lane;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t other = __shfl_xor(key, o, kWave);
+        key = other < key ? other : key;
+    }
+    return Uniform(key) & 15u;
+}
+
+// |X| for the folded level d = last - 1: the candidates x of position `last`
+// (support-mask bit, every backward-neighbour list) distinct from positions
+// 0..d-1.  The last position's backward neighbours all precede d, so X is
+// the same for every candidate of position d.
+__device__ uint32_t FoldCount(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, unsigned long long& scanned,
+                              unsigned long long& probes) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t last = d + 1, lane = LaneId();
+    const uint32_t nbn = p.go_bni_offs[last + 1] - p.go_bni_offs[last];
+    if (nbn == 1 && T.P->all_ones[ei][p.go_vs[last]]) {
+        // |list| minus the positions 0..d-1 that occur in it.
+        const List L = BnList(T, ei, last, 0, matched);
+        bool hit = false;
+        if (lane < d) hit = InList(L, matched[lane], probes);
+        return Uniform(L.bl + L.al) - (uint32_t)__popcll(__ballot(hit));
+    }
+    const uint32_t sel = ChooseList(T, ei, last, matched);
+    const List L = BnList(T, ei, last, sel, matched);
+    const uint32_t total = Uniform(L.bl + L.al);
+    uint32_t x = 0;
+    for (uint32_t off = 0; off < total; off += kWave) {
+        const uint32_t i = off + lane;
+        bool ok = false;
+        if (i < total) {
+            scanned++;
+            ok = AcceptSel(T, ei, last, matched, At(L, i), sel, d, probes);
+        }
+        x += (uint32_t)__popcll(__ballot(ok));
+    }
+    return x;
+}
+
+// Final DFS.  Work items are frontier rows (taken kGrab at a time from a
+// global cursor) and, once those run out, partial matches stolen from the
+// global queue.  A wavefront that is descending at a shallow level while
+// others idle donates the tail of its survivor list (items one level
+// deeper) or the rest of a long candidate range (items at the same level,
+// same scanned list); `pending` counts items not yet finished, so idle
+// wavefronts leave once it reaches zero.
+//
+// Count mode refinements (identical counts):
+//  * each level scans its shortest backward-neighbour list when the batch's
+//    update lists hold no duplicate value (then every list is a set);
+//  * when the last position's backward neighbours all precede position
+//    last-1, position last is never visited: a candidate c of last-1 adds
+//    |X| - [c in X] (FoldCount).
 template <bool kTuples>
 __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
                                                  uint32_t D, Counters* __restrict__ cnt, StealQueue Q,
@@ -251,6 +328,9 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     DfsLevel* lvl = s_lvl[wv];
     const cgbdsm_plan& p = T.P->p;
     const uint32_t qv = p.qv, last = qv - 1;
+    // Every list is a set when the update lists hold no duplicate value.
+    const bool sets = Q.shortest && Uniform64(cnt->u_dups) == 0;
+    const bool fold = !kTuples && sets && Q.fold && T.P->fold_last;
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
     uint64_t r_next = 0, r_end = 0;
     bool primary = true, idle = false, stop = false;
@@ -284,7 +364,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     };
     while (!stop) {
         // ---------------- obtain a work item
-        uint32_t D0 = 0, ei = 0, lo0 = 0, hi0 = kFull;
+        uint32_t D0 = 0, ei = 0, lo0 = 0, hi0 = kFull, sel0 = kNoSel;
         bool got = false;
         if (primary) {
             if (r_next >= r_end) {
@@ -301,8 +381,6 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 ei = w0 >> T.P->tag_shift;
                 if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : rows[r * D + lane];
                 D0 = D;
-                lo0 = 0;
-                hi0 = kFull;
                 got = true;
             }
         }
@@ -334,7 +412,9 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 WaveSync();
                 const StealItem* it = &Q.items[slot];
                 D0 = Uniform(it->depth);
-                ei = Uniform(it->ei);
+                const uint32_t tag = Uniform(it->ei);
+                ei = tag & 0xFFu;
+                sel0 = tag >> 8;
                 lo0 = Uniform(it->lo);
                 hi0 = Uniform(it->hi);
                 if (lane < D0) matched[lane] = it->v[lane];
@@ -361,7 +441,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
         }
         // ---------------- DFS from position D0
-        if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0};
+        if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0, sel0, 0};
         WaveSync();
         unsigned long long count = 0;
         uint32_t d = D0;
@@ -370,7 +450,9 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 stop = true;
                 break;
             }
-            if (d < last) {
+            // Positions below `term` descend into their survivors.
+            const uint32_t term = fold ? last - 1 : last;
+            if (d < term) {
                 const uint32_t pos = Uniform(lvl[d].pos);
                 uint32_t ccount = Uniform(lvl[d].cnt);
                 if (pos < ccount) {
@@ -383,53 +465,54 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         }
                         h = __shfl(h, 0, kWave);
                         t = __shfl(t, 0, kWave);
-                        {
-                            uint32_t k = ccount - pos - 1;
-                            const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
-                            k = k < want ? k : want;
-                            if (k && (t - h) + k <= Q.cap / 2) {
-                                unsigned long long t0 = 0;
-                                if (lane == 0) {
-                                    atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
-                                    t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
-                                }
-                                t0 = Uniform64(__shfl(t0, 0, kWave));
-                                if (lane < k) {
-                                    const uint64_t slot = (t0 + lane) & (Q.cap - 1);
-                                    StealItem* it = &Q.items[slot];
-                                    it->depth = d + 1;
-                                    it->ei = ei;
-                                    for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
-                                    it->v[d] = s_surv[wv][d][ccount - k + lane];
-                                    it->lo = 0;
-                                    it->hi = kFull;
-                                    __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-                                }
-                                donated += k;
-                                ccount -= k;
-                                idl_hint = idl_hint > k ? idl_hint - k : 0;
-                                if (lane == 0) lvl[d].cnt = ccount;
-                                WaveSync();
+                        uint32_t k = ccount - pos - 1;
+                        const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
+                        k = k < want ? k : want;
+                        if (k && (t - h) + k <= Q.cap / 2) {
+                            unsigned long long t0 = 0;
+                            if (lane == 0) {
+                                atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
+                                t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
                             }
+                            t0 = Uniform64(__shfl(t0, 0, kWave));
+                            if (lane < k) {
+                                const uint64_t slot = (t0 + lane) & (Q.cap - 1);
+                                StealItem* it = &Q.items[slot];
+                                it->depth = d + 1;
+                                it->ei = ei | (kNoSel << 8);
+                                for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
+                                it->v[d] = s_surv[wv][d][ccount - k + lane];
+                                it->lo = 0;
+                                it->hi = kFull;
+                                __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                            donated += k;
+                            ccount -= k;
+                            idl_hint = idl_hint > k ? idl_hint - k : 0;
+                            if (lane == 0) lvl[d].cnt = ccount;
+                            WaveSync();
                         }
                     }
                     const uint32_t x = Uniform(s_surv[wv][d][pos]);
                     if (lane == 0) {
                         lvl[d].pos = pos + 1;
                         matched[d] = x;
-                        lvl[d + 1] = DfsLevel{0, 0, 0, kFull};
+                        lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel, 0};
                     }
                     WaveSync();
                     d++;
                     continue;
                 }
             }
-            const uint32_t ud = p.go_vs[d];
-            const uint32_t pq = p.go_bni[p.go_bni_offs[d]];
-            const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
-            const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
-            const uint32_t total = L.bl + L.al;
+            // Scan the next 64 candidates of position d.
+            uint32_t sel = Uniform(lvl[d].sel);
+            if (sel == kNoSel) {
+                sel = sets ? ChooseList(T, ei, d, matched) : 0;
+                if (lane == 0) lvl[d].sel = sel;
+            }
+            const List L = BnList(T, ei, d, sel, matched);
+            const uint32_t total = Uniform(L.bl + L.al);
             const uint32_t off = Uniform(lvl[d].off);
             uint32_t limit = Uniform(lvl[d].end);
             limit = limit < total ? limit : total;
@@ -463,7 +546,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                         StealItem* it = &Q.items[slot];
                         it->depth = d;
-                        it->ei = ei;
+                        it->ei = ei | (sel << 8);
                         for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
                         const uint32_t plo = off + (lane + 1) * piece;
                         it->lo = plo;
@@ -477,13 +560,23 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                     WaveSync();
                 }
             }
+            const bool folded = fold && d + 1 == last;
+            uint32_t X = 0;
+            if (folded) {
+                X = Uniform(lvl[d].x);
+                if (X == 0) {
+                    X = FoldCount(T, ei, d, matched, my_scanned, my_probes) + 1;
+                    if (lane == 0) lvl[d].x = X;
+                }
+                X -= 1;
+            }
             const uint32_t i = off + lane;
             uint32_t c = 0;
             bool ok = false;
             if (i < limit) {
                 c = At(L, i);
                 my_scanned++;
-                ok = AcceptCounted(T, ei, d, matched, c, my_probes);
+                ok = AcceptSel(T, ei, d, matched, c, sel, d, my_probes);
             }
             const uint64_t m = __ballot(ok);
             if (d == last) {
@@ -503,8 +596,21 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 WaveSync();
                 continue;
             }
+            if (folded) {
+                // c itself is excluded from X only when it belongs to X.
+                const bool in_x = ok && AcceptSel(T, ei, last, matched, c, kNoSkip, 0, my_probes);
+                count += (unsigned long long)__popcll(m) * X - (unsigned long long)__popcll(__ballot(in_x));
+                if (lane == 0) lvl[d].off = off + kWave;
+                WaveSync();
+                continue;
+            }
             if (ok) s_surv[wv][d][RankBelow(m)] = c;
-            if (lane == 0) lvl[d] = DfsLevel{off + kWave, (uint32_t)__popcll(m), 0, limit};
+            if (lane == 0) {
+                lvl[d].off = off + kWave;
+                lvl[d].cnt = (uint32_t)__popcll(m);
+                lvl[d].pos = 0;
+                lvl[d].end = limit;
+            }
             WaveSync();
         }
         if (lane == 0) {

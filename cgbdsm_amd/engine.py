@@ -145,6 +145,10 @@ class DeviceQuery:
     def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=4):
         _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
 
+    def set_dfs_options(self, shortest_list=True, fold_last=True):
+        """Count-mode refinements of the final DFS (both on by default)."""
+        _check(self.L.cgbdsm_set_dfs_options(self.h, int(bool(shortest_list)), int(bool(fold_last))))
+
     def export(self, which):
         n = C.c_uint64()
         _check(self.L.cgbdsm_index_export(self.h, which, None, 0, C.byref(n)))

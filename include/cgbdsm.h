@@ -184,6 +184,16 @@ cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q);
 cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit,
                                       uint64_t frontier_cap, uint32_t donate_min);
 
+/* Final-DFS refinements, both on by default; counts are identical either
+ * way (no reference counterpart: the reference always scans the first
+ * backward neighbour's list, kernels/enumeration.cu:884-1000).
+ * shortest_list: each position scans its shortest backward-neighbour list
+ *   when the batch's update lists hold no duplicate value.
+ * fold_last: when the last position's backward neighbours all precede
+ *   position qv-2, the count of the last position is computed once per
+ *   partial match of qv-2 positions instead of being enumerated. */
+cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int fold_last);
+
 /* Cooperative deadline for the final DFS of cgbdsm_matching (default 120 s,
  * 0 = none).
  * Past the deadline the host raises a flag the kernel polls; the call then

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from cgbdsm_amd.engine import CgbdsmError, Engine
-from cgbdsm_amd.host import HostSession, synth_spec
+from cgbdsm_amd.host import HostSession, generate_files, synth_spec
 from tests import oracle_lib as O
 from tests import parity
 
@@ -69,18 +69,42 @@ def test_tuples_parity(eng):
     assert res["mismatches"] == 0
 
 
-@pytest.mark.parametrize("params", [
-    {"bfs_row_limit": 0},                          # final DFS straight from the seeds
-    {"bfs_row_limit": 1 << 40},                    # BFS down to the last level
-    {"bfs_row_limit": 1 << 40, "frontier_cap": 32},  # overflow -> DFS from the last good level
-    {"bfs_row_limit": 0, "donate_min": 2},         # aggressive work donation between wavefronts
-    {"bfs_row_limit": 0, "donate_min": 0},         # no donation
-], ids=["dfs-only", "bfs-deep", "frontier-overflow", "steal-eager", "steal-off"])
-def test_matching_paths(eng, params):
-    print(f"case paths {params}", flush=True)
+@pytest.mark.parametrize("params,dfs", [
+    ({"bfs_row_limit": 0}, None),                          # final DFS straight from the seeds
+    ({"bfs_row_limit": 1 << 40}, None),                    # BFS down to the last level
+    ({"bfs_row_limit": 1 << 40, "frontier_cap": 32}, None),  # overflow -> DFS from the last good level
+    ({"bfs_row_limit": 0, "donate_min": 2}, None),         # aggressive work donation between wavefronts
+    ({"bfs_row_limit": 0, "donate_min": 0}, None),         # no donation
+    ({"bfs_row_limit": 0}, {"shortest_list": False, "fold_last": False}),  # first-list scans, no folding
+    ({"bfs_row_limit": 0}, {"shortest_list": True, "fold_last": False}),   # shortest list, no folding
+], ids=["dfs-only", "bfs-deep", "frontier-overflow", "steal-eager", "steal-off", "first-list", "no-fold"])
+def test_matching_paths(eng, params, dfs):
+    print(f"case paths {params} {dfs}", flush=True)
     spec = synth_spec(11, 8, 4, seed=8, query_size=6, query_kind="dense", query_seed=8)
     host = HostSession(spec=spec, batch_edges=300, threads=4)
-    res = parity.run_stream(host, eng, match_params=params, check_index=False, check_masks=False, log=_log)
+    res = parity.run_stream(host, eng, match_params=params, dfs_options=dfs, check_index=False, check_masks=False,
+                            log=_log)
+    assert res["mismatches"] == 0
+
+
+def test_duplicate_stream_pairs(eng, tmp_path):
+    """Update streams that repeat pairs inside one batch: the update lists
+    then hold duplicates, which the reference counts once per occurrence
+    (its U lists are not uniqued); the DFS must keep to first-list scans."""
+    print("case duplicate pairs", flush=True)
+    spec = synth_spec(10, 8, 3, seed=12, query_size=5, query_kind="dense", query_seed=12)
+    qp, dp, up = tmp_path / "q.graph", tmp_path / "d.graph", tmp_path / "u.txt"
+    generate_files(spec, qp, dp, up)
+    lines = up.read_text().splitlines()
+    dup = []
+    for i, ln in enumerate(lines):
+        dup.append(ln)
+        if i % 7 == 0:
+            dup.append(ln)
+    up.write_text("\n".join(dup) + "\n")
+    host = HostSession(files=(qp, dp, up), batch_edges=200, threads=4)
+    res = parity.run_stream(host, eng, log=_log)
+    assert res["batches"] > 0
     assert res["mismatches"] == 0
 
 
