@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--deadline-ms", type=float, default=60_000)
     ap.add_argument("--donate-min", type=int, default=4, help="survivors left before a DFS wavefront donates (0: off)")
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
+    ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
+    ap.add_argument("--no-fold", action="store_true", help="final DFS enumerates the last position")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -135,6 +137,7 @@ def main():
     q.set_plan(plan)
     q.set_deadline(args.deadline_ms)
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
+    q.set_dfs_options(shortest_list=not args.no_shortest_list, fold_last=not args.no_fold)
     for kb in batches[: warm + steps]:
         q.enqueue(kb)
 
@@ -148,6 +151,7 @@ def main():
     q.sync()
     t0 = time.perf_counter()
     stage_ms = {"apply": 0.0, "update_index": 0.0, "masks": 0.0, "match": 0.0, "commit": 0.0}
+    dfs = {"rows": 0, "scanned": 0, "probes": 0, "donated": 0}
     for i in range(steps):
         tb = time.perf_counter()
         q.stage_next()
@@ -157,6 +161,13 @@ def main():
         st = q.stats()
         for k in stage_ms:
             stage_ms[k] += st["ms_" + k]
+        dfs["rows"] += st["last_frontier_rows"]
+        dfs["scanned"] += st["last_dfs_scanned"]
+        dfs["probes"] += st["last_dfs_probes"]
+        dfs["donated"] += st["last_dfs_donated"]
+        log(f"batch {warm + i}: {c} matches in {lat_ms[-1]:.1f} ms, frontier {st['last_frontier_rows']} rows at "
+            f"depth {st['last_frontier_depth']}, scanned {st['last_dfs_scanned']}, probes {st['last_dfs_probes']}, "
+            f"donated {st['last_dfs_donated']}")
     q.sync()
     elapsed = time.perf_counter() - t0
     kt = q.kernel_times()
@@ -201,6 +212,7 @@ def main():
         "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
         "s1_cpu_ms_per_batch": round(statistics.mean(s1_ms), 3),
         "kernels_ms": {k: round(v["ms"], 4) for k, v in kt.items()},
+        "dfs_totals": dfs,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -34,7 +34,8 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "batches", "dv_count", "main_entries", "pool_used", "pool_cap", "last_ins", "last_del",
         "last_upd", "last_rows_touched", "last_idx_bytes", "last_frontier_rows",
-        "last_frontier_depth", "last_matches", "compactions")] + [
+        "last_frontier_depth", "last_matches", "compactions", "last_dfs_scanned", "last_dfs_probes",
+        "last_dfs_donated")] + [
         (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit")]
 
 

@@ -394,6 +394,7 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
 // batch fails with CGBDSM_ERR_OVERFLOW (its count would be incomplete).
 void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
     SetCounter(q, &Counters::flags, 0);
+    LaunchQueueSync(q->d_cnt, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_rows.v, 0, 1, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_pending.v, n, 1, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_idle.v, 0, 1, q->s);
@@ -913,6 +914,9 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
         if (q->h_cnt->flags & (kFlagSteal | kFlagAborted))
             Throw(CGBDSM_ERR_OVERFLOW, "final DFS did not complete (flags " + std::to_string(q->h_cnt->flags) + ")");
         if (n) KAdd(q, CGBDSM_K_DFS, 4ull * (q->h_cnt->scanned + q->h_cnt->probes), n);
+        q->st.last_dfs_scanned = q->h_cnt->scanned;
+        q->st.last_dfs_probes = q->h_cnt->probes;
+        q->st.last_dfs_donated = q->h_cnt->donated;
         unsigned long long total = 0;
         for (uint32_t e = 0; e < q->qe; e++) {
             total += q->h_cnt->per_edge[e];
@@ -1091,7 +1095,6 @@ cgbdsm_status cgbdsm_stats_get(cgbdsm_query* q, cgbdsm_stats* out) {
         q->st.pool_used = q->h_cnt->pool_top;
         q->st.pool_cap = q->pool.cap;
         q->st.last_rows_touched = q->h_cnt->rows_touched;
-        q->st.last_idx_bytes = q->h_cnt->bytes;
         *out = q->st;
     });
 }

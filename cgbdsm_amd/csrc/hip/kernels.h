@@ -60,4 +60,6 @@ uint32_t DfsBlocksPerCU();
 
 void LaunchCountDups(const uint64_t* k, uint64_t n, Counters* cnt, hipStream_t s);
 
+void LaunchQueueSync(Counters* cnt, hipStream_t s);
+
 }  // namespace cgd

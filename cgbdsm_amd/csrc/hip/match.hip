@@ -194,6 +194,25 @@ __device__ __forceinline__ unsigned long long LoadAgent(const unsigned long long
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Steal-queue items cross XCDs, whose L2s are not coherent with each other.
+// Agent-scope acquire/release fences would keep them coherent by writing
+// back or invalidating the whole L2 at every hand-off, which starves every
+// other wave on the XCD.  Instead every item word is written and read with
+// agent-scope atomic accesses (coherent without cache maintenance) and
+// ordered with wait-count fences: the donor's item stores complete before
+// its sequence store, and the thief reads the item only after seeing it.
+__device__ __forceinline__ void PutAgent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t GetAgent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void PublishItem(StealQueue& Q, uint64_t slot, unsigned long long ticket) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_store(&Q.seq[slot], ticket + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // List of backward neighbour k of position d (owner = its matched vertex);
 // k must be the same in every lane.
 __device__ __forceinline__ List BnList(const Tables& T, uint32_t ei, uint32_t d, uint32_t k, const uint32_t* matched) {
@@ -332,8 +351,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     const bool sets = Q.shortest && Uniform64(cnt->u_dups) == 0;
     const bool fold = !kTuples && sets && Q.fold && T.P->fold_last;
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
-    uint64_t r_next = 0, r_end = 0;
-    bool primary = true, idle = false, stop = false;
+    uint64_t r_next = 0, r_end = 0, ticket = 0;
+    bool primary = true, idle = false, stop = false, have_ticket = false;
     uint32_t iter = 0, backoff = 1;
     // Shared words (idle count, flags) and the host flag are read at most
     // once per poll_ticks / host_ticks of wall clock per wave: thousands of
@@ -385,58 +404,48 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
         }
         if (!got) {
-            unsigned long long h = 0;
-            int ok = 0;
-            if (lane == 0) {
-                h = LoadAgent(&cnt->q_head.v);
-                const unsigned long long t = LoadAgent(&cnt->q_tail.v);
-                if (h < t && atomicCAS(&cnt->q_head.v, h, h + 1) == h) ok = 1;
-            }
-            ok = __shfl(ok, 0, kWave);
-            if (ok) {
-                h = Uniform64(__shfl(h, 0, kWave));
-                const uint64_t slot = h & (Q.cap - 1);
+            // One ticket per idle wave (fetch-and-add, no retries): the wave
+            // then waits on its own slot until a donor fills it or no work
+            // is left anywhere (`pending` = rows and items not yet finished;
+            // donations only come from waves holding one, so none can
+            // arrive once it is zero).
+            if (!have_ticket) {
+                unsigned long long tk = 0;
                 if (lane == 0) {
-                    uint32_t spins = 0;
-                    // Relaxed spin, one acquire fence after: an acquire per
-                    // iteration would invalidate this CU's L1 each time.
-                    while (LoadAgent(&Q.seq[slot]) != h + 1) {
-                        __builtin_amdgcn_s_sleep(2);
-                        if (++spins > (1u << 24)) {
-                            atomicOr(&cnt->flags, kFlagSteal);
-                            break;
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    tk = atomicAdd(&cnt->q_head.v, 1ull);
+                    if (!idle) atomicAdd(&cnt->q_idle.v, 1ull);
                 }
-                WaveSync();
-                const StealItem* it = &Q.items[slot];
-                D0 = Uniform(it->depth);
-                const uint32_t tag = Uniform(it->ei);
+                ticket = Uniform64(__shfl(tk, 0, kWave));
+                have_ticket = true;
+                idle = true;
+            }
+            const uint64_t slot = ticket & (Q.cap - 1);
+            unsigned long long sq = 0;
+            if (lane == 0) sq = LoadAgent(&Q.seq[slot]);
+            sq = __shfl(sq, 0, kWave);
+            if (sq == ticket + 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                StealItem* it = &Q.items[slot];
+                D0 = Uniform(GetAgent(&it->depth));
+                const uint32_t tag = Uniform(GetAgent(&it->ei));
                 ei = tag & 0xFFu;
                 sel0 = tag >> 8;
-                lo0 = Uniform(it->lo);
-                hi0 = Uniform(it->hi);
-                if (lane < D0) matched[lane] = it->v[lane];
+                lo0 = Uniform(GetAgent(&it->lo));
+                hi0 = Uniform(GetAgent(&it->hi));
+                if (lane < D0) matched[lane] = GetAgent(&it->v[lane]);
                 got = true;
+                have_ticket = false;
                 backoff = 1;
-                if (idle) {
-                    if (lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
-                    idle = false;
-                }
+                if (lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
+                idle = false;
             } else {
                 unsigned long long pend = 0;
                 if (lane == 0) pend = LoadAgent(&cnt->q_pending.v);
                 pend = __shfl(pend, 0, kWave);
                 if (pend == 0) break;
-                if (!idle) {
-                    if (lane == 0) atomicAdd(&cnt->q_idle.v, 1ull);
-                    idle = true;
-                }
                 if (poll()) break;
-                // Exponential backoff keeps idle pollers off the queue lines.
-                for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(127);
-                backoff = backoff < 32 ? backoff * 2 : 32;
+                for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(63);
+                backoff = backoff < 16 ? backoff * 2 : 16;
                 continue;
             }
         }
@@ -457,7 +466,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 uint32_t ccount = Uniform(lvl[d].cnt);
                 if (pos < ccount) {
                     // Donate the tail of this level when others are idle.
-                    if (Q.donate_min && idl_hint > 0 && d + 2 <= last && ccount - pos >= Q.donate_min) {
+                    // Items handed out hold at least one full level below them.
+                    if (Q.donate_min && idl_hint > 0 && d + 2 <= term && ccount - pos >= Q.donate_min) {
                         unsigned long long h = 0, t = 0;
                         if (lane == 0) {
                             h = LoadAgent(&cnt->q_head.v);
@@ -468,7 +478,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         uint32_t k = ccount - pos - 1;
                         const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
                         k = k < want ? k : want;
-                        if (k && (t - h) + k <= Q.cap / 2) {
+                        if (k && (long long)(t - h) + (long long)k <= (long long)(Q.cap / 2)) {
                             unsigned long long t0 = 0;
                             if (lane == 0) {
                                 atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
@@ -478,14 +488,13 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                             if (lane < k) {
                                 const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                                 StealItem* it = &Q.items[slot];
-                                it->depth = d + 1;
-                                it->ei = ei | (kNoSel << 8);
-                                for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
-                                it->v[d] = s_surv[wv][d][ccount - k + lane];
-                                it->lo = 0;
-                                it->hi = kFull;
-                                __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
+                                PutAgent(&it->depth, d + 1);
+                                PutAgent(&it->ei, ei | (kNoSel << 8));
+                                for (uint32_t f = 0; f < d; f++) PutAgent(&it->v[f], matched[f]);
+                                PutAgent(&it->v[d], s_surv[wv][d][ccount - k + lane]);
+                                PutAgent(&it->lo, 0);
+                                PutAgent(&it->hi, kFull);
+                                PublishItem(Q, slot, t0 + lane);
                             }
                             donated += k;
                             ccount -= k;
@@ -533,7 +542,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 const uint32_t rem = limit - off;
                 uint32_t parts = (uint32_t)(idl_hint < 15 ? idl_hint + 1 : 16);
                 if (parts > rem / kRangeMin) parts = rem / kRangeMin;
-                if (parts >= 2 && (t - h) + parts <= Q.cap / 2) {
+                if (parts >= 2 && (long long)(t - h) + (long long)parts <= (long long)(Q.cap / 2)) {
                     const uint32_t piece = ((rem + parts - 1) / parts + kWave - 1) & ~(uint32_t)(kWave - 1);
                     const uint32_t k = (rem - 1) / piece;  // donated pieces after the kept one
                     unsigned long long t0 = 0;
@@ -545,13 +554,13 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                     if (lane < k) {
                         const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                         StealItem* it = &Q.items[slot];
-                        it->depth = d;
-                        it->ei = ei | (sel << 8);
-                        for (uint32_t f = 0; f < d; f++) it->v[f] = matched[f];
+                        PutAgent(&it->depth, d);
+                        PutAgent(&it->ei, ei | (sel << 8));
+                        for (uint32_t f = 0; f < d; f++) PutAgent(&it->v[f], matched[f]);
                         const uint32_t plo = off + (lane + 1) * piece;
-                        it->lo = plo;
-                        it->hi = plo + piece < limit ? plo + piece : limit;
-                        __hip_atomic_store(&Q.seq[slot], t0 + lane + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        PutAgent(&it->lo, plo);
+                        PutAgent(&it->hi, plo + piece < limit ? plo + piece : limit);
+                        PublishItem(Q, slot, t0 + lane);
                     }
                     donated += k;
                     idl_hint = idl_hint > k ? idl_hint - k : 0;
@@ -628,6 +637,17 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     }
 }
 
+// Before a DFS launch: tickets taken but never filled in the previous launch
+// leave head above tail; both restart at the larger value, so tickets (and
+// the sequence numbers that publish them) are never reused.
+__global__ void KQueueSync(Counters* __restrict__ cnt) {
+    if (threadIdx.x == 0) {
+        const unsigned long long h = cnt->q_head.v, t = cnt->q_tail.v, m = h > t ? h : t;
+        cnt->q_head.v = m;
+        cnt->q_tail.v = m;
+    }
+}
+
 unsigned Grid1(uint64_t n, unsigned per, unsigned cap) {
     uint64_t g = (n + per - 1) / per;
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
@@ -647,6 +667,8 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
 }
 
 uint32_t DfsBlocksPerCU() { return 8; }
+
+void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>(cnt); }
 
 void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {

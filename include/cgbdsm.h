@@ -92,6 +92,9 @@ typedef struct {
     uint64_t last_frontier_depth;
     uint64_t last_matches;
     uint64_t compactions;
+    uint64_t last_dfs_scanned;    /* candidate list entries read by the final DFS */
+    uint64_t last_dfs_probes;     /* binary-search probes of the final DFS        */
+    uint64_t last_dfs_donated;    /* work items handed between DFS wavefronts     */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
 } cgbdsm_stats;
 
