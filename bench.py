@@ -77,6 +77,22 @@ def cpu_baseline(keys, dv, plan, batches, budget_s, threads):
     return edges, secs, counts
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy-kernel
+    calibration), or None when no summary covers this workload."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*.traffic.txt")):
+        kv = {}
+        for ln in open(path):
+            if ":" in ln and not ln.startswith("#"):
+                k, v = ln.split(":", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("kernel") == kernel and kv.get("workload") == workload:
+            return int(kv["traffic_bytes_per_launch"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,8 +154,12 @@ def main():
     q.set_deadline(args.deadline_ms)
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
     q.set_dfs_options(shortest_list=not args.no_shortest_list, fold_last=not args.no_fold)
+    enq_ms = []
     for kb in batches[: warm + steps]:
+        t_e = time.perf_counter()
         q.enqueue(kb)
+        q.sync()
+        enq_ms.append(1e3 * (time.perf_counter() - t_e))
 
     counts, lat_ms = [], []
     for i in range(warm):
@@ -182,8 +202,11 @@ def main():
     launch_ms = d["ms"] / max(d["launches"], 1)
     bytes_per_launch = d["bytes"] / max(d["launches"], 1)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    workload = (f"rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g} "
+                f"90/10 stream, {args.batch_edges}-edge batches, plain single query: "
+                f"{args.query_size}-vertex {args.query_kind} random-walk query seed {args.query_seed}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, workload), "kernel": dom,
                 "launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
     cpu = None
@@ -202,12 +225,13 @@ def main():
         "metric": METRIC, "value": round(value, 3), "unit": "update edges/s", "n_gpus": 1, "steps": steps,
         "warmup": warm, "ms_per_step": round(1e3 * elapsed / steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g} "
-                               f"90/10 stream, {args.batch_edges}-edge batches, plain single query: "
-                               f"{args.query_size}-vertex {args.query_kind} random-walk query seed {args.query_seed}",
+        "config": {"workload": workload,
                    "batch_edges": args.batch_edges, "query_edges": int(qedges.size // 2),
                    "parallelism": "single GPU"},
         "p50_batch_ms": round(percentile(lat_ms, 50), 4), "p99_batch_ms": round(percentile(lat_ms, 99), 4),
+        # Host keys -> HBM over PCIe (cgbdsm_batch_enqueue), outside the timed region.
+        "enqueue_ms_per_batch": round(statistics.mean(enq_ms[warm:]), 4),
+        "pcie_inclusive_update_edges_per_s": round(edges / (elapsed + 1e-3 * sum(enq_ms[warm:])), 3),
         "matches": int(matches), "matches_per_s": round(matches / elapsed, 3),
         "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
         "s1_cpu_ms_per_batch": round(statistics.mean(s1_ms), 3),
