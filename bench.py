@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--donate-min", type=int, default=4, help="survivors left before a DFS wavefront donates (0: off)")
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
-    ap.add_argument("--no-fold", action="store_true", help="final DFS enumerates the last position")
+    ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -153,7 +153,7 @@ def main():
     q.set_plan(plan)
     q.set_deadline(args.deadline_ms)
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
-    q.set_dfs_options(shortest_list=not args.no_shortest_list, fold_last=not args.no_fold)
+    q.set_dfs_options(shortest_list=not args.no_shortest_list, product_suffix=not args.no_product)
     enq_ms = []
     for kb in batches[: warm + steps]:
         t_e = time.perf_counter()

@@ -21,7 +21,7 @@ class Plan(C.Structure):
         ("dir_to_edge", C.c_uint8 * MAX_DIR),
         ("qe_u", C.c_uint8 * MAX_QE), ("qe_v", C.c_uint8 * MAX_QE),
         ("go_vs", C.c_uint8 * MAX_QV), ("go_bni_offs", C.c_uint8 * (MAX_QV + 1)),
-        ("go_bni", C.c_uint8 * MAX_DIR), ("go_cp", C.c_uint8 * MAX_QV),
+        ("go_bni", C.c_uint8 * MAX_DIR), ("go_cp", C.c_uint8 * MAX_QV), ("go_lcls", C.c_uint8 * MAX_QV),
         ("io_vs", (C.c_uint8 * MAX_QV) * MAX_QE),
         ("io_bni_offs", (C.c_uint8 * (MAX_QV + 1)) * MAX_QE),
         ("io_bni", (C.c_uint8 * MAX_DIR) * MAX_QE),

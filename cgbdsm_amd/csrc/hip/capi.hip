@@ -52,7 +52,7 @@ struct cgbdsm_query {
     uint8_t edge_ok[kMaxQE] = {};
     uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
     uint32_t donate_min = 4;
-    bool dfs_shortest = true, dfs_fold = true;
+    bool dfs_shortest = true, dfs_suffix = true;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;
@@ -382,7 +382,7 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
     Q.cap = q->steal_cap;
     Q.donate_min = donate ? q->donate_min : 0;
     Q.shortest = q->dfs_shortest ? 1 : 0;
-    Q.fold = q->dfs_fold ? 1 : 0;
+    Q.suffix = q->dfs_suffix ? 1 : 0;
     Q.abort = q->d_abort;
     Q.poll_ticks = std::max<long long>(1, q->ctx->wall_khz / 20);  // 50 us
     Q.host_ticks = (long long)q->ctx->wall_khz * 2;              // 2 ms
@@ -569,15 +569,44 @@ cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
         q->hplan.p = *plan;
         for (uint32_t e = 0; e < q->qe; e++)
             for (uint32_t v = 0; v < q->qv; v++) q->hplan.all_ones[e][v] = plan->rebuild_v[e][v] ? 0 : 1;
-        // Position qv-1 can be folded into qv-2 when none of its backward
-        // neighbours is position qv-2 (its candidates then do not depend on it).
-        q->hplan.fold_last = 0;
-        if (plan->qv >= 3) {
-            const uint32_t last = plan->qv - 1;
-            bool fold = plan->go_bni_offs[last + 1] > plan->go_bni_offs[last];
-            for (uint32_t k = plan->go_bni_offs[last]; k < plan->go_bni_offs[last + 1]; k++)
-                if (plan->go_bni[k] >= last - 1) fold = false;
-            q->hplan.fold_last = fold ? 1 : 0;
+        // Product suffix: the longest tail of the order whose positions are
+        // mutually non-adjacent (every backward neighbour precedes the tail),
+        // as Plan::GenerateCartesianProductInfoGlobal (graph/plan.cpp:899-952)
+        // classifies it, shortened until no label class in it has more than
+        // three members (the closed forms cover classes of up to 3).
+        const uint32_t qv = plan->qv, last = qv - 1;
+        uint32_t cps = qv;
+        for (uint32_t st = last; st >= 2; st--) {
+            bool ok = true;
+            for (uint32_t j = st; j <= last && ok; j++)
+                for (uint32_t k = plan->go_bni_offs[j]; k < plan->go_bni_offs[j + 1]; k++)
+                    if (plan->go_bni[k] >= st) ok = false;
+            if (!ok) break;
+            cps = st;
+        }
+        for (; cps < qv; cps++) {
+            uint32_t worst = 0;
+            for (uint32_t j = cps; j <= last; j++) {
+                uint32_t m = 0;
+                for (uint32_t i = cps; i <= last; i++) m += plan->go_lcls[i] == plan->go_lcls[j];
+                worst = std::max(worst, m);
+            }
+            if (worst <= 3) break;
+        }
+        q->hplan.cp_start = cps;
+        for (uint32_t j = 0; j < (uint32_t)kMaxQV; j++) {
+            uint32_t mb = 0;
+            if (j < qv)
+                for (uint32_t k = plan->go_bni_offs[j]; k < plan->go_bni_offs[j + 1]; k++)
+                    mb = std::max<uint32_t>(mb, plan->go_bni[k]);
+            q->hplan.maxbn[j] = (uint8_t)mb;
+        }
+        for (uint32_t x = 0; x < (uint32_t)kMaxQV; x++) {
+            uint32_t m = 0;
+            for (uint32_t j = 0; j < qv && x < qv; j++)
+                for (uint32_t k = plan->go_bni_offs[j]; k < plan->go_bni_offs[j + 1]; k++)
+                    if (plan->go_bni[k] == x) m |= 1u << j;
+            q->hplan.dep_mask[x] = m;
         }
         UploadPlan(q);
         q->have_order = true;
@@ -847,11 +876,11 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit, u
     });
 }
 
-cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int fold_last) {
+cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int product_suffix) {
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
         q->dfs_shortest = shortest_list != 0;
-        q->dfs_fold = fold_last != 0;
+        q->dfs_suffix = product_suffix != 0;
     });
 }
 

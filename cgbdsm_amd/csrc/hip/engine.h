@@ -90,7 +90,7 @@ struct StealQueue {
     long long poll_ticks;     // wall-clock ticks between a wave's reads of shared queue words
     long long host_ticks;     // wall-clock ticks between a wave's reads of the host flag
     uint32_t shortest;        // scan the shortest backward-neighbour list when lists are sets
-    uint32_t fold;            // fold the last position into its predecessor where the plan allows
+    uint32_t suffix;          // count the product suffix of the order in closed form
 };
 
 // Per-query parameter block in device memory (read through scalar loads).
@@ -102,7 +102,9 @@ struct DevPlan {
     uint8_t sa[kMaxQE], sb[kMaxQE];    // the two directions of edge ei
     uint32_t tag_shift;                // frontier word 0 = ei << tag_shift | v0
     uint32_t v_mask;                   // (1 << tag_shift) - 1
-    uint32_t fold_last;                // the last position's backward neighbours all precede qv-2
+    uint32_t cp_start;                 // first position of the product suffix (qv: none)
+    uint8_t maxbn[kMaxQV];             // largest backward-neighbour position of each position
+    uint32_t dep_mask[kMaxQV];         // positions whose candidate set depends on position x
 };
 
 // Device pointers of one query's index, passed by value to kernels.

@@ -178,7 +178,6 @@ __global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __
 struct DfsLevel {
     uint32_t off, cnt, pos, end;  // scan offset, survivors, next survivor, range end (exclusive)
     uint32_t sel;                 // backward neighbour whose list is scanned (kNoSel: not chosen yet)
-    uint32_t x;                   // folded level: |X| + 1 once computed, 0 before
 };
 
 constexpr uint32_t kFull = 0xFFFFFFFFu;
@@ -289,36 +288,144 @@ __device__ __forceinline__ uint32_t ChooseList(const Tables& T, uint32_t ei, uin
     return Uniform(key) & 15u;
 }
 
-// |X| for the folded level d = last - 1: the candidates x of position `last`
-// (support-mask bit, every backward-neighbour list) distinct from positions
-// 0..d-1.  The last position's backward neighbours all precede d, so X is
-// the same for every candidate of position d.
-__device__ uint32_t FoldCount(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, unsigned long long& scanned,
-                              unsigned long long& probes) {
+// Size of C_B = the intersection of the candidate sets of the positions in
+// B (bit mask, all at or after j0): every backward-neighbour list of every
+// position of B, and the support-mask bit of each.  `raw` leaves the
+// positions 0..j0-1 in; `hits` counts those of them that lie in C_B (only
+// positions of the same label class can).  All lists must be sets.
+__device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B, uint32_t j0, const uint32_t* matched, bool need_raw,
+                        uint32_t& raw, uint32_t& hits, unsigned long long& scanned, unsigned long long& probes) {
     const cgbdsm_plan& p = T.P->p;
-    const uint32_t last = d + 1, lane = LaneId();
-    const uint32_t nbn = p.go_bni_offs[last + 1] - p.go_bni_offs[last];
-    if (nbn == 1 && T.P->all_ones[ei][p.go_vs[last]]) {
-        // |list| minus the positions 0..d-1 that occur in it.
-        const List L = BnList(T, ei, last, 0, matched);
-        bool hit = false;
-        if (lane < d) hit = InList(L, matched[lane], probes);
-        return Uniform(L.bl + L.al) - (uint32_t)__popcll(__ballot(hit));
-    }
-    const uint32_t sel = ChooseList(T, ei, last, matched);
-    const List L = BnList(T, ei, last, sel, matched);
-    const uint32_t total = Uniform(L.bl + L.al);
-    uint32_t x = 0;
-    for (uint32_t off = 0; off < total; off += kWave) {
-        const uint32_t i = off + lane;
-        bool ok = false;
-        if (i < total) {
-            scanned++;
-            ok = AcceptSel(T, ei, last, matched, At(L, i), sel, d, probes);
+    const uint32_t lane = LaneId();
+    const uint32_t jf = (uint32_t)__builtin_ctz(B);
+    const uint32_t cls = p.go_lcls[jf];
+    // Source: the shortest list over all (position, backward neighbour) pairs.
+    uint32_t best = 0xFFFFFFFFu, src_j = jf, src_k = 0, npairs = 0;
+    for (uint32_t bb = B; bb; bb &= bb - 1) {
+        const uint32_t j = (uint32_t)__builtin_ctz(bb);
+        const uint32_t nbn = p.go_bni_offs[j + 1] - p.go_bni_offs[j];
+        for (uint32_t k = 0; k < nbn; k++, npairs++) {
+            const List L = BnList(T, ei, j, k, matched);
+            const uint32_t n = Uniform(L.bl + L.al);
+            if (n < best) {
+                best = n;
+                src_j = j;
+                src_k = k;
+            }
         }
-        x += (uint32_t)__popcll(__ballot(ok));
     }
-    return x;
+    const bool single = (B & (B - 1)) == 0 && npairs == 1 && T.P->all_ones[ei][p.go_vs[jf]];
+    auto member = [&](uint32_t c, bool skip_src) -> bool {
+        for (uint32_t bb = B; bb; bb &= bb - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctz(bb);
+            if (!TileBit(T, ei, p.go_vs[j], c)) return false;
+            const uint32_t nbn = p.go_bni_offs[j + 1] - p.go_bni_offs[j];
+            for (uint32_t k = 0; k < nbn; k++) {
+                if (skip_src && j == src_j && k == src_k) continue;
+                if (!InList(BnList(T, ei, j, k, matched), c, probes)) return false;
+            }
+        }
+        return true;
+    };
+    if (need_raw) {
+        if (single || best == 0) {
+            raw = best;
+        } else {
+            const List S = BnList(T, ei, src_j, src_k, matched);
+            uint32_t n = 0;
+            for (uint32_t off = 0; off < best; off += kWave) {
+                const uint32_t i = off + lane;
+                bool ok = false;
+                if (i < best) {
+                    scanned++;
+                    ok = member(At(S, i), true);
+                }
+                n += (uint32_t)__popcll(__ballot(ok));
+            }
+            raw = n;
+        }
+    }
+    bool hit = false;
+    if (lane < j0 && p.go_lcls[lane] == cls && raw) hit = member(matched[lane], false);
+    hits = (uint32_t)__popcll(__ballot(hit));
+}
+
+// |A_B| = |C_B| minus the positions 0..j0-1 in it, for B = {i}; the raw
+// size is cached per wave (valid bit in `cvalid`) until a position that the
+// set depends on changes.
+__device__ __forceinline__ long long SingleSize(const Tables& T, uint32_t ei, uint32_t i, uint32_t j0,
+                                                const uint32_t* matched, uint32_t* csize, uint32_t& cvalid,
+                                                unsigned long long& scanned, unsigned long long& probes) {
+    const bool cached = (cvalid >> i) & 1u;
+    uint32_t raw = cached ? Uniform(csize[i]) : 0, hits = 0;
+    SetSize(T, ei, 1u << i, j0, matched, !cached, raw, hits, scanned, probes);
+    if (!cached) {
+        if (LaneId() == 0) csize[i] = raw;
+        cvalid |= 1u << i;
+    }
+    return (long long)raw - (long long)hits;
+}
+
+__device__ __forceinline__ long long JointSize(const Tables& T, uint32_t ei, uint32_t B, uint32_t j0,
+                                               const uint32_t* matched, unsigned long long& scanned,
+                                               unsigned long long& probes) {
+    uint32_t raw = 0, hits = 0;
+    SetSize(T, ei, B, j0, matched, true, raw, hits, scanned, probes);
+    return (long long)raw - (long long)hits;
+}
+
+// Number of injective assignments of the product suffix j0..qv-1 given
+// positions 0..j0-1.  Positions of different label classes have disjoint
+// candidate sets, so the count is the product over classes of the count
+// within a class, by inclusion-exclusion over the ways members coincide:
+//   1 member:  |A|
+//   2 members: |A||B| - |AB|
+//   3 members: |A||B||C| - |AB||C| - |AC||B| - |BC||A| + 2|ABC|
+// (A_X = intersection of the members' candidate sets minus positions
+// 0..j0-1; the planner keeps classes of the suffix to 3 members).
+__device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t j0, const uint32_t* matched,
+                                          uint32_t* csize, uint32_t& cvalid, unsigned long long& scanned,
+                                          unsigned long long& probes) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t qv = p.qv;
+    uint32_t done = 0;
+    unsigned long long result = 1;
+    for (uint32_t j = j0; j < qv; j++) {
+        if ((done >> j) & 1u) continue;
+        uint32_t members = 0;
+        for (uint32_t i = j; i < qv; i++)
+            if (p.go_lcls[i] == p.go_lcls[j]) members |= 1u << i;
+        done |= members;
+        const uint32_t m0 = j;
+        const long long a = SingleSize(T, ei, m0, j0, matched, csize, cvalid, scanned, probes);
+        if (a <= 0) return 0;
+        const uint32_t rest = members & ~(1u << m0);
+        if (!rest) {
+            result *= (unsigned long long)a;
+            continue;
+        }
+        const uint32_t m1 = (uint32_t)__builtin_ctz(rest);
+        const long long b = SingleSize(T, ei, m1, j0, matched, csize, cvalid, scanned, probes);
+        if (b <= 0) return 0;
+        const long long ab = JointSize(T, ei, (1u << m0) | (1u << m1), j0, matched, scanned, probes);
+        const uint32_t rest2 = rest & ~(1u << m1);
+        __int128 nc;
+        if (!rest2) {
+            nc = (__int128)a * b - ab;
+        } else {
+            const uint32_t m2 = (uint32_t)__builtin_ctz(rest2);
+            const long long c = SingleSize(T, ei, m2, j0, matched, csize, cvalid, scanned, probes);
+            if (c <= 0) return 0;
+            const long long ac = JointSize(T, ei, (1u << m0) | (1u << m2), j0, matched, scanned, probes);
+            const long long bc = JointSize(T, ei, (1u << m1) | (1u << m2), j0, matched, scanned, probes);
+            const long long abc =
+                JointSize(T, ei, (1u << m0) | (1u << m1) | (1u << m2), j0, matched, scanned, probes);
+            nc = (__int128)a * b * c - (__int128)ab * c - (__int128)ac * b - (__int128)bc * a + 2 * (__int128)abc;
+        }
+        if (nc <= 0) return 0;
+        result *= (unsigned long long)nc;
+    }
+    return result;
 }
 
 // Final DFS.  Work items are frontier rows (taken kGrab at a time from a
@@ -332,9 +439,9 @@ __device__ uint32_t FoldCount(const Tables& T, uint32_t ei, uint32_t d, const ui
 // Count mode refinements (identical counts):
 //  * each level scans its shortest backward-neighbour list when the batch's
 //    update lists hold no duplicate value (then every list is a set);
-//  * when the last position's backward neighbours all precede position
-//    last-1, position last is never visited: a candidate c of last-1 adds
-//    |X| - [c in X] (FoldCount).
+//  * the product suffix of the order (positions cp_start..qv-1, mutually
+//    non-adjacent) is never enumerated: a partial match of the positions
+//    before it adds SuffixCount.
 template <bool kTuples>
 __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
                                                  uint32_t D, Counters* __restrict__ cnt, StealQueue Q,
@@ -347,9 +454,13 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     DfsLevel* lvl = s_lvl[wv];
     const cgbdsm_plan& p = T.P->p;
     const uint32_t qv = p.qv, last = qv - 1;
+    __shared__ uint32_t s_csize[kWpb][kMaxQV];
     // Every list is a set when the update lists hold no duplicate value.
-    const bool sets = Q.shortest && Uniform64(cnt->u_dups) == 0;
-    const bool fold = !kTuples && sets && Q.fold && T.P->fold_last;
+    const bool lists_are_sets = Uniform64(cnt->u_dups) == 0;
+    const bool sets = Q.shortest && lists_are_sets;
+    const uint32_t cps = T.P->cp_start;
+    const bool cart = !kTuples && lists_are_sets && Q.suffix && cps < p.qv;
+    uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
     uint64_t r_next = 0, r_end = 0, ticket = 0;
     bool primary = true, idle = false, stop = false, have_ticket = false;
@@ -450,7 +561,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
         }
         // ---------------- DFS from position D0
-        if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0, sel0, 0};
+        if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0, sel0};
+        cvalid = 0;
         WaveSync();
         unsigned long long count = 0;
         uint32_t d = D0;
@@ -459,8 +571,15 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 stop = true;
                 break;
             }
+            // The product suffix is counted, not enumerated.
+            if (cart && d >= cps) {
+                count += SuffixCount(T, ei, d, matched, s_csize[wv], cvalid, my_scanned, my_probes);
+                if (d == D0) break;
+                d--;
+                continue;
+            }
             // Positions below `term` descend into their survivors.
-            const uint32_t term = fold ? last - 1 : last;
+            const uint32_t term = cart ? cps : last;
             if (d < term) {
                 const uint32_t pos = Uniform(lvl[d].pos);
                 uint32_t ccount = Uniform(lvl[d].cnt);
@@ -507,8 +626,9 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                     if (lane == 0) {
                         lvl[d].pos = pos + 1;
                         matched[d] = x;
-                        lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel, 0};
+                        lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel};
                     }
+                    cvalid &= ~T.P->dep_mask[d];
                     WaveSync();
                     d++;
                     continue;
@@ -569,16 +689,6 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                     WaveSync();
                 }
             }
-            const bool folded = fold && d + 1 == last;
-            uint32_t X = 0;
-            if (folded) {
-                X = Uniform(lvl[d].x);
-                if (X == 0) {
-                    X = FoldCount(T, ei, d, matched, my_scanned, my_probes) + 1;
-                    if (lane == 0) lvl[d].x = X;
-                }
-                X -= 1;
-            }
             const uint32_t i = off + lane;
             uint32_t c = 0;
             bool ok = false;
@@ -601,14 +711,6 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         atomicOr(&cnt->flags, kFlagTuples);
                     }
                 }
-                if (lane == 0) lvl[d].off = off + kWave;
-                WaveSync();
-                continue;
-            }
-            if (folded) {
-                // c itself is excluded from X only when it belongs to X.
-                const bool in_x = ok && AcceptSel(T, ei, last, matched, c, kNoSkip, 0, my_probes);
-                count += (unsigned long long)__popcll(m) * X - (unsigned long long)__popcll(__ballot(in_x));
                 if (lane == 0) lvl[d].off = off + kWave;
                 WaveSync();
                 continue;
@@ -666,7 +768,18 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
     KExtend<<<Grid1(n_in, kWpb, 65535), kThreads, 0, s>>>(T, in, n_in, depth, out, cap_rows, cnt);
 }
 
-uint32_t DfsBlocksPerCU() { return 8; }
+uint32_t DfsBlocksPerCU() {
+    // Resident blocks of the count kernel per CU (registers bound it); the
+    // persistent grid is sized to exactly what can run at once.
+    static uint32_t cached = 0;
+    if (!cached) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KDfs<false>, kThreads, 0) != hipSuccess || nb <= 0)
+            nb = 4;
+        cached = (uint32_t)nb;
+    }
+    return cached;
+}
 
 void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>(cnt); }
 

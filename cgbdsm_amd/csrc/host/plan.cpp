@@ -404,6 +404,14 @@ void Plan::Fill(cgbdsm_plan* out, uint32_t ebits) const {
     memcpy(out->go_bni_offs, global.bni_offs, kMaxQV + 1);
     memcpy(out->go_bni, global.bni, kMaxDir);
     memcpy(out->go_cp, cp, kMaxQV);
+    for (uint32_t a = 0; a < q->vcount; a++) {
+        out->go_lcls[a] = (uint8_t)a;
+        for (uint32_t b = 0; b < a; b++)
+            if (q->vlabels[global.vs[b]] == q->vlabels[global.vs[a]]) {
+                out->go_lcls[a] = (uint8_t)b;
+                break;
+            }
+    }
     for (uint32_t i = 0; i < q->ecount; i++)
         for (uint32_t u = 0; u < q->vcount; u++) out->rebuild_v[i][u] = rebuild_v[i][u] ? 1 : 0;
 }

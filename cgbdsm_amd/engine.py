@@ -145,9 +145,9 @@ class DeviceQuery:
     def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=4):
         _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
 
-    def set_dfs_options(self, shortest_list=True, fold_last=True):
+    def set_dfs_options(self, shortest_list=True, product_suffix=True):
         """Count-mode refinements of the final DFS (both on by default)."""
-        _check(self.L.cgbdsm_set_dfs_options(self.h, int(bool(shortest_list)), int(bool(fold_last))))
+        _check(self.L.cgbdsm_set_dfs_options(self.h, int(bool(shortest_list)), int(bool(product_suffix))))
 
     def export(self, which):
         n = C.c_uint64()

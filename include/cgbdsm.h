@@ -71,6 +71,9 @@ typedef struct {
     uint8_t go_bni_offs[CGBDSM_MAX_QV + 1];
     uint8_t go_bni[CGBDSM_MAX_DIR];                /* positions, sorted by selectivity */
     uint8_t go_cp[CGBDSM_MAX_QV];                  /* global_cartesian_product_info_   */
+    uint8_t go_lcls[CGBDSM_MAX_QV];                /* label class of each position: the
+                                                      first position with the same query
+                                                      vertex label (no reference field) */
     /* per-edge indexing orders (Plan::indexing_orders_) */
     uint8_t io_vs[CGBDSM_MAX_QE][CGBDSM_MAX_QV];
     uint8_t io_bni_offs[CGBDSM_MAX_QE][CGBDSM_MAX_QV + 1];
@@ -188,14 +191,16 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit,
                                       uint64_t frontier_cap, uint32_t donate_min);
 
 /* Final-DFS refinements, both on by default; counts are identical either
- * way (no reference counterpart: the reference always scans the first
- * backward neighbour's list, kernels/enumeration.cu:884-1000).
+ * way (no reference counterpart: the reference scans the first backward
+ * neighbour's list, kernels/enumeration.cu:884-1000, and enumerates its
+ * cartesian products, kernels/cartesian_product.cu:264-427).
  * shortest_list: each position scans its shortest backward-neighbour list
  *   when the batch's update lists hold no duplicate value.
- * fold_last: when the last position's backward neighbours all precede
- *   position qv-2, the count of the last position is computed once per
- *   partial match of qv-2 positions instead of being enumerated. */
-cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int fold_last);
+ * product_suffix: the tail of the order whose positions are mutually
+ *   non-adjacent is counted in closed form (products of candidate-set
+ *   sizes, inclusion-exclusion within a label class) once per partial
+ *   match of the positions before it. */
+cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int product_suffix);
 
 /* Cooperative deadline for the final DFS of cgbdsm_matching (default 120 s,
  * 0 = none).

@@ -75,9 +75,11 @@ def test_tuples_parity(eng):
     ({"bfs_row_limit": 1 << 40, "frontier_cap": 32}, None),  # overflow -> DFS from the last good level
     ({"bfs_row_limit": 0, "donate_min": 2}, None),         # aggressive work donation between wavefronts
     ({"bfs_row_limit": 0, "donate_min": 0}, None),         # no donation
-    ({"bfs_row_limit": 0}, {"shortest_list": False, "fold_last": False}),  # first-list scans, no folding
-    ({"bfs_row_limit": 0}, {"shortest_list": True, "fold_last": False}),   # shortest list, no folding
-], ids=["dfs-only", "bfs-deep", "frontier-overflow", "steal-eager", "steal-off", "first-list", "no-fold"])
+    ({"bfs_row_limit": 0}, {"shortest_list": False, "product_suffix": False}),  # first-list scans, full enumeration
+    ({"bfs_row_limit": 0}, {"shortest_list": True, "product_suffix": False}),   # shortest list, full enumeration
+    ({"bfs_row_limit": 1 << 40}, {"shortest_list": True, "product_suffix": True}),  # products from BFS rows
+], ids=["dfs-only", "bfs-deep", "frontier-overflow", "steal-eager", "steal-off", "first-list", "no-product",
+        "bfs-product"])
 def test_matching_paths(eng, params, dfs):
     print(f"case paths {params} {dfs}", flush=True)
     spec = synth_spec(11, 8, 4, seed=8, query_size=6, query_kind="dense", query_seed=8)
