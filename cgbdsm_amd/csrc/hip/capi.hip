@@ -594,6 +594,34 @@ cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
             if (worst <= 3) break;
         }
         q->hplan.cp_start = cps;
+        // The suffix position with the most backward neighbours (two or more)
+        // keeps the intersection of its lists from all but the deepest one in
+        // LDS; it only changes when one of those shallower positions does.
+        q->hplan.stab_pos = 0xFF;
+        q->hplan.stab_var_k = 0;
+        q->hplan.stab_deps = 0;
+        {
+            uint32_t best = 1;
+            for (uint32_t j = cps; j < qv; j++) {
+                const uint32_t nbn = plan->go_bni_offs[j + 1] - plan->go_bni_offs[j];
+                if (nbn > best) {
+                    best = nbn;
+                    q->hplan.stab_pos = j;
+                }
+            }
+            if (q->hplan.stab_pos != 0xFF) {
+                const uint32_t j = q->hplan.stab_pos;
+                uint32_t mb = 0, mk = 0;
+                for (uint32_t k = plan->go_bni_offs[j]; k < plan->go_bni_offs[j + 1]; k++)
+                    if (plan->go_bni[k] >= mb) {
+                        mb = plan->go_bni[k];
+                        mk = k - plan->go_bni_offs[j];
+                    }
+                q->hplan.stab_var_k = mk;
+                for (uint32_t k = plan->go_bni_offs[j]; k < plan->go_bni_offs[j + 1]; k++)
+                    if (plan->go_bni[k] != mb) q->hplan.stab_deps |= 1u << plan->go_bni[k];
+            }
+        }
         for (uint32_t j = 0; j < (uint32_t)kMaxQV; j++) {
             uint32_t mb = 0;
             if (j < qv)

@@ -103,6 +103,9 @@ struct DevPlan {
     uint32_t tag_shift;                // frontier word 0 = ei << tag_shift | v0
     uint32_t v_mask;                   // (1 << tag_shift) - 1
     uint32_t cp_start;                 // first position of the product suffix (qv: none)
+    uint32_t stab_pos;                 // suffix position whose stable lists are cached (0xFF: none)
+    uint32_t stab_var_k;               // its backward neighbour at the deepest position
+    uint32_t stab_deps;                // positions the cached intersection depends on
     uint8_t maxbn[kMaxQV];             // largest backward-neighbour position of each position
     uint32_t dep_mask[kMaxQV];         // positions whose candidate set depends on position x
 };

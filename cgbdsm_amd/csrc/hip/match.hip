@@ -288,17 +288,124 @@ __device__ __forceinline__ uint32_t ChooseList(const Tables& T, uint32_t ei, uin
     return Uniform(key) & 15u;
 }
 
+constexpr uint32_t kStabCap = 768;  // u32 per wave for the cached stable intersection
+
+__device__ __forceinline__ bool LdsContains(const uint32_t* a, uint32_t n, uint32_t c) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (a[m] < c) lo = m + 1;
+        else hi = m;
+    }
+    return lo < n && a[lo] == c;
+}
+
+// Per-wave cache of S = the lists of position stab_pos from its shallower
+// backward neighbours, intersected, with its support-mask bit applied, in
+// LDS as two sorted runs: buf[0..la) from the source list's main part and
+// buf[la..len) from its appended update part (the parts are disjoint but
+// not ordered against each other).  state 0: stale, 1: valid, 2: larger
+// than the buffer (not cached).
+struct StabCache {
+    uint32_t* buf;
+    uint32_t len, la, state;
+};
+
+__device__ __forceinline__ bool StabContains(const StabCache& sc, uint32_t c) {
+    return LdsContains(sc.buf, sc.la, c) || LdsContains(sc.buf + sc.la, sc.len - sc.la, c);
+}
+
 // Size of C_B = the intersection of the candidate sets of the positions in
 // B (bit mask, all at or after j0): every backward-neighbour list of every
 // position of B, and the support-mask bit of each.  `raw` leaves the
 // positions 0..j0-1 in; `hits` counts those of them that lie in C_B (only
 // positions of the same label class can).  All lists must be sets.
 __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B, uint32_t j0, const uint32_t* matched, bool need_raw,
-                        uint32_t& raw, uint32_t& hits, unsigned long long& scanned, unsigned long long& probes) {
+                        uint32_t& raw, uint32_t& hits, StabCache& sc, unsigned long long& scanned,
+                        unsigned long long& probes) {
     const cgbdsm_plan& p = T.P->p;
     const uint32_t lane = LaneId();
     const uint32_t jf = (uint32_t)__builtin_ctz(B);
     const uint32_t cls = p.go_lcls[jf];
+    if (B == (1u << T.P->stab_pos)) {
+        const uint32_t vk = T.P->stab_var_k;
+        const uint32_t nbn = p.go_bni_offs[jf + 1] - p.go_bni_offs[jf];
+        if (sc.state == 0) {
+            // Build S from the shortest stable list, checking the others.
+            uint32_t best = 0xFFFFFFFFu, bk = 0;
+            for (uint32_t k = 0; k < nbn; k++) {
+                if (k == vk) continue;
+                const List L = BnList(T, ei, jf, k, matched);
+                const uint32_t n = Uniform(L.bl + L.al);
+                if (n < best) {
+                    best = n;
+                    bk = k;
+                }
+            }
+            const List S0 = BnList(T, ei, jf, bk, matched);
+            uint32_t len = 0, la = 0;
+            bool big = false;
+            for (uint32_t off = 0; off < best; off += kWave) {
+                const uint32_t i = off + lane;
+                bool ok = false;
+                uint32_t c = 0;
+                if (i < best) {
+                    scanned++;
+                    c = At(S0, i);
+                    ok = TileBit(T, ei, p.go_vs[jf], c);
+                    for (uint32_t k = 0; k < nbn && ok; k++)
+                        if (k != vk && k != bk) ok = InList(BnList(T, ei, jf, k, matched), c, probes);
+                }
+                const uint64_t m = __ballot(ok);
+                const uint32_t cnt = (uint32_t)__popcll(m);
+                if (len + cnt > kStabCap) {
+                    big = true;
+                    break;
+                }
+                if (ok) sc.buf[len + RankBelow(m)] = c;
+                len += cnt;
+                la += (uint32_t)__popcll(__ballot(ok && i < S0.bl));
+            }
+            WaveSync();
+            sc.len = len;
+            sc.la = la;
+            sc.state = big ? 2u : 1u;
+        }
+        if (sc.state == 1) {
+            const List V = BnList(T, ei, jf, vk, matched);
+            const uint32_t nv = Uniform(V.bl + V.al);
+            if (need_raw) {
+                uint32_t n = 0;
+                if (sc.len <= nv) {
+                    for (uint32_t off = 0; off < sc.len; off += kWave) {
+                        const uint32_t i = off + lane;
+                        bool ok = false;
+                        if (i < sc.len) {
+                            scanned++;
+                            ok = InList(V, sc.buf[i], probes);
+                        }
+                        n += (uint32_t)__popcll(__ballot(ok));
+                    }
+                } else {
+                    for (uint32_t off = 0; off < nv; off += kWave) {
+                        const uint32_t i = off + lane;
+                        bool ok = false;
+                        if (i < nv) {
+                            scanned++;
+                            ok = StabContains(sc, At(V, i));
+                        }
+                        n += (uint32_t)__popcll(__ballot(ok));
+                    }
+                }
+                raw = n;
+            }
+            bool hit = false;
+            if (lane < j0 && p.go_lcls[lane] == cls && raw)
+                hit = StabContains(sc, matched[lane]) && InList(V, matched[lane], probes);
+            hits = (uint32_t)__popcll(__ballot(hit));
+            return;
+        }
+    }
     // Source: the shortest list over all (position, backward neighbour) pairs.
     uint32_t best = 0xFFFFFFFFu, src_j = jf, src_k = 0, npairs = 0;
     for (uint32_t bb = B; bb; bb &= bb - 1) {
@@ -350,30 +457,6 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
     hits = (uint32_t)__popcll(__ballot(hit));
 }
 
-// |A_B| = |C_B| minus the positions 0..j0-1 in it, for B = {i}; the raw
-// size is cached per wave (valid bit in `cvalid`) until a position that the
-// set depends on changes.
-__device__ __forceinline__ long long SingleSize(const Tables& T, uint32_t ei, uint32_t i, uint32_t j0,
-                                                const uint32_t* matched, uint32_t* csize, uint32_t& cvalid,
-                                                unsigned long long& scanned, unsigned long long& probes) {
-    const bool cached = (cvalid >> i) & 1u;
-    uint32_t raw = cached ? Uniform(csize[i]) : 0, hits = 0;
-    SetSize(T, ei, 1u << i, j0, matched, !cached, raw, hits, scanned, probes);
-    if (!cached) {
-        if (LaneId() == 0) csize[i] = raw;
-        cvalid |= 1u << i;
-    }
-    return (long long)raw - (long long)hits;
-}
-
-__device__ __forceinline__ long long JointSize(const Tables& T, uint32_t ei, uint32_t B, uint32_t j0,
-                                               const uint32_t* matched, unsigned long long& scanned,
-                                               unsigned long long& probes) {
-    uint32_t raw = 0, hits = 0;
-    SetSize(T, ei, B, j0, matched, true, raw, hits, scanned, probes);
-    return (long long)raw - (long long)hits;
-}
-
 // Number of injective assignments of the product suffix j0..qv-1 given
 // positions 0..j0-1.  Positions of different label classes have disjoint
 // candidate sets, so the count is the product over classes of the count
@@ -382,10 +465,12 @@ __device__ __forceinline__ long long JointSize(const Tables& T, uint32_t ei, uin
 //   2 members: |A||B| - |AB|
 //   3 members: |A||B||C| - |AB||C| - |AC||B| - |BC||A| + 2|ABC|
 // (A_X = intersection of the members' candidate sets minus positions
-// 0..j0-1; the planner keeps classes of the suffix to 3 members).
+// 0..j0-1; the planner keeps classes of the suffix to 3 members).  Raw
+// sizes of single positions are cached per wave (valid bits in `cvalid`)
+// until a position they depend on changes.
 __device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t j0, const uint32_t* matched,
-                                          uint32_t* csize, uint32_t& cvalid, unsigned long long& scanned,
-                                          unsigned long long& probes) {
+                                          uint32_t* csize, uint32_t& cvalid, StabCache& sc,
+                                          unsigned long long& scanned, unsigned long long& probes) {
     const cgbdsm_plan& p = T.P->p;
     const uint32_t qv = p.qv;
     uint32_t done = 0;
@@ -396,32 +481,38 @@ __device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t
         for (uint32_t i = j; i < qv; i++)
             if (p.go_lcls[i] == p.go_lcls[j]) members |= 1u << i;
         done |= members;
-        const uint32_t m0 = j;
-        const long long a = SingleSize(T, ei, m0, j0, matched, csize, cvalid, scanned, probes);
-        if (a <= 0) return 0;
-        const uint32_t rest = members & ~(1u << m0);
-        if (!rest) {
-            result *= (unsigned long long)a;
-            continue;
+        const uint32_t p0 = j;
+        const uint32_t r1 = members & ~(1u << p0);
+        const uint32_t p1 = r1 ? (uint32_t)__builtin_ctz(r1) : 0;
+        const uint32_t r2 = r1 & ~(1u << p1);
+        const uint32_t p2 = r2 ? (uint32_t)__builtin_ctz(r2) : 0;
+        const uint32_t m = (uint32_t)__popc(members);
+        long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // indexed by local subset
+        bool empty = false;
+        // Singletons first (subsets 1, 2, 4), then the joint ones.
+        for (uint32_t t = 0; t < (1u << m) - 1 && !empty; t++) {
+            const uint32_t S = t < m ? (1u << t) : (t == m ? 3u : t == m + 1 ? 5u : t == m + 2 ? 6u : 7u);
+            const uint32_t B = ((S & 1u) ? 1u << p0 : 0u) | ((S & 2u) ? 1u << p1 : 0u) | ((S & 4u) ? 1u << p2 : 0u);
+            const bool single = t < m;
+            const uint32_t pi = t == 0 ? p0 : t == 1 ? p1 : p2;
+            const bool cached = single && ((cvalid >> pi) & 1u);
+            uint32_t raw = cached ? Uniform(csize[pi]) : 0, hits = 0;
+            SetSize(T, ei, B, j0, matched, !cached, raw, hits, sc, scanned, probes);
+            if (single && !cached) {
+                if (LaneId() == 0) csize[pi] = raw;
+                cvalid |= 1u << pi;
+            }
+            const long long val = (long long)raw - (long long)hits;
+            v[S] = val;
+            if (single && val <= 0) empty = true;
         }
-        const uint32_t m1 = (uint32_t)__builtin_ctz(rest);
-        const long long b = SingleSize(T, ei, m1, j0, matched, csize, cvalid, scanned, probes);
-        if (b <= 0) return 0;
-        const long long ab = JointSize(T, ei, (1u << m0) | (1u << m1), j0, matched, scanned, probes);
-        const uint32_t rest2 = rest & ~(1u << m1);
+        if (empty) return 0;
         __int128 nc;
-        if (!rest2) {
-            nc = (__int128)a * b - ab;
-        } else {
-            const uint32_t m2 = (uint32_t)__builtin_ctz(rest2);
-            const long long c = SingleSize(T, ei, m2, j0, matched, csize, cvalid, scanned, probes);
-            if (c <= 0) return 0;
-            const long long ac = JointSize(T, ei, (1u << m0) | (1u << m2), j0, matched, scanned, probes);
-            const long long bc = JointSize(T, ei, (1u << m1) | (1u << m2), j0, matched, scanned, probes);
-            const long long abc =
-                JointSize(T, ei, (1u << m0) | (1u << m1) | (1u << m2), j0, matched, scanned, probes);
-            nc = (__int128)a * b * c - (__int128)ab * c - (__int128)ac * b - (__int128)bc * a + 2 * (__int128)abc;
-        }
+        if (m == 1) nc = v[1];
+        else if (m == 2) nc = (__int128)v[1] * v[2] - v[3];
+        else
+            nc = (__int128)v[1] * v[2] * v[4] - (__int128)v[3] * v[4] - (__int128)v[5] * v[2] - (__int128)v[6] * v[1] +
+                 2 * (__int128)v[7];
         if (nc <= 0) return 0;
         result *= (unsigned long long)nc;
     }
@@ -455,6 +546,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     const cgbdsm_plan& p = T.P->p;
     const uint32_t qv = p.qv, last = qv - 1;
     __shared__ uint32_t s_csize[kWpb][kMaxQV];
+    __shared__ uint32_t s_stab[kWpb][kStabCap];
+    StabCache sc{s_stab[threadIdx.x / kWave], 0, 0, 0};
     // Every list is a set when the update lists hold no duplicate value.
     const bool lists_are_sets = Uniform64(cnt->u_dups) == 0;
     const bool sets = Q.shortest && lists_are_sets;
@@ -563,6 +656,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
         // ---------------- DFS from position D0
         if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0, sel0};
         cvalid = 0;
+        sc.state = 0;
         WaveSync();
         unsigned long long count = 0;
         uint32_t d = D0;
@@ -573,7 +667,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
             // The product suffix is counted, not enumerated.
             if (cart && d >= cps) {
-                count += SuffixCount(T, ei, d, matched, s_csize[wv], cvalid, my_scanned, my_probes);
+                count += SuffixCount(T, ei, d, matched, s_csize[wv], cvalid, sc, my_scanned, my_probes);
                 if (d == D0) break;
                 d--;
                 continue;
@@ -585,8 +679,10 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 uint32_t ccount = Uniform(lvl[d].cnt);
                 if (pos < ccount) {
                     // Donate the tail of this level when others are idle.
-                    // Items handed out hold at least one full level below them.
-                    if (Q.donate_min && idl_hint > 0 && d + 2 <= term && ccount - pos >= Q.donate_min) {
+                    // Items handed out hold at least one scanned level (or, in
+                    // product mode, one suffix count) below them.
+                    if (Q.donate_min && idl_hint > 0 && d + (cart ? 1u : 2u) <= term &&
+                        ccount - pos >= Q.donate_min) {
                         unsigned long long h = 0, t = 0;
                         if (lane == 0) {
                             h = LoadAgent(&cnt->q_head.v);
@@ -629,6 +725,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel};
                     }
                     cvalid &= ~T.P->dep_mask[d];
+                    if ((T.P->stab_deps >> d) & 1u) sc.state = 0;
                     WaveSync();
                     d++;
                     continue;
