@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--labels", type=int, default=16)
     ap.add_argument("--zipf", type=float, default=1.2)
     ap.add_argument("--batch-edges", type=int, default=100_000)
+    ap.add_argument("--initial-frac", type=float, default=0.9)
+    ap.add_argument("--stream-edges", type=int, default=0, help="edges in the update stream (0: all the rest)")
     ap.add_argument("--query-size", type=int, default=6)
     ap.add_argument("--query-kind", default="dense", choices=["dense", "tree", "sparse"])
     ap.add_argument("--query-seed", type=int, default=1)
@@ -127,7 +129,8 @@ def main():
 
     threads = min(16, os.cpu_count() or 1)
     t_setup = time.perf_counter()
-    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1, initial_frac=0.9,
+    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                      initial_frac=args.initial_frac, stream_edges=args.stream_edges,
                       query_size=args.query_size, query_kind=args.query_kind, query_seed=args.query_seed)
     host = HostSession(spec=spec, batch_edges=args.batch_edges, threads=threads)
     keys, dv, card, avg = host.initial_keys()
@@ -187,7 +190,7 @@ def main():
         dfs["donated"] += st["last_dfs_donated"]
         log(f"batch {warm + i}: {c} matches in {lat_ms[-1]:.1f} ms, frontier {st['last_frontier_rows']} rows at "
             f"depth {st['last_frontier_depth']}, scanned {st['last_dfs_scanned']}, probes {st['last_dfs_probes']}, "
-            f"donated {st['last_dfs_donated']}")
+            f"donated {st['last_dfs_donated']}, busy {st['last_dfs_busy']:.3f}")
     q.sync()
     elapsed = time.perf_counter() - t0
     kt = q.kernel_times()
@@ -202,8 +205,11 @@ def main():
     launch_ms = d["ms"] / max(d["launches"], 1)
     bytes_per_launch = d["bytes"] / max(d["launches"], 1)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    split = f"{round(100 * args.initial_frac)}/{round(100 * (1 - args.initial_frac))}"
+    if args.stream_edges:
+        split += f" ({args.stream_edges} streamed)"
     workload = (f"rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g} "
-                f"90/10 stream, {args.batch_edges}-edge batches, plain single query: "
+                f"{split} stream, {args.batch_edges}-edge batches, plain single query: "
                 f"{args.query_size}-vertex {args.query_kind} random-walk query seed {args.query_seed}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, workload), "kernel": dom,
@@ -236,6 +242,7 @@ def main():
         "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
         "s1_cpu_ms_per_batch": round(statistics.mean(s1_ms), 3),
         "kernels_ms": {k: round(v["ms"], 4) for k, v in kt.items()},
+        "kernels_gbs": {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 3) for k, v in kt.items() if v["ms"] > 0 and v["bytes"]},
         "dfs_totals": dfs,
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -36,7 +36,8 @@ class Stats(C.Structure):
         "last_upd", "last_rows_touched", "last_idx_bytes", "last_frontier_rows",
         "last_frontier_depth", "last_matches", "compactions", "last_dfs_scanned", "last_dfs_probes",
         "last_dfs_donated")] + [
-        (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit")]
+        (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit",
+                                  "last_dfs_busy")]
 
 
 class KTime(C.Structure):
@@ -116,7 +117,8 @@ def dev_lib():
     """The HIP engine; raises if it is not built (there is no CPU fallback)."""
     global _dev
     if _dev is None:
-        L = C.CDLL(_lib_path("libcgbdsm.so"))
+        # CGBDSM_DEV_LIB may name another build of the same library (tuning runs).
+        L = C.CDLL(os.environ.get("CGBDSM_DEV_LIB") or _lib_path("libcgbdsm.so"))
         P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
         L.cgbdsm_last_error.restype = C.c_char_p
         L.cgbdsm_version.restype = C.c_char_p

@@ -965,7 +965,9 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
             depth++;
         }
         // Step 5: final DFS count from the current level.
-        for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated}) SetCounter(q, f, 0);
+        for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
+                       &Counters::idle_ticks})
+            SetCounter(q, f, 0);
         RunDfs(q, q->front[cur].p, n, depth, nullptr, 0);
         ReadCounters(q);
         if (q->h_cnt->flags & (kFlagSteal | kFlagAborted))
@@ -974,6 +976,9 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
         q->st.last_dfs_scanned = q->h_cnt->scanned;
         q->st.last_dfs_probes = q->h_cnt->probes;
         q->st.last_dfs_donated = q->h_cnt->donated;
+        q->st.last_dfs_busy = q->h_cnt->wave_ticks
+                                  ? 1.0 - (double)q->h_cnt->idle_ticks / (double)q->h_cnt->wave_ticks
+                                  : 0.0;
         unsigned long long total = 0;
         for (uint32_t e = 0; e < q->qe; e++) {
             total += q->h_cnt->per_edge[e];

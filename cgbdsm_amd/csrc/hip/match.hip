@@ -21,6 +21,14 @@
 #include "kernels.h"
 #include "wave.h"
 
+// Register budget of the final DFS (occupancy against spills), a build-time
+// tuning knob: -DCGD_DFS_WAVES_PER_EU=N asks for N resident waves per SIMD.
+#if defined(CGD_DFS_WAVES_PER_EU) && CGD_DFS_WAVES_PER_EU > 0
+#define CGD_DFS_ATTR __attribute__((amdgpu_waves_per_eu(CGD_DFS_WAVES_PER_EU)))
+#else
+#define CGD_DFS_ATTR
+#endif
+
 namespace cgd {
 
 namespace {
@@ -35,15 +43,114 @@ struct List {
     uint32_t bl, al;
 };
 
-__device__ __forceinline__ bool TileBit(const Tables& T, uint32_t ei, uint32_t qv, uint32_t v) {
-    if (T.P->all_ones[ei][qv]) return true;
-    return (T.masks[((uint64_t)ei * kMaxQV + qv) * T.W + (v >> 6)] >> (v & 63)) & 1ull;
+// Plan facts the matching kernels read in their inner loops, kept where a
+// read costs no vector-memory round trip: per-block LDS tables (filled once
+// at kernel start) and uniform registers.  The plan's byte tables cannot be
+// read with scalar loads, and a vector load of a plan byte ahead of every
+// list access would put one more dependent trip to L2 on every step.
+struct PlanLds {
+    uint16_t pair[kMaxQV][kMaxQV];  // backward neighbour k of position d: dir | position << 8
+    uint64_t app[kMaxQE][2];        // edge tag ei: dirs (bits 0..127) whose undirected edge index < ei
+    uint32_t ones[kMaxQE];          // edge tag ei: query vertices whose support tile is all ones
+    uint32_t own[kMaxQE];           // edge tag ei: its two dirs, sa | sb << 8
+    uint32_t dep_mask[kMaxQV];      // positions whose candidate set has position x as backward neighbour
+    uint64_t vs, cls, nbn;          // per position, 4 bits each: query vertex, label class, backward neighbours
+};
+
+__device__ void FillPlanLds(const Tables& T, PlanLds* L) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < (uint32_t)(kMaxQV * kMaxQV); i += blockDim.x) {
+        const uint32_t d = i / kMaxQV, k = i % kMaxQV;
+        uint16_t v = 0;
+        if (d < p.qv && k < (uint32_t)(p.go_bni_offs[d + 1] - p.go_bni_offs[d])) {
+            const uint32_t bq = p.go_bni[p.go_bni_offs[d] + k];
+            v = (uint16_t)(p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]] | (bq << 8));
+        }
+        L->pair[d][k] = v;
+    }
+    for (uint32_t e = t; e < (uint32_t)kMaxQE; e += blockDim.x) {
+        uint64_t a0 = 0, a1 = 0;
+        uint32_t ones = 0, own = 0;
+        if (e < p.qe) {
+            for (uint32_t dir = 0; dir < p.ndir; dir++)
+                if (p.dir_to_edge[dir] < e) {
+                    if (dir < 64) a0 |= 1ull << dir;
+                    else a1 |= 1ull << (dir - 64);
+                }
+            for (uint32_t q = 0; q < p.qv; q++)
+                if (T.P->all_ones[e][q]) ones |= 1u << q;
+            own = (uint32_t)T.P->sa[e] | ((uint32_t)T.P->sb[e] << 8);
+        }
+        L->app[e][0] = a0;
+        L->app[e][1] = a1;
+        L->ones[e] = ones;
+        L->own[e] = own;
+    }
+    for (uint32_t x = t; x < (uint32_t)kMaxQV; x += blockDim.x) L->dep_mask[x] = T.P->dep_mask[x];
+    if (t == 0) {
+        uint64_t vs = 0, cls = 0, nbn = 0;
+        for (uint32_t d = 0; d < p.qv; d++) {
+            vs |= (uint64_t)p.go_vs[d] << (4 * d);
+            cls |= (uint64_t)p.go_lcls[d] << (4 * d);
+            nbn |= (uint64_t)(p.go_bni_offs[d + 1] - p.go_bni_offs[d]) << (4 * d);
+        }
+        L->vs = vs;
+        L->cls = cls;
+        L->nbn = nbn;
+    }
+    __syncthreads();
 }
 
-__device__ __forceinline__ List Resolve(const Tables& T, uint32_t dir, uint32_t owner, uint32_t ei) {
+// Uniform view of the plan for one edge tag.
+struct Ctx {
+    const PlanLds* L;
+    uint64_t vs, cls, nbn;
+    uint32_t qv, ei, sa, sb, ones;
+    uint64_t app0, app1;
+};
+
+__device__ __forceinline__ Ctx MakeCtx(const Tables& T, const PlanLds* L) {
+    Ctx X;
+    X.L = L;
+    X.vs = Uniform64(L->vs);
+    X.cls = Uniform64(L->cls);
+    X.nbn = Uniform64(L->nbn);
+    X.qv = T.qv;
+    X.ei = 0;
+    X.sa = X.sb = 0xFFu;
+    X.ones = 0;
+    X.app0 = X.app1 = 0;
+    return X;
+}
+
+__device__ __forceinline__ void SetTag(Ctx& X, uint32_t ei) {
+    X.ei = ei;
+    const uint32_t own = Uniform(X.L->own[ei]);
+    X.sa = own & 0xFFu;
+    X.sb = own >> 8;
+    X.ones = Uniform(X.L->ones[ei]);
+    X.app0 = Uniform64(X.L->app[ei][0]);
+    X.app1 = Uniform64(X.L->app[ei][1]);
+}
+
+__device__ __forceinline__ uint32_t Vs(const Ctx& X, uint32_t d) { return (uint32_t)(X.vs >> (4 * (d & 15u))) & 15u; }
+__device__ __forceinline__ uint32_t Cls(const Ctx& X, uint32_t d) { return (uint32_t)(X.cls >> (4 * (d & 15u))) & 15u; }
+__device__ __forceinline__ uint32_t Nbn(const Ctx& X, uint32_t d) { return (uint32_t)(X.nbn >> (4 * (d & 15u))) & 15u; }
+__device__ __forceinline__ uint32_t Pair(const Ctx& X, uint32_t d, uint32_t k) { return Uniform(X.L->pair[d][k]); }
+
+__device__ __forceinline__ bool TileBit(const Ctx& X, const Tables& T, uint32_t qv, uint32_t v) {
+    if ((X.ones >> qv) & 1u) return true;
+    return (T.masks[((uint64_t)X.ei * kMaxQV + qv) * T.W + (v >> 6)] >> (v & 63)) & 1ull;
+}
+
+// List of directed query edge `dir` at data vertex `owner` under the
+// current edge tag: the tag's own two directions read U, every other one M,
+// with U appended when that direction's undirected edge precedes the tag.
+__device__ __forceinline__ List Resolve(const Ctx& X, const Tables& T, uint32_t dir, uint32_t owner) {
     List L;
     const uint64_t r = (uint64_t)owner * T.ndir + dir;
-    if (dir == T.P->sa[ei] || dir == T.P->sb[ei]) {
+    if (dir == X.sa || dir == X.sb) {
         const URec u = T.u_rec[r];
         L.base = T.u_vals + u.off;
         L.bl = u.len;
@@ -52,7 +159,8 @@ __device__ __forceinline__ List Resolve(const Tables& T, uint32_t dir, uint32_t 
         L.base = T.pool + m.off;
         L.bl = m.len;
     }
-    if (T.P->p.dir_to_edge[dir] < ei) {
+    const bool app = dir < 64 ? ((X.app0 >> dir) & 1ull) : ((X.app1 >> (dir - 64)) & 1ull);
+    if (app) {
         const URec u = T.u_rec[r];
         L.app = T.u_vals + u.off;
         L.al = u.len;
@@ -65,18 +173,17 @@ __device__ __forceinline__ List Resolve(const Tables& T, uint32_t dir, uint32_t 
 
 __device__ __forceinline__ uint32_t At(const List& L, uint32_t i) { return i < L.bl ? L.base[i] : L.app[i - L.bl]; }
 
-// Candidate c for position d given positions 0..d-1 in `matched` (LDS, uniform).
-__device__ __forceinline__ bool Accept(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, uint32_t c) {
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t ud = p.go_vs[d];
-    if (!TileBit(T, ei, ud, c)) return false;
+// Candidate c for position d given positions 0..d-1 in `matched` (LDS,
+// uniform): support-mask bit, injective, present in the list of every
+// backward neighbour after the first (whose list supplied c).
+__device__ __forceinline__ bool Accept(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched, uint32_t c) {
+    if (!TileBit(X, T, Vs(X, d), c)) return false;
     for (uint32_t i = 0; i < d; i++)
         if (matched[i] == c) return false;
-    const uint32_t s = p.go_bni_offs[d] + 1u, e = p.go_bni_offs[d + 1];
-    for (uint32_t k = s; k < e; k++) {
-        const uint32_t bq = p.go_bni[k];
-        const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + ud];
-        const List L = Resolve(T, dir, Uniform(matched[bq]), ei);
+    const uint32_t nbn = Nbn(X, d);
+    for (uint32_t k = 1; k < nbn; k++) {
+        const uint32_t pr = Pair(X, d, k);
+        const List L = Resolve(X, T, pr & 0xFFu, Uniform(matched[pr >> 8]));
         if (!SortedContains(L.base, L.bl, c) && !(L.al && SortedContains(L.app, L.al, c))) return false;
     }
     return true;
@@ -84,13 +191,16 @@ __device__ __forceinline__ bool Accept(const Tables& T, uint32_t ei, uint32_t d,
 
 __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t* __restrict__ rows, uint64_t cap_rows,
                                                   Counters* __restrict__ cnt) {
+    __shared__ PlanLds s_plan;
+    FillPlanLds(T, &s_plan);
+    Ctx X = MakeCtx(T, &s_plan);
     const uint32_t ei = EL.ei[blockIdx.y];
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t q0 = p.go_vs[0], q1 = p.go_vs[1];
-    const uint32_t idx = p.eidx[q0 * kMaxQV + q1];
+    SetTag(X, ei);
+    const uint32_t q0 = Vs(X, 0), q1 = Vs(X, 1);
+    const uint32_t idx = Pair(X, 1, 0) & 0xFFu;
     const uint32_t lane = LaneId();
     const uint32_t tag = ei << T.P->tag_shift;
-    const bool all0 = T.P->all_ones[ei][q0];
+    const bool all0 = (X.ones >> q0) & 1u;
     const uint64_t* tile0 = T.masks + ((uint64_t)ei * kMaxQV + q0) * T.W;
     const uint64_t nw = (uint64_t)gridDim.x * kWpb;
     for (uint64_t w = (uint64_t)blockIdx.x * kWpb + threadIdx.x / kWave; w < T.W; w += nw) {
@@ -101,8 +211,8 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
         List L{nullptr, nullptr, 0, 0};
         uint32_t c = 0;
         if (ok) {
-            L = Resolve(T, idx, v, ei);
-            for (uint32_t t = 0; t < L.bl + L.al; t++) c += TileBit(T, ei, q1, At(L, t)) ? 1u : 0u;
+            L = Resolve(X, T, idx, v);
+            for (uint32_t t = 0; t < L.bl + L.al; t++) c += TileBit(X, T, q1, At(L, t)) ? 1u : 0u;
         }
         const uint32_t incl = WaveInclusiveScan(c);
         const uint32_t total = __shfl(incl, 63, kWave);
@@ -118,7 +228,7 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
         if (ok) {
             for (uint32_t t = 0; t < L.bl + L.al; t++) {
                 const uint32_t nb = At(L, t);
-                if (TileBit(T, ei, q1, nb)) {
+                if (TileBit(X, T, q1, nb)) {
                     rows[2 * pos] = tag | v;
                     rows[2 * pos + 1] = nb;
                     pos++;
@@ -132,19 +242,22 @@ __global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __
                                                     uint32_t D, uint32_t* __restrict__ out, uint64_t cap_rows,
                                                     Counters* __restrict__ cnt) {
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
+    __shared__ PlanLds s_plan;
+    FillPlanLds(T, &s_plan);
+    Ctx X = MakeCtx(T, &s_plan);
     const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
     uint32_t* matched = s_matched[wv];
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t ud = p.go_vs[D];
-    const uint32_t pq = p.go_bni[p.go_bni_offs[D]];
-    const uint32_t pe = p.eidx[p.go_vs[pq] * kMaxQV + ud];
+    const uint32_t p0 = Pair(X, D, 0);
+    const uint32_t pe = p0 & 0xFFu, pq = p0 >> 8;
+    const uint32_t tag_shift = T.P->tag_shift, v_mask = T.P->v_mask;
     const uint64_t nw = (uint64_t)gridDim.x * kWpb;
     for (uint64_t r = (uint64_t)blockIdx.x * kWpb + wv; r < n_in; r += nw) {
         const uint32_t w0 = Uniform(in[r * D]);
-        const uint32_t ei = w0 >> T.P->tag_shift;
-        if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : in[r * D + lane];
+        const uint32_t ei = w0 >> tag_shift;
+        if (ei != X.ei || r == (uint64_t)blockIdx.x * kWpb + wv) SetTag(X, ei);
+        if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : in[r * D + lane];
         WaveSync();
-        const List L = Resolve(T, pe, Uniform(matched[pq]), ei);
+        const List L = Resolve(X, T, pe, Uniform(matched[pq]));
         const uint32_t total = L.bl + L.al;
         for (uint32_t off = 0; off < total; off += kWave) {
             const uint32_t i = off + lane;
@@ -152,7 +265,7 @@ __global__ void __launch_bounds__(kThreads) KExtend(Tables T, const uint32_t* __
             bool ok = false;
             if (i < total) {
                 c = At(L, i);
-                ok = Accept(T, ei, D, matched, c);
+                ok = Accept(X, T, D, matched, c);
             }
             const uint64_t m = __ballot(ok);
             const uint32_t k = (uint32_t)__popcll(m);
@@ -214,11 +327,9 @@ __device__ __forceinline__ void PublishItem(StealQueue& Q, uint64_t slot, unsign
 
 // List of backward neighbour k of position d (owner = its matched vertex);
 // k must be the same in every lane.
-__device__ __forceinline__ List BnList(const Tables& T, uint32_t ei, uint32_t d, uint32_t k, const uint32_t* matched) {
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t bq = p.go_bni[p.go_bni_offs[d] + k];
-    const uint32_t dir = p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]];
-    return Resolve(T, dir, Uniform(matched[bq]), ei);
+__device__ __forceinline__ List BnList(const Ctx& X, const Tables& T, uint32_t d, uint32_t k, const uint32_t* matched) {
+    const uint32_t pr = Pair(X, d, k);
+    return Resolve(X, T, pr & 0xFFu, Uniform(matched[pr >> 8]));
 }
 
 // Occurrences test by binary search in both parts of a list (probes counted).
@@ -252,16 +363,15 @@ __device__ __forceinline__ bool InList(const List& L, uint32_t c, unsigned long 
 
 // Candidate c for position d: support-mask bit, distinct from positions
 // 0..ninj-1, present in the list of every backward neighbour except `skip`.
-__device__ __forceinline__ bool AcceptSel(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched, uint32_t c,
+__device__ __forceinline__ bool AcceptSel(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched, uint32_t c,
                                           uint32_t skip, uint32_t ninj, unsigned long long& probes) {
-    const cgbdsm_plan& p = T.P->p;
-    if (!TileBit(T, ei, p.go_vs[d], c)) return false;
+    if (!TileBit(X, T, Vs(X, d), c)) return false;
     for (uint32_t i = 0; i < ninj; i++)
         if (matched[i] == c) return false;
-    const uint32_t nbn = p.go_bni_offs[d + 1] - p.go_bni_offs[d];
+    const uint32_t nbn = Nbn(X, d);
     for (uint32_t k = 0; k < nbn; k++) {
         if (k == skip) continue;
-        if (!InList(BnList(T, ei, d, k, matched), c, probes)) return false;
+        if (!InList(BnList(X, T, d, k, matched), c, probes)) return false;
     }
     return true;
 }
@@ -269,15 +379,14 @@ __device__ __forceinline__ bool AcceptSel(const Tables& T, uint32_t ei, uint32_t
 // Backward neighbour of position d with the shortest visible list (lowest
 // index on ties).  Scanning it instead of the first one enumerates the same
 // set when no list holds a value twice.
-__device__ __forceinline__ uint32_t ChooseList(const Tables& T, uint32_t ei, uint32_t d, const uint32_t* matched) {
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t nbn = p.go_bni_offs[d + 1] - p.go_bni_offs[d];
+__device__ __forceinline__ uint32_t ChooseList(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched) {
+    const uint32_t nbn = Nbn(X, d);
     if (nbn <= 1) return 0;
     const uint32_t lane = LaneId();
     uint32_t key = 0xFFFFFFFFu;
     if (lane < nbn) {
-        const uint32_t bq = p.go_bni[p.go_bni_offs[d] + lane];
-        const List L = Resolve(T, p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]], matched[bq], ei);
+        const uint32_t pr = X.L->pair[d][lane];
+        const List L = Resolve(X, T, pr & 0xFFu, matched[pr >> 8]);
         const uint32_t n = L.bl + L.al;
         key = ((n < (1u << 27) ? n : (1u << 27) - 1) << 4) | lane;
     }
@@ -309,6 +418,7 @@ __device__ __forceinline__ bool LdsContains(const uint32_t* a, uint32_t n, uint3
 struct StabCache {
     uint32_t* buf;
     uint32_t len, la, state;
+    uint32_t pos, vk, deps;  // the cached position, its deepest backward neighbour, positions S depends on
 };
 
 __device__ __forceinline__ bool StabContains(const StabCache& sc, uint32_t c) {
@@ -320,29 +430,34 @@ __device__ __forceinline__ bool StabContains(const StabCache& sc, uint32_t c) {
 // position of B, and the support-mask bit of each.  `raw` leaves the
 // positions 0..j0-1 in; `hits` counts those of them that lie in C_B (only
 // positions of the same label class can).  All lists must be sets.
-__device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B, uint32_t j0, const uint32_t* matched, bool need_raw,
+__device__ __forceinline__ void SetSize(const Ctx& X, const Tables& T, uint32_t B, uint32_t j0, const uint32_t* matched, bool need_raw,
                         uint32_t& raw, uint32_t& hits, StabCache& sc, unsigned long long& scanned,
                         unsigned long long& probes) {
-    const cgbdsm_plan& p = T.P->p;
     const uint32_t lane = LaneId();
     const uint32_t jf = (uint32_t)__builtin_ctz(B);
-    const uint32_t cls = p.go_lcls[jf];
-    if (B == (1u << T.P->stab_pos)) {
-        const uint32_t vk = T.P->stab_var_k;
-        const uint32_t nbn = p.go_bni_offs[jf + 1] - p.go_bni_offs[jf];
+    const uint32_t cls = Cls(X, jf);
+    // Only earlier positions with the same label can lie in C_B.
+    const bool may_hit = __ballot(lane < j0 && Cls(X, lane) == cls) != 0;
+    if (!need_raw && !may_hit) {
+        hits = 0;
+        return;
+    }
+    if (sc.pos < kMaxQV && B == (1u << sc.pos)) {
+        const uint32_t vk = sc.vk;
+        const uint32_t nbn = Nbn(X, jf);
         if (sc.state == 0) {
             // Build S from the shortest stable list, checking the others.
             uint32_t best = 0xFFFFFFFFu, bk = 0;
             for (uint32_t k = 0; k < nbn; k++) {
                 if (k == vk) continue;
-                const List L = BnList(T, ei, jf, k, matched);
+                const List L = BnList(X, T, jf, k, matched);
                 const uint32_t n = Uniform(L.bl + L.al);
                 if (n < best) {
                     best = n;
                     bk = k;
                 }
             }
-            const List S0 = BnList(T, ei, jf, bk, matched);
+            const List S0 = BnList(X, T, jf, bk, matched);
             uint32_t len = 0, la = 0;
             bool big = false;
             for (uint32_t off = 0; off < best; off += kWave) {
@@ -352,9 +467,9 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
                 if (i < best) {
                     scanned++;
                     c = At(S0, i);
-                    ok = TileBit(T, ei, p.go_vs[jf], c);
+                    ok = TileBit(X, T, Vs(X, jf), c);
                     for (uint32_t k = 0; k < nbn && ok; k++)
-                        if (k != vk && k != bk) ok = InList(BnList(T, ei, jf, k, matched), c, probes);
+                        if (k != vk && k != bk) ok = InList(BnList(X, T, jf, k, matched), c, probes);
                 }
                 const uint64_t m = __ballot(ok);
                 const uint32_t cnt = (uint32_t)__popcll(m);
@@ -372,11 +487,16 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
             sc.state = big ? 2u : 1u;
         }
         if (sc.state == 1) {
-            const List V = BnList(T, ei, jf, vk, matched);
+            const List V = BnList(X, T, jf, vk, matched);
             const uint32_t nv = Uniform(V.bl + V.al);
             if (need_raw) {
                 uint32_t n = 0;
-                if (sc.len <= nv) {
+                // Scan S (LDS) and binary-search V in global memory, or scan V
+                // (one coalesced load per 64) and binary-search S in LDS:
+                // whichever takes fewer dependent global round trips.
+                const uint32_t it_s = (sc.len + kWave - 1) / kWave, it_v = (nv + kWave - 1) / kWave;
+                const uint32_t depth_v = 32u - (uint32_t)__builtin_clz(nv | 1u);
+                if (it_s * (depth_v + 1) < it_v) {
                     for (uint32_t off = 0; off < sc.len; off += kWave) {
                         const uint32_t i = off + lane;
                         bool ok = false;
@@ -400,7 +520,7 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
                 raw = n;
             }
             bool hit = false;
-            if (lane < j0 && p.go_lcls[lane] == cls && raw)
+            if (lane < j0 && Cls(X, lane) == cls && raw)
                 hit = StabContains(sc, matched[lane]) && InList(V, matched[lane], probes);
             hits = (uint32_t)__popcll(__ballot(hit));
             return;
@@ -410,9 +530,9 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
     uint32_t best = 0xFFFFFFFFu, src_j = jf, src_k = 0, npairs = 0;
     for (uint32_t bb = B; bb; bb &= bb - 1) {
         const uint32_t j = (uint32_t)__builtin_ctz(bb);
-        const uint32_t nbn = p.go_bni_offs[j + 1] - p.go_bni_offs[j];
+        const uint32_t nbn = Nbn(X, j);
         for (uint32_t k = 0; k < nbn; k++, npairs++) {
-            const List L = BnList(T, ei, j, k, matched);
+            const List L = BnList(X, T, j, k, matched);
             const uint32_t n = Uniform(L.bl + L.al);
             if (n < best) {
                 best = n;
@@ -421,15 +541,15 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
             }
         }
     }
-    const bool single = (B & (B - 1)) == 0 && npairs == 1 && T.P->all_ones[ei][p.go_vs[jf]];
+    const bool single = (B & (B - 1)) == 0 && npairs == 1 && ((X.ones >> Vs(X, jf)) & 1u);
     auto member = [&](uint32_t c, bool skip_src) -> bool {
         for (uint32_t bb = B; bb; bb &= bb - 1) {
             const uint32_t j = (uint32_t)__builtin_ctz(bb);
-            if (!TileBit(T, ei, p.go_vs[j], c)) return false;
-            const uint32_t nbn = p.go_bni_offs[j + 1] - p.go_bni_offs[j];
+            if (!TileBit(X, T, Vs(X, j), c)) return false;
+            const uint32_t nbn = Nbn(X, j);
             for (uint32_t k = 0; k < nbn; k++) {
                 if (skip_src && j == src_j && k == src_k) continue;
-                if (!InList(BnList(T, ei, j, k, matched), c, probes)) return false;
+                if (!InList(BnList(X, T, j, k, matched), c, probes)) return false;
             }
         }
         return true;
@@ -438,7 +558,7 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
         if (single || best == 0) {
             raw = best;
         } else {
-            const List S = BnList(T, ei, src_j, src_k, matched);
+            const List S = BnList(X, T, src_j, src_k, matched);
             uint32_t n = 0;
             for (uint32_t off = 0; off < best; off += kWave) {
                 const uint32_t i = off + lane;
@@ -453,7 +573,7 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
         }
     }
     bool hit = false;
-    if (lane < j0 && p.go_lcls[lane] == cls && raw) hit = member(matched[lane], false);
+    if (lane < j0 && Cls(X, lane) == cls && raw) hit = member(matched[lane], false);
     hits = (uint32_t)__popcll(__ballot(hit));
 }
 
@@ -468,18 +588,17 @@ __device__ __forceinline__ void SetSize(const Tables& T, uint32_t ei, uint32_t B
 // 0..j0-1; the planner keeps classes of the suffix to 3 members).  Raw
 // sizes of single positions are cached per wave (valid bits in `cvalid`)
 // until a position they depend on changes.
-__device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t j0, const uint32_t* matched,
+__device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_t j0, const uint32_t* matched,
                                           uint32_t* csize, uint32_t& cvalid, StabCache& sc,
                                           unsigned long long& scanned, unsigned long long& probes) {
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t qv = p.qv;
+    const uint32_t qv = X.qv;
     uint32_t done = 0;
     unsigned long long result = 1;
     for (uint32_t j = j0; j < qv; j++) {
         if ((done >> j) & 1u) continue;
         uint32_t members = 0;
         for (uint32_t i = j; i < qv; i++)
-            if (p.go_lcls[i] == p.go_lcls[j]) members |= 1u << i;
+            if (Cls(X, i) == Cls(X, j)) members |= 1u << i;
         done |= members;
         const uint32_t p0 = j;
         const uint32_t r1 = members & ~(1u << p0);
@@ -497,7 +616,7 @@ __device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t
             const uint32_t pi = t == 0 ? p0 : t == 1 ? p1 : p2;
             const bool cached = single && ((cvalid >> pi) & 1u);
             uint32_t raw = cached ? Uniform(csize[pi]) : 0, hits = 0;
-            SetSize(T, ei, B, j0, matched, !cached, raw, hits, sc, scanned, probes);
+            SetSize(X, T, B, j0, matched, !cached, raw, hits, sc, scanned, probes);
             if (single && !cached) {
                 if (LaneId() == 0) csize[pi] = raw;
                 cvalid |= 1u << pi;
@@ -534,7 +653,7 @@ __device__ unsigned long long SuffixCount(const Tables& T, uint32_t ei, uint32_t
 //    non-adjacent) is never enumerated: a partial match of the positions
 //    before it adds SuffixCount.
 template <bool kTuples>
-__global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
+__global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
                                                  uint32_t D, Counters* __restrict__ cnt, StealQueue Q,
                                                  uint32_t* __restrict__ tuples, uint64_t tup_cap) {
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
@@ -543,16 +662,19 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
     uint32_t* matched = s_matched[wv];
     DfsLevel* lvl = s_lvl[wv];
-    const cgbdsm_plan& p = T.P->p;
-    const uint32_t qv = p.qv, last = qv - 1;
+    __shared__ PlanLds s_plan;
+    FillPlanLds(T, &s_plan);
+    Ctx X = MakeCtx(T, &s_plan);
+    const uint32_t qv = X.qv, last = qv - 1;
+    const uint32_t tag_shift = T.P->tag_shift, v_mask = T.P->v_mask;
     __shared__ uint32_t s_csize[kWpb][kMaxQV];
     __shared__ uint32_t s_stab[kWpb][kStabCap];
-    StabCache sc{s_stab[threadIdx.x / kWave], 0, 0, 0};
+    StabCache sc{s_stab[threadIdx.x / kWave], 0, 0, 0, T.P->stab_pos, T.P->stab_var_k, T.P->stab_deps};
     // Every list is a set when the update lists hold no duplicate value.
     const bool lists_are_sets = Uniform64(cnt->u_dups) == 0;
     const bool sets = Q.shortest && lists_are_sets;
     const uint32_t cps = T.P->cp_start;
-    const bool cart = !kTuples && lists_are_sets && Q.suffix && cps < p.qv;
+    const bool cart = !kTuples && lists_are_sets && Q.suffix && cps < qv;
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
     uint64_t r_next = 0, r_end = 0, ticket = 0;
@@ -563,6 +685,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
     // waves polling one address would otherwise saturate the memory-side
     // path that agent-scope loads take and stall the waves that work.
     long long t_poll = Uniform64(wall_clock64()), t_host = t_poll;
+    const long long t_born = t_poll;
+    long long t_idle0 = 0, idle_sum = 0;
     unsigned long long idl_hint = 0;
     // Returns true when the kernel is asked to stop.
     auto poll = [&]() -> bool {
@@ -601,8 +725,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             if (primary) {
                 const uint64_t r = r_next++;
                 const uint32_t w0 = Uniform(rows[r * D]);
-                ei = w0 >> T.P->tag_shift;
-                if (lane < D) matched[lane] = lane == 0 ? (w0 & T.P->v_mask) : rows[r * D + lane];
+                ei = w0 >> tag_shift;
+                if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : rows[r * D + lane];
                 D0 = D;
                 got = true;
             }
@@ -622,6 +746,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 ticket = Uniform64(__shfl(tk, 0, kWave));
                 have_ticket = true;
                 idle = true;
+                t_idle0 = (long long)Uniform64((unsigned long long)wall_clock64());
             }
             const uint64_t slot = ticket & (Q.cap - 1);
             unsigned long long sq = 0;
@@ -642,6 +767,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                 backoff = 1;
                 if (lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
                 idle = false;
+                idle_sum += (long long)Uniform64((unsigned long long)wall_clock64()) - t_idle0;
             } else {
                 unsigned long long pend = 0;
                 if (lane == 0) pend = LoadAgent(&cnt->q_pending.v);
@@ -654,6 +780,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
         }
         // ---------------- DFS from position D0
+        if (ei != X.ei || X.sa == 0xFFu) SetTag(X, ei);
         if (lane == 0) lvl[D0] = DfsLevel{lo0, 0, 0, hi0, sel0};
         cvalid = 0;
         sc.state = 0;
@@ -667,7 +794,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             }
             // The product suffix is counted, not enumerated.
             if (cart && d >= cps) {
-                count += SuffixCount(T, ei, d, matched, s_csize[wv], cvalid, sc, my_scanned, my_probes);
+                count += SuffixCount(X, T, d, matched, s_csize[wv], cvalid, sc, my_scanned, my_probes);
                 if (d == D0) break;
                 d--;
                 continue;
@@ -724,8 +851,8 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
                         matched[d] = x;
                         lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel};
                     }
-                    cvalid &= ~T.P->dep_mask[d];
-                    if ((T.P->stab_deps >> d) & 1u) sc.state = 0;
+                    cvalid &= ~Uniform(s_plan.dep_mask[d]);
+                    if ((sc.deps >> d) & 1u) sc.state = 0;
                     WaveSync();
                     d++;
                     continue;
@@ -734,10 +861,10 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             // Scan the next 64 candidates of position d.
             uint32_t sel = Uniform(lvl[d].sel);
             if (sel == kNoSel) {
-                sel = sets ? ChooseList(T, ei, d, matched) : 0;
+                sel = sets ? ChooseList(X, T, d, matched) : 0;
                 if (lane == 0) lvl[d].sel = sel;
             }
-            const List L = BnList(T, ei, d, sel, matched);
+            const List L = BnList(X, T, d, sel, matched);
             const uint32_t total = Uniform(L.bl + L.al);
             const uint32_t off = Uniform(lvl[d].off);
             uint32_t limit = Uniform(lvl[d].end);
@@ -792,7 +919,7 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
             if (i < limit) {
                 c = At(L, i);
                 my_scanned++;
-                ok = AcceptSel(T, ei, d, matched, c, sel, d, my_probes);
+                ok = AcceptSel(X, T, d, matched, c, sel, d, my_probes);
             }
             const uint64_t m = __ballot(ok);
             if (d == last) {
@@ -827,6 +954,14 @@ __global__ void __launch_bounds__(kThreads) KDfs(Tables T, const uint32_t* __res
         }
     }
     if (idle && lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
+    {
+        const long long now = (long long)Uniform64((unsigned long long)wall_clock64());
+        if (idle) idle_sum += now - t_idle0;
+        if (lane == 0) {
+            atomicAdd(&cnt->wave_ticks, (unsigned long long)(now - t_born));
+            atomicAdd(&cnt->idle_ticks, (unsigned long long)idle_sum);
+        }
+    }
     const unsigned long long s_tot = __shfl(WaveInclusiveScan(my_scanned), 63, kWave);
     const unsigned long long p_tot = __shfl(WaveInclusiveScan(my_probes), 63, kWave);
     if (lane == 0) {

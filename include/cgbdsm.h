@@ -99,6 +99,7 @@ typedef struct {
     uint64_t last_dfs_probes;     /* binary-search probes of the final DFS        */
     uint64_t last_dfs_donated;    /* work items handed between DFS wavefronts     */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
+    double   last_dfs_busy;       /* share of final-DFS wave time spent working   */
 } cgbdsm_stats;
 
 typedef struct cgbdsm_ctx cgbdsm_ctx;
