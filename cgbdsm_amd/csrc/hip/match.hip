@@ -35,7 +35,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWpb = kThreads / kWave;
-constexpr uint32_t kGrab = 4;  // rows taken per queue atomic
 
 struct List {
     const uint32_t* base;
@@ -638,7 +637,7 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
     return result;
 }
 
-// Final DFS.  Work items are frontier rows (taken kGrab at a time from a
+// Final DFS.  Work items are frontier rows (taken `grab` at a time from a
 // global cursor) and, once those run out, partial matches stolen from the
 // global queue.  A wavefront that is descending at a shallow level while
 // others idle donates the tail of its survivor list (items one level
@@ -654,8 +653,12 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
 //    before it adds SuffixCount.
 template <bool kTuples>
 __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
-                                                 uint32_t D, Counters* __restrict__ cnt, StealQueue Q,
+                                                 uint32_t D, uint32_t grab, Counters* __restrict__ cnt, StealQueue Q,
                                                  uint32_t* __restrict__ tuples, uint64_t tup_cap) {
+    // Per-edge counts gather in LDS and reach global memory once per block:
+    // a device-scope atomic per item on one address would serialize.
+    __shared__ unsigned long long s_pe[kMaxQE];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) s_pe[t] = 0;
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
     __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
     __shared__ uint32_t s_surv[kWpb][kMaxQV][kWave];
@@ -677,8 +680,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
     const bool cart = !kTuples && lists_are_sets && Q.suffix && cps < qv;
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
-    uint64_t r_next = 0, r_end = 0, ticket = 0;
-    bool primary = true, idle = false, stop = false, have_ticket = false;
+    uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
+    bool primary = true, idle = false, stop = false, have_ticket = false, from_row = false;
     uint32_t iter = 0, backoff = 1;
     // Shared words (idle count, flags) and the host flag are read at most
     // once per poll_ticks / host_ticks of wall clock per wave: thousands of
@@ -698,7 +701,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
         unsigned long long idl = 0, fl = 0;
         int ab = 0;
         if (lane == 0) {
-            idl = LoadAgent(&cnt->q_idle.v);
+            // Waves waiting for work hold tickets at or above the tail.
+            const unsigned long long h = LoadAgent(&cnt->q_head.v), t = LoadAgent(&cnt->q_tail.v);
+            idl = h > t ? h - t : 0;
             fl = LoadAgent(&cnt->flags);
             if (host && AbortRequested(Q)) {
                 atomicOr(&cnt->flags, kFlagAborted);
@@ -716,14 +721,16 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
         if (primary) {
             if (r_next >= r_end) {
                 unsigned long long r0 = 0;
-                if (lane == 0) r0 = atomicAdd(&cnt->q_rows.v, (unsigned long long)kGrab);
+                if (lane == 0) r0 = atomicAdd(&cnt->q_rows.v, (unsigned long long)grab);
                 r0 = Uniform64(__shfl(r0, 0, kWave));
                 r_next = r0;
-                r_end = r0 + kGrab < n_rows ? r0 + kGrab : n_rows;
+                r_end = r0 + grab < n_rows ? r0 + grab : n_rows;
+                r_first = r0;
                 if (r0 >= n_rows) primary = false;
             }
             if (primary) {
                 const uint64_t r = r_next++;
+                from_row = true;
                 const uint32_t w0 = Uniform(rows[r * D]);
                 ei = w0 >> tag_shift;
                 if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : rows[r * D + lane];
@@ -741,7 +748,6 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                 unsigned long long tk = 0;
                 if (lane == 0) {
                     tk = atomicAdd(&cnt->q_head.v, 1ull);
-                    if (!idle) atomicAdd(&cnt->q_idle.v, 1ull);
                 }
                 ticket = Uniform64(__shfl(tk, 0, kWave));
                 have_ticket = true;
@@ -763,9 +769,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                 hi0 = Uniform(GetAgent(&it->hi));
                 if (lane < D0) matched[lane] = GetAgent(&it->v[lane]);
                 got = true;
+                from_row = false;
                 have_ticket = false;
                 backoff = 1;
-                if (lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
                 idle = false;
                 idle_sum += (long long)Uniform64((unsigned long long)wall_clock64()) - t_idle0;
             } else {
@@ -949,11 +955,12 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
             WaveSync();
         }
         if (lane == 0) {
-            if (count) atomicAdd(&cnt->per_edge[ei], count);
-            atomicSub(&cnt->q_pending.v, 1ull);
+            if (count) atomicAdd(&s_pe[ei], count);
+            // Rows retire from `pending` a whole grab at a time, stolen items one by one.
+            if (!from_row) atomicSub(&cnt->q_pending.v, 1ull);
+            else if (r_next >= r_end) atomicSub(&cnt->q_pending.v, (unsigned long long)(r_end - r_first));
         }
     }
-    if (idle && lane == 0) atomicSub(&cnt->q_idle.v, 1ull);
     {
         const long long now = (long long)Uniform64((unsigned long long)wall_clock64());
         if (idle) idle_sum += now - t_idle0;
@@ -969,6 +976,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
         atomicAdd(&cnt->probes, p_tot);
         if (donated) atomicAdd(&cnt->donated, donated);
     }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x)
+        if (s_pe[t]) atomicAdd(&cnt->per_edge[t], s_pe[t]);
 }
 
 // Before a DFS launch: tickets taken but never filled in the previous launch
@@ -1018,11 +1028,14 @@ void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>
 void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
     if (!n) return;
-    const unsigned g = Grid1(n, kWpb * kGrab, grid_blocks);
+    const unsigned g = Grid1(n, kWpb, grid_blocks);
+    // Rows per cursor atomic (4 measured best on dense-6 streams: larger grabs
+    // leave a tail that stealing has to repair).
+    const uint32_t grab = 4;
     if (tuples)
-        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, Q, tuples, tup_cap);
+        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, grab, cnt, Q, tuples, tup_cap);
     else
-        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, cnt, Q, nullptr, 0);
+        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, grab, cnt, Q, nullptr, 0);
 }
 
 }  // namespace cgd
