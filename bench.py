@@ -93,6 +93,91 @@ def measured_traffic(kernel, workload):
     return None
 
 
+# Registered queries of the multi-query comparison: (kind, vertices, seed) of
+# seeded random walks on the same graph, uneven in size and shape.
+MQ_MIX = (("dense", 5, 101), ("tree", 5, 101), ("sparse", 5, 103), ("dense", 4, 101),
+          ("tree", 4, 100), ("sparse", 4, 104), ("dense", 6, 103), ("tree", 6, 104))
+
+
+def multi_query(args, threads, log):
+    """Several registered queries of uneven size over the same update stream,
+    each with its own device query (index, stream).  `plain`: per batch the
+    queries' Stage 2 runs one after another; `served`: the queries' Stage 2
+    runs concurrently, one host thread per query.  Both consume identical
+    batches from separate device queries, so every per-query count must agree."""
+    import threading
+
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import HostSession, synth_spec
+
+    n = min(args.multi_queries, len(MQ_MIX))
+    mix, sessions = [], []
+    for kind, size, seed in MQ_MIX[:n]:
+        spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                          initial_frac=args.initial_frac, stream_edges=args.stream_edges, query_size=size,
+                          query_kind=kind, query_seed=seed)
+        sessions.append(HostSession(spec=spec, batch_edges=args.batch_edges, threads=threads))
+        mix.append(f"{size}-{kind}-s{seed}")
+    nb = min(s.num_batches for s in sessions)
+    warm = max(0, min(args.warmup, nb - 1))
+    steps = max(1, min(args.steps, nb - warm))
+    eng = Engine(0)
+    sets = {"plain": [], "served": []}
+    for s in sessions:
+        keys, dv, card, avg = s.initial_keys()
+        plan = s.make_plan(card, avg)
+        batches = [s.next_batch() for _ in range(warm + steps)]
+        for name in sets:
+            q = eng.query(s.query_tables)
+            q.index_init(keys, dv)
+            q.set_plan(plan)
+            q.set_deadline(args.deadline_ms)
+            for kb in batches:
+                q.enqueue(kb)
+            sets[name].append(q)
+    log(f"multi-query: {n} queries {mix}, {warm} warmup + {steps} timed batches")
+
+    def one(q, out, k):
+        q.stage_next()
+        c, _, _ = q.run_batch()
+        out[k] = c
+
+    res = {}
+    for name, qs in sets.items():
+        lat, counts = [], []
+        for b in range(warm + steps):
+            out = [0] * n
+            t0 = time.perf_counter()
+            if name == "plain":
+                for k, q in enumerate(qs):
+                    one(q, out, k)
+            else:
+                ths = [threading.Thread(target=one, args=(q, out, k)) for k, q in enumerate(qs)]
+                for t in ths:
+                    t.start()
+                for t in ths:
+                    t.join()
+            dt = 1e3 * (time.perf_counter() - t0)
+            if b >= warm:
+                lat.append(dt)
+            counts.append(out)
+        res[name] = (lat, counts)
+        log(f"multi-query {name}: {statistics.mean(lat):.1f} ms per batch")
+    pairs = args.batch_edges
+    out = {"queries": n, "mix": mix, "timed_batches": steps,
+           "counts_equal": res["plain"][1] == res["served"][1],
+           "matches": int(sum(sum(c) for c in res["plain"][1][warm:]))}
+    for name, (lat, _) in res.items():
+        out[name] = {"update_edges_per_s": round(pairs * len(lat) / (1e-3 * sum(lat)), 3),
+                     "p50_batch_ms": round(percentile(lat, 50), 4), "p99_batch_ms": round(percentile(lat, 99), 4)}
+    out["served_vs_plain"] = round(out["served"]["update_edges_per_s"] / out["plain"]["update_edges_per_s"], 4)
+    for qs in sets.values():
+        for q in qs:
+            q.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +200,8 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
+    ap.add_argument("--multi-queries", type=int, default=8,
+                    help="queries in the multi-query (plain vs served) comparison; 0 skips it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -215,6 +302,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom, workload), "kernel": dom,
                 "launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
+    mq = multi_query(args, threads, log) if args.multi_queries > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         c_edges, c_secs, c_counts = cpu_baseline(keys, dv, plan, batches, args.cpu_seconds, threads)
@@ -244,7 +333,7 @@ def main():
         "kernels_ms": {k: round(v["ms"], 4) for k, v in kt.items()},
         "kernels_gbs": {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 3) for k, v in kt.items() if v["ms"] > 0 and v["bytes"]},
         "dfs_totals": dfs,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "multi_query": mq,
     }
     print(json.dumps(line), flush=True)
 

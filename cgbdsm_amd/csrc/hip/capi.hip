@@ -24,7 +24,8 @@ struct DBuf {
 
 struct cgbdsm_ctx {
     int device = 0;
-    uint64_t budget = 0, used = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};  // queries of one context may run on different threads
     int num_cus = 256;
     int wall_khz = 100000;  // device wall clock (wall_clock64) rate
 };
@@ -114,8 +115,8 @@ cgbdsm_status Guard(F&& f) {
 
 void* DevAlloc(cgbdsm_ctx* ctx, size_t bytes) {
     if (!bytes) return nullptr;
-    if (ctx->budget && ctx->used + bytes > ctx->budget)
-        Throw(CGBDSM_ERR_OOM, "device memory budget exceeded (" + std::to_string(ctx->used + bytes) + " > " +
+    if (ctx->budget && ctx->used.load() + bytes > ctx->budget)
+        Throw(CGBDSM_ERR_OOM, "device memory budget exceeded (" + std::to_string(ctx->used.load() + bytes) + " > " +
                                   std::to_string(ctx->budget) + " bytes)");
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -123,14 +124,16 @@ void* DevAlloc(cgbdsm_ctx* ctx, size_t bytes) {
         (void)hipGetLastError();
         Throw(CGBDSM_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
-    ctx->used += bytes;
+    ctx->used.fetch_add(bytes);
     return p;
 }
 
 void DevFree(cgbdsm_ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
-    ctx->used -= std::min<uint64_t>(ctx->used, bytes);
+    uint64_t cur = ctx->used.load();
+    while (!ctx->used.compare_exchange_weak(cur, cur - std::min<uint64_t>(cur, bytes))) {
+    }
 }
 
 template <class T>
