@@ -35,7 +35,7 @@ class Stats(C.Structure):
         "batches", "dv_count", "main_entries", "pool_used", "pool_cap", "last_ins", "last_del",
         "last_upd", "last_rows_touched", "last_idx_bytes", "last_frontier_rows",
         "last_frontier_depth", "last_matches", "compactions", "last_dfs_scanned", "last_dfs_probes",
-        "last_dfs_donated")] + [
+        "last_dfs_donated", "gpma_rebuilds", "gpma_windows", "gpma_top_level")] + [
         (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit",
                                   "last_dfs_busy")]
 
@@ -45,7 +45,8 @@ class KTime(C.Structure):
     _fields_ = [("ms", C.c_double), ("launches", C.c_uint64), ("bytes", C.c_uint64), ("items", C.c_uint64)]
 
 
-KERNEL_CLASSES = ["sort", "merge_insert", "merge_remove", "update_index", "masks", "seed", "extend", "dfs"]
+KERNEL_CLASSES = ["sort", "merge_insert", "merge_remove", "update_index", "masks", "seed", "extend", "dfs",
+                  "gpma_update", "gpma_extract"]
 
 
 class SynthSpec(C.Structure):
@@ -143,6 +144,7 @@ def dev_lib():
         L.cgbdsm_sync.argtypes = [P]
         L.cgbdsm_set_deadline_ms.argtypes = [P, C.c_double]
         L.cgbdsm_set_dfs_options.argtypes = [P, C.c_int, C.c_int]
+        L.cgbdsm_query_set_layout.argtypes = [P, C.c_int]
         L.cgbdsm_profile.argtypes = [P, I]
         L.cgbdsm_kernel_times.argtypes = [P, C.c_void_p, I]
         L.cgbdsm_batch_enqueue.argtypes = [P, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U32]
@@ -158,6 +160,6 @@ DEV_SYMBOLS = [
     "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_batch_update_global_index",
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
     "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
-    "cgbdsm_profile",
+    "cgbdsm_profile", "cgbdsm_query_set_layout",
     "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
 ]

@@ -26,7 +26,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", f"-I{INC}"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INC}", "-pthread"]
-CLI_TARGETS = ["cgcsm_merged_matching", "cgcsm_gamma", "cgcsm_merged_matching_global"]
+CLI_TARGETS = ["cgcsm_merged_matching", "cgcsm_gamma", "cgcsm_merged_matching_global", "cgcsm_gpma"]
 
 
 def _run(cmd, verbose):

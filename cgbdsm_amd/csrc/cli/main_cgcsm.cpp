@@ -21,6 +21,8 @@
 //                                 only difference from the LDS variant in
 //                                 the reference (enumeration.cu:1157-1490)
 //                                 does not exist here
+//   cgcsm_gpma                    the main index kept as a packed-memory
+//                                 array (CGBDSM_LAYOUT_GPMA)
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -107,6 +109,7 @@ int main(int argc, char* argv[]) {
         cgbdsm_plan tables;
         cg::FillQueryTables(sess->query(), &sess->plan(), sess->ebits(), &tables);
         Check(cgbdsm_query_create(ctx, &tables, &q), "cgbdsm_query_create");
+        if (prog == "cgcsm_gpma") Check(cgbdsm_query_set_layout(q, CGBDSM_LAYOUT_GPMA), "cgbdsm_query_set_layout");
         std::cout << "----------- GPU Memory Allocation ------------\n";
         const std::vector<uint64_t>& keys = sess->InitialKeys();
         uint32_t card[CGBDSM_MAX_DIR] = {};

@@ -83,6 +83,17 @@ struct cgbdsm_query {
         uint32_t dv = 0;
     };
     std::deque<Resident> fifo;
+    // GPMA layout state (packed-memory array of all main-index keys)
+    int layout = CGBDSM_LAYOUT_GAMMA;
+    bool inited = false;
+    struct Pma {
+        DBuf<uint64_t> slot, shadow, lo, S, keys, keys2;
+        DBuf<uint32_t> cnt, ins_cnt, win_of, gpos, wins, leaf, leaf2, keep, pos, chosen;
+        DBuf<PmaWin> meta;
+        uint32_t nleaves = 0, height = 0;
+        uint64_t n = 0;      // keys stored
+        bool stale = false;  // flat lists lag the array (after a commit)
+    } pma;
 };
 
 namespace {
@@ -448,6 +459,158 @@ float Elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+
+// ---------------------------------------------------------------- GPMA layout
+void PmaAlloc(cgbdsm_query* q, uint32_t nleaves) {
+    auto& P = q->pma;
+    const uint64_t slots = (uint64_t)nleaves * kPmaLeaf;
+    Ensure(q, P.slot, slots);
+    Ensure(q, P.shadow, slots);
+    Ensure(q, P.lo, nleaves);
+    Ensure(q, P.S, 2ull * nleaves);
+    Ensure(q, P.cnt, nleaves + 1ull);
+    Ensure(q, P.ins_cnt, nleaves);
+    Ensure(q, P.win_of, nleaves);
+    Ensure(q, P.gpos, nleaves + 1ull);
+    Ensure(q, P.wins, nleaves);
+    Ensure(q, P.meta, nleaves);
+    Ensure(q, P.chosen, (2ull * nleaves + 3) / 4);
+    P.nleaves = nleaves;
+    P.height = 0;
+    while ((1u << P.height) < nleaves) P.height++;
+}
+
+uint32_t PmaLeavesFor(uint64_t n) {
+    // Bulk load / rebuild at about half density.
+    const uint64_t want = std::max<uint64_t>(1, (2 * n + kPmaLeaf - 1) / kPmaLeaf);
+    uint32_t l = 1;
+    while (l < want) l <<= 1;
+    return l;
+}
+
+// Even spread of n sorted PMA keys in P.keys over a fresh array.
+void PmaSpread(cgbdsm_query* q, uint64_t n) {
+    auto& P = q->pma;
+    PmaAlloc(q, PmaLeavesFor(n));
+    LaunchPmaSpread(P.keys.p, n, P.slot.p, P.cnt.p, P.lo.p, P.nleaves, q->s);
+    FillU32(P.cnt.p + P.nleaves, 0, 1, q->s);
+    P.n = n;
+}
+
+// Bulk load from n sorted unique index keys (device encoding).
+void PmaBulk(cgbdsm_query* q, const uint64_t* keys, uint64_t n) {
+    auto& P = q->pma;
+    Ensure(q, P.keys, std::max<uint64_t>(n, 1));
+    LaunchPmaEncode(keys, n, q->vbits, P.keys.p, q->s);
+    PmaSpread(q, n);
+}
+
+// Flat lists (M for matching) from the array: one streaming pass.
+void PmaExtract(cgbdsm_query* q) {
+    auto& P = q->pma;
+    const int k = KBegin(q, CGBDSM_K_GPMA_EXTRACT);
+    ScanU32(P.cnt.p, P.gpos.p, P.nleaves + 1ull, (uint64_t*)Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+    ReadCounters(q);
+    const uint64_t n = q->h_cnt->n_segs;
+    if (q->pool.cap < std::max<uint64_t>(n, 1)) {
+        Release(q, q->pool);
+        const uint64_t want = n + n / 2 + (16ull << 20);
+        q->pool.p = (uint32_t*)DevAlloc(q->ctx, want * sizeof(uint32_t));
+        q->pool.cap = want;
+    }
+    LaunchPmaFlat(P.slot.p, P.cnt.p, P.gpos.p, P.nleaves, P.shadow.p, q->s);
+    const uint64_t rows = (uint64_t)q->dv * q->ndir;
+    LaunchPmaRows(P.shadow.p, n, q->pool.p, q->m_rec.p, rows, q->s);
+    KEnd(q, k);
+    KAdd(q, CGBDSM_K_GPMA_EXTRACT, 8ull * (uint64_t)P.nleaves * kPmaLeaf + 12ull * n + 16ull * rows, n);
+    P.n = n;
+    P.stale = false;
+    q->st.main_entries = n;
+}
+
+// Rebuild at a larger size with `extra` sorted absent PMA keys merged in
+// (the whole array overflowed its root density bound).
+void PmaRebuild(cgbdsm_query* q, const uint64_t* extra, uint64_t n_extra) {
+    auto& P = q->pma;
+    ScanU32(P.cnt.p, P.gpos.p, P.nleaves + 1ull, (uint64_t*)Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+    ReadCounters(q);
+    const uint64_t n_old = q->h_cnt->n_segs, n = n_old + n_extra;
+    // The extra keys live in P.keys2: copy them out before P.keys grows.
+    DBuf<uint64_t> tmp;
+    tmp.p = (uint64_t*)DevAlloc(q->ctx, std::max<uint64_t>(n_extra, 1) * sizeof(uint64_t));
+    tmp.cap = std::max<uint64_t>(n_extra, 1);
+    if (n_extra) CGD_HIP(hipMemcpyAsync(tmp.p, extra, n_extra * 8, hipMemcpyDeviceToDevice, q->s));
+    Ensure(q, P.keys, std::max<uint64_t>(n, 1));
+    LaunchPmaFlat(P.slot.p, P.cnt.p, P.gpos.p, P.nleaves, P.keys.p, q->s);
+    if (n_extra) CGD_HIP(hipMemcpyAsync(P.keys.p + n_old, tmp.p, n_extra * 8, hipMemcpyDeviceToDevice, q->s));
+    CGD_HIP(hipStreamSynchronize(q->s));
+    DevFree(q->ctx, tmp.p, tmp.cap * sizeof(uint64_t));
+    Ensure(q, P.keys2, std::max<uint64_t>(n, 1));
+    Ensure(q, q->rs_hist, RadixHistWords(n));
+    const uint32_t bits = kPmaVBits + BitsFor((uint64_t)q->dv_cap * q->ndir + 1);
+    RadixSortU64(P.keys.p, P.keys2.p, n, bits, q->rs_hist.p, q->s);
+    PmaSpread(q, n);
+}
+
+// M <- M u keys: nk sorted unique index keys (device encoding).
+void PmaInsert(cgbdsm_query* q, const uint64_t* keys, uint64_t nk) {
+    auto& P = q->pma;
+    if (!nk) return;
+    const int k = KBegin(q, CGBDSM_K_GPMA_UPDATE);
+    Ensure(q, P.keys, nk);
+    Ensure(q, P.keys2, nk);
+    Ensure(q, P.leaf, nk);
+    Ensure(q, P.leaf2, nk);
+    Ensure(q, P.keep, nk + 1);
+    Ensure(q, P.pos, nk + 1);
+    Ensure(q, q->scan_scratch, ScanScratchBytes(std::max<uint64_t>(nk + 2, P.nleaves + 2ull)) + 64);
+    LaunchPmaEncode(keys, nk, q->vbits, P.keys.p, q->s);
+    LaunchPmaRoute(P.keys.p, nk, P.lo.p, P.nleaves, P.leaf.p, q->s);
+    LaunchPmaAbsent(P.keys.p, nk, P.leaf.p, P.slot.p, P.cnt.p, P.keep.p, q->s);
+    unsigned long long* n_new = Cnt(q, &Counters::n_segs);
+    ScanU32(P.keep.p, P.pos.p, nk, (uint64_t*)n_new, q->scan_scratch.p, q->s);
+    LaunchPmaCompact(P.keys.p, P.leaf.p, nk, P.keep.p, P.pos.p, P.keys2.p, P.leaf2.p, q->s);
+    SetCounter(q, &Counters::flags, 0);
+    LaunchPmaWindows(P.cnt.p, P.ins_cnt.p, P.leaf2.p, n_new, nk, P.S.p, (uint8_t*)P.chosen.p, P.nleaves, P.height,
+                     q->d_cnt, q->s);
+    ReadCounters(q);
+    const uint64_t nn = q->h_cnt->n_segs;
+    if (nn && (q->h_cnt->flags & kFlagPool)) {
+        KEnd(q, k);
+        PmaRebuild(q, P.keys2.p, nn);
+        q->st.gpma_rebuilds++;
+    } else if (nn) {
+        ScanU32(P.cnt.p, P.gpos.p, P.nleaves + 1ull, (uint64_t*)Cnt(q, &Counters::pool_top), q->scan_scratch.p,
+                q->s);
+        SetCounter(q, &Counters::rows_out, 0);
+        LaunchPmaRebalance((const uint8_t*)P.chosen.p, P.nleaves, P.height, P.wins.p, Cnt(q, &Counters::rows_out),
+                           P.win_of.p, P.gpos.p, P.leaf2.p, P.keys2.p, n_new, nk, P.meta.p, P.slot.p, P.shadow.p,
+                           P.cnt.p, P.lo.p, Cnt(q, &Counters::pma_level), q->s);
+        KEnd(q, k);
+        ReadCounters(q);
+        q->st.gpma_windows += q->h_cnt->rows_out;
+        q->st.gpma_top_level = std::max<uint64_t>(q->st.gpma_top_level, q->h_cnt->pma_level);
+    } else {
+        KEnd(q, k);
+    }
+    KAdd(q, CGBDSM_K_GPMA_UPDATE, 8ull * nk, nk);
+    P.n += nn;
+}
+
+// M <- M \\ keys: n sorted index keys (device encoding).
+void PmaDelete(cgbdsm_query* q, const uint64_t* keys, uint64_t n) {
+    auto& P = q->pma;
+    if (!n) return;
+    const int k = KBegin(q, CGBDSM_K_GPMA_UPDATE);
+    Ensure(q, P.keys, n);
+    Ensure(q, P.leaf, n);
+    LaunchPmaEncode(keys, n, q->vbits, P.keys.p, q->s);
+    LaunchPmaRoute(P.keys.p, n, P.lo.p, P.nleaves, P.leaf.p, q->s);
+    LaunchPmaDelete(P.keys.p, P.leaf.p, n, P.slot.p, P.cnt.p, q->s);
+    KEnd(q, k);
+    KAdd(q, CGBDSM_K_GPMA_UPDATE, 8ull * n, n);
+}
+
 }  // namespace
 
 extern "C" {
@@ -538,6 +701,12 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
                         &q->front[0], &q->front[1]})
             Release(q, *b);
         Release(q, q->scan_scratch);
+        for (auto* b : {&q->pma.slot, &q->pma.shadow, &q->pma.lo, &q->pma.S, &q->pma.keys, &q->pma.keys2})
+            Release(q, *b);
+        for (auto* b : {&q->pma.cnt, &q->pma.ins_cnt, &q->pma.win_of, &q->pma.gpos, &q->pma.wins, &q->pma.leaf,
+                        &q->pma.leaf2, &q->pma.keep, &q->pma.pos, &q->pma.chosen})
+            Release(q, *b);
+        Release(q, q->pma.meta);
         for (auto& r : q->fifo) {
             Release(q, r.ins);
             Release(q, r.del);
@@ -644,6 +813,15 @@ cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
     });
 }
 
+cgbdsm_status cgbdsm_query_set_layout(cgbdsm_query* q, int layout) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (layout != CGBDSM_LAYOUT_GAMMA && layout != CGBDSM_LAYOUT_GPMA) Throw(CGBDSM_ERR_INVALID, "unknown layout");
+        if (q->inited) Throw(CGBDSM_ERR_INVALID, "the layout is chosen before cgbdsm_index_init");
+        q->layout = layout;
+    });
+}
+
 cgbdsm_status cgbdsm_index_init(cgbdsm_query* q, const uint64_t* keys, uint64_t n, uint32_t dv_count,
                                 uint32_t* cardinalities, float* avg_degrees) {
     return Guard([&] {
@@ -687,6 +865,13 @@ cgbdsm_status cgbdsm_index_init(cgbdsm_query* q, const uint64_t* keys, uint64_t 
             if (avg_degrees) avg_degrees[e] = nd ? (float)(uint32_t)c / (float)(uint32_t)nd : 0.0f;
         }
         q->st.main_entries = q->h_cnt->n_unique;
+        if (q->layout == CGBDSM_LAYOUT_GPMA) {
+            // The lists built above give the cardinalities; the matching
+            // lists come from the array from here on.
+            PmaBulk(q, q->k_ins.p, q->h_cnt->n_unique);
+            PmaExtract(q);
+        }
+        q->inited = true;
         q->phase = 0;
     });
 }
@@ -803,6 +988,22 @@ cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
         SetCounter(q, &Counters::bytes, 0);
         SetCounter(q, &Counters::rows_touched, 0);
         q->batch_idx_bytes = 0;
+        if (q->layout == CGBDSM_LAYOUT_GPMA) {
+            if (q->n_ins) {
+                DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, q->n_ins);
+                LaunchUnique(q->k_alt.p, q->k_ins.p, q->n_ins, q->flags.p, q->scan_scratch.p,
+                             Cnt(q, &Counters::n_unique), q->s);
+                ReadCounters(q);
+                PmaInsert(q, q->k_ins.p, q->h_cnt->n_unique);
+            }
+            if (q->n_del) {
+                DecodeSort(q, q->k_del.p, q->k_alt.p, q->k_del.p, q->n_del);
+                PmaDelete(q, q->k_alt.p, q->n_del);
+            }
+            PmaExtract(q);
+            q->phase = 2;
+            return;
+        }
         if (q->n_ins) {
             DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, q->n_ins);
             LaunchUnique(q->k_alt.p, q->k_ins.p, q->n_ins, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
@@ -1034,9 +1235,15 @@ cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q) {
         if (q->n_upd) {
             LaunchUnique(q->k_usort.p, q->k_alt.p, q->n_upd, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
                          q->s);
-            LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
-                           q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
-            InsertSorted(q, q->k_alt.p, Cnt(q, &Counters::n_segs), q->n_upd);
+            if (q->layout == CGBDSM_LAYOUT_GPMA) {
+                ReadCounters(q);
+                PmaInsert(q, q->k_alt.p, q->h_cnt->n_unique);
+                q->pma.stale = true;
+            } else {
+                LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_upd, q->vbits, q->flags.p, q->seg_row.p,
+                               q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+                InsertSorted(q, q->k_alt.p, Cnt(q, &Counters::n_segs), q->n_upd);
+            }
         }
         q->st.batches++;
         q->phase = 0;
@@ -1102,6 +1309,7 @@ cgbdsm_status cgbdsm_kernel_times(cgbdsm_query* q, cgbdsm_ktime* out, int reset)
 cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out) {
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (which == 0 && q->layout == CGBDSM_LAYOUT_GPMA && q->pma.stale) PmaExtract(q);
         const uint64_t nrows = (uint64_t)q->dv * q->ndir;
         uint64_t* lens = (uint64_t*)DevAlloc(q->ctx, (nrows + 1) * 8);
         void* scr = DevAlloc(q->ctx, ScanScratchBytes(nrows + 1) + 64);

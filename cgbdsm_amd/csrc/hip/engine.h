@@ -56,6 +56,7 @@ struct alignas(128) Counters {
     unsigned long long scanned;       // candidate list entries read by the final DFS
     unsigned long long probes;        // binary-search probes of the final DFS
     unsigned long long donated;       // partial matches handed to the stealing queue
+    unsigned long long pma_level;     // GPMA: highest rebalanced window level + 1 (since reset)
     unsigned long long wave_ticks;    // final DFS: summed wave lifetimes (wall-clock ticks)
     unsigned long long idle_ticks;    // final DFS: summed time waves waited for work
     // Words polled by many wavefronts at once, one 128-byte L2 line each.

@@ -62,4 +62,34 @@ void LaunchCountDups(const uint64_t* k, uint64_t n, Counters* cnt, hipStream_t s
 
 void LaunchQueueSync(Counters* cnt, hipStream_t s);
 
+// ---------------------------------------------------------------- GPMA layout
+constexpr uint32_t kPmaLeaf = 512;   // slots per leaf (u64 keys)
+constexpr uint32_t kPmaVBits = 27;   // PMA key = (row << 27) | v
+struct PmaWin {                       // a window being rebalanced
+    uint32_t a, nl;                   // first leaf, leaves
+    uint64_t E, ilo, ihi;             // existing keys, insert range [ilo, ihi)
+};
+void LaunchPmaEncode(const uint64_t* in, uint64_t n, uint32_t vbits, uint64_t* out, hipStream_t s);
+void LaunchPmaSpread(const uint64_t* keys, uint64_t n, uint64_t* slot, uint32_t* cnt, uint64_t* lo, uint32_t nleaves,
+                     hipStream_t s);
+void LaunchPmaRoute(const uint64_t* keys, uint64_t n, const uint64_t* lo, uint32_t nleaves, uint32_t* leaf_of,
+                    hipStream_t s);
+void LaunchPmaAbsent(const uint64_t* keys, uint64_t n, const uint32_t* leaf_of, const uint64_t* slot,
+                     const uint32_t* cnt, uint32_t* keep, hipStream_t s);
+void LaunchPmaCompact(const uint64_t* keys, const uint32_t* leaf_of, uint64_t n, const uint32_t* keep,
+                      const uint32_t* pos, uint64_t* out_keys, uint32_t* out_leaf, hipStream_t s);
+void LaunchPmaWindows(const uint32_t* cnt, uint32_t* ins_cnt, const uint32_t* ins_leaf,
+                      const unsigned long long* n_ins_dev, uint64_t n_upper, uint64_t* S, uint8_t* chosen,
+                      uint32_t nleaves, uint32_t height, Counters* counters, hipStream_t s);
+void LaunchPmaRebalance(const uint8_t* chosen, uint32_t nleaves, uint32_t height, uint32_t* wins,
+                        unsigned long long* nwin, uint32_t* win_of, const uint32_t* gpos, const uint32_t* ins_leaf,
+                        const uint64_t* ins_keys, const unsigned long long* n_ins_dev, uint64_t n_upper, PmaWin* meta,
+                        uint64_t* slot, uint64_t* shadow, uint32_t* cnt, uint64_t* lo, unsigned long long* level_seen,
+                        hipStream_t s);
+void LaunchPmaDelete(const uint64_t* dkeys, const uint32_t* dleaf, uint64_t n, uint64_t* slot, uint32_t* cnt,
+                     hipStream_t s);
+void LaunchPmaFlat(const uint64_t* slot, const uint32_t* cnt, const uint32_t* gpos, uint32_t nleaves, uint64_t* flat,
+                   hipStream_t s);
+void LaunchPmaRows(const uint64_t* flat, uint64_t n, uint32_t* pool, MRec* m_rec, uint64_t rows, hipStream_t s);
+
 }  // namespace cgd

@@ -145,6 +145,10 @@ class DeviceQuery:
     def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=4):
         _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
 
+    def set_layout(self, layout):
+        """'gamma' (per-list merges) or 'gpma' (packed-memory array); before index_init."""
+        _check(self.L.cgbdsm_query_set_layout(self.h, {"gamma": 0, "gpma": 1}[layout]))
+
     def set_dfs_options(self, shortest_list=True, product_suffix=True):
         """Count-mode refinements of the final DFS (both on by default)."""
         _check(self.L.cgbdsm_set_dfs_options(self.h, int(bool(shortest_list)), int(bool(product_suffix))))

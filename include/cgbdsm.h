@@ -98,6 +98,9 @@ typedef struct {
     uint64_t last_dfs_scanned;    /* candidate list entries read by the final DFS */
     uint64_t last_dfs_probes;     /* binary-search probes of the final DFS        */
     uint64_t last_dfs_donated;    /* work items handed between DFS wavefronts     */
+    uint64_t gpma_rebuilds;       /* GPMA layout: growths by full rebuild         */
+    uint64_t gpma_windows;        /* GPMA layout: windows rebalanced              */
+    uint64_t gpma_top_level;      /* GPMA layout: highest window level + 1        */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
     double   last_dfs_busy;       /* share of final-DFS wave time spent working   */
 } cgbdsm_stats;
@@ -127,6 +130,16 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q);
 /* Upload the global matching order, rebuild flags and cartesian tags:
  * GPUGraphLoader::LoadPlan (graph/graph_gpu.h:159, graph/graph_gpu.cu:174-193). */
 cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan);
+
+/* Storage layout of the main index, chosen before cgbdsm_index_init.
+ * GAMMA: per-(edge, vertex) sorted lists with slack, merged in place
+ *   (index/calig_helper_gamma.cu, kernels/indexing.cu:296-458).
+ * GPMA: a packed-memory array of all index keys with density-bounded
+ *   rebalancing (index/gpma_wrapper.cu:19-116, gpma/gpma.cu), extracted into
+ *   the same flat lists for matching every batch
+ *   (index/calig_helper_gpma.cu:383-414).  Both yield identical lists. */
+enum { CGBDSM_LAYOUT_GAMMA = 0, CGBDSM_LAYOUT_GPMA = 1 };
+cgbdsm_status cgbdsm_query_set_layout(cgbdsm_query* q, int layout);
 
 /* Initial main index from host keys (any order, duplicates allowed):
  * CaLiGHelperGamma::ConvertGlobalIndex device half
@@ -221,7 +234,9 @@ enum {
     CGBDSM_K_SEED = 5,          /* depth-2 frontier                         */
     CGBDSM_K_EXTEND = 6,        /* BFS levels                               */
     CGBDSM_K_DFS = 7,           /* final DFS count                          */
-    CGBDSM_K_COUNT = 8
+    CGBDSM_K_GPMA_UPDATE = 8,   /* GPMA layout: route, rebalance, delete    */
+    CGBDSM_K_GPMA_EXTRACT = 9,  /* GPMA layout: gap array -> flat lists     */
+    CGBDSM_K_COUNT = 10
 };
 typedef struct {
     double ms;          /* summed device time */

@@ -34,9 +34,11 @@ def _masks_equal(q, idx_masks, plan, dv, edge_ok):
 
 
 def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, match_params=None,
-               dfs_options=None, max_batches=None, log=None):
+               dfs_options=None, layout=None, max_batches=None, log=None):
     keys, dv, card, avg = host.initial_keys()
     q = eng.query(host.query_tables)
+    if layout:
+        q.set_layout(layout)
     if match_params:
         q.set_match_params(**match_params)
     if dfs_options:

@@ -13,7 +13,7 @@ from tests import oracle_lib as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "cgbdsm_amd", "bin")
-TARGETS = ["cgcsm_merged_matching", "cgcsm_gamma", "cgcsm_merged_matching_global"]
+TARGETS = ["cgcsm_merged_matching", "cgcsm_gamma", "cgcsm_merged_matching_global", "cgcsm_gpma"]
 
 SUMMARY_LABELS = [
     "Num Positive Matches: ",
@@ -88,7 +88,7 @@ def test_odd_batch_ints_reproduce_the_pairing_quirk(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,batch_ints", [("cgcsm_merged_matching", 600), ("cgcsm_gamma", 601),
-                                             ("cgcsm_merged_matching_global", None)])
+                                             ("cgcsm_merged_matching_global", None), ("cgcsm_gpma", 600)])
 def test_cli_counts_and_output_contract(tmp_path, name, batch_ints):
     print(f"case cli {name} batch_ints={batch_ints}", flush=True)
     spec = synth_spec(11, 8, 4, seed=2, query_size=5, query_kind="dense", query_seed=2)

@@ -110,6 +110,36 @@ def test_duplicate_stream_pairs(eng, tmp_path):
     assert res["mismatches"] == 0
 
 
+# (name, scale, avg_degree, labels, query size, kind, seed, initial share, batch edges)
+GPMA_STREAMS = [
+    ("gpma-s11-dense5", 11, 8, 4, 5, "dense", 2, 0.9, 300),
+    ("gpma-s12-tree5-b2000", 12, 8, 8, 5, "tree", 3, 0.9, 2000),
+    # a small initial graph and large batches: windows at high levels and
+    # root overflows that rebuild the array at a larger size
+    ("gpma-s11-grow", 11, 10, 3, 4, "dense", 4, 0.2, 4000),
+]
+
+
+@pytest.mark.parametrize("case", GPMA_STREAMS, ids=[c[0] for c in GPMA_STREAMS])
+def test_gpma_layout_parity(eng, case):
+    """The GPMA layout yields the same lists (and counts) as the oracle's
+    set algebra after every apply and every commit."""
+    name, scale, deg, labels, qs, kind, seed, frac, be = case
+    print(f"case {name}", flush=True)
+    spec = synth_spec(scale, deg, labels, seed=seed, initial_frac=frac, query_size=qs, query_kind=kind,
+                      query_seed=seed)
+    host = HostSession(spec=spec, batch_edges=be, threads=4)
+    res = parity.run_stream(host, eng, layout="gpma", log=_log)
+    assert res["batches"] > 0
+    assert res["mismatches"] == 0
+    st = res["stats"]
+    _log(f"gpma rebuilds {st['gpma_rebuilds']}, windows {st['gpma_windows']}, top level {st['gpma_top_level']}")
+    assert st["gpma_windows"] > 0
+    if name == "gpma-s11-grow":
+        assert st["gpma_rebuilds"] >= 1       # the root bound was exceeded
+        assert st["gpma_top_level"] >= 2      # windows above the leaf level
+
+
 def test_rejects_out_of_range_keys(eng):
     spec = synth_spec(8, 4, 2, seed=9, query_size=4, query_kind="tree", query_seed=9)
     host = HostSession(spec=spec, batch_edges=None, threads=2)
