@@ -178,6 +178,75 @@ def multi_query(args, threads, log):
     return out
 
 
+def index_stress(args, threads, log):
+    """Index maintenance alone (apply, update index, commit; no matching),
+    GAMMA lists vs the GPMA array on the same stream; the final main index
+    of both layouts must be identical."""
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import HostSession, synth_spec
+
+    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                      initial_frac=args.initial_frac, stream_edges=args.stream_edges,
+                      query_size=args.query_size, query_kind=args.query_kind, query_seed=args.query_seed)
+    t0 = time.perf_counter()
+    host = HostSession(spec=spec, batch_edges=args.batch_edges, threads=threads)
+    keys, dv, card, avg = host.initial_keys()
+    plan = host.make_plan(card, avg)
+    batches = [host.next_batch() for _ in range(host.num_batches)]
+    log(f"index stress: {keys.size} initial entries, {len(batches)} batches, Stage 1 done in "
+        f"{time.perf_counter() - t0:.1f}s")
+    warm = max(0, min(args.warmup, len(batches) - 1))
+    steps = max(1, min(args.steps, len(batches) - warm))
+    eng = Engine(0)
+    out, finals = {}, {}
+    for layout in ("gamma", "gpma"):
+        q = eng.query(host.query_tables)
+        q.set_layout(layout)
+        q.index_init(keys, dv)
+        q.set_plan(plan)
+        for kb in batches[: warm + steps]:
+            q.enqueue(kb)
+        lat = []
+        for i in range(warm + steps):
+            if i == warm:
+                q.profile(True)
+                q.kernel_times(reset=True)
+            q.sync()
+            tb = time.perf_counter()
+            q.stage_next()
+            q.transfer_update_all()
+            q.transfer_update_index()
+            q.batch_update_global_index()
+            q.sync()
+            if i >= warm:
+                lat.append(1e3 * (time.perf_counter() - tb))
+        kt = q.kernel_times()
+        st = q.stats()
+        finals[layout] = q.export(0)
+        edges = sum(kb.pairs for kb in batches[warm: warm + steps])
+        out[layout] = {"update_edges_per_s": round(edges / (1e-3 * sum(lat)), 3),
+                       "ms_per_batch": round(statistics.mean(lat), 3),
+                       "p99_batch_ms": round(percentile(lat, 99), 3),
+                       "main_entries": int(finals[layout].size),
+                       "kernels_ms": {k: round(v["ms"], 3) for k, v in kt.items() if v["ms"] > 0},
+                       "kernels_gbs": {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 3) for k, v in kt.items()
+                                       if v["ms"] > 0 and v["bytes"]}}
+        if layout == "gpma":
+            out[layout]["rebuilds"] = st["gpma_rebuilds"]
+            out[layout]["windows"] = st["gpma_windows"]
+        log(f"index stress {layout}: {statistics.mean(lat):.1f} ms per batch")
+        q.close()
+    eng.close()
+    import numpy as np
+    return {"metric": "index maintenance update edges/s (apply + update index + commit), GAMMA vs GPMA",
+            "unit": "update edges/s", "steps": steps, "warmup": warm,
+            "config": {"workload": f"rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g}, "
+                                   f"{args.initial_frac:g} initial, {args.stream_edges} streamed in "
+                                   f"{args.batch_edges}-edge batches, {args.query_size}-vertex {args.query_kind} query"},
+            "gamma": out["gamma"], "gpma": out["gpma"],
+            "lists_equal": bool(np.array_equal(finals["gamma"], finals["gpma"]))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +269,9 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
+    ap.add_argument("--layout", default="gamma", choices=["gamma", "gpma"], help="main-index layout")
+    ap.add_argument("--index-stress", action="store_true",
+                    help="index maintenance only, GAMMA vs GPMA (prints its own JSON line)")
     ap.add_argument("--multi-queries", type=int, default=8,
                     help="queries in the multi-query (plain vs served) comparison; 0 skips it")
     args = ap.parse_args()
@@ -215,6 +287,9 @@ def main():
     from cgbdsm_amd.host import HostSession, synth_spec
 
     threads = min(16, os.cpu_count() or 1)
+    if args.index_stress:
+        print(json.dumps(index_stress(args, threads, log)), flush=True)
+        return
     t_setup = time.perf_counter()
     spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
                       initial_frac=args.initial_frac, stream_edges=args.stream_edges,
@@ -239,6 +314,7 @@ def main():
 
     eng = Engine(0)
     q = eng.query(host.query_tables)
+    q.set_layout(args.layout)
     q.index_init(keys, dv)
     q.set_plan(plan)
     q.set_deadline(args.deadline_ms)
