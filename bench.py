@@ -269,7 +269,8 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
-    ap.add_argument("--layout", default="gamma", choices=["gamma", "gpma"], help="main-index layout")
+    ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
+                    help="main-index layout (both give identical lists; GPMA maintains them faster)")
     ap.add_argument("--index-stress", action="store_true",
                     help="index maintenance only, GAMMA vs GPMA (prints its own JSON line)")
     ap.add_argument("--multi-queries", type=int, default=8,
@@ -397,7 +398,7 @@ def main():
         "warmup": warm, "ms_per_step": round(1e3 * elapsed / steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": workload,
-                   "batch_edges": args.batch_edges, "query_edges": int(qedges.size // 2),
+                   "batch_edges": args.batch_edges, "query_edges": int(qedges.size // 2), "layout": args.layout,
                    "parallelism": "single GPU"},
         "p50_batch_ms": round(percentile(lat_ms, 50), 4), "p99_batch_ms": round(percentile(lat_ms, 99), 4),
         # Host keys -> HBM over PCIe (cgbdsm_batch_enqueue), outside the timed region.
