@@ -129,6 +129,7 @@ def multi_query(args, threads, log):
         batches = [s.next_batch() for _ in range(warm + steps)]
         for name in sets:
             q = eng.query(s.query_tables)
+            q.set_layout(args.layout)
             q.index_init(keys, dv)
             q.set_plan(plan)
             q.set_deadline(args.deadline_ms)
