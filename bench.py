@@ -101,11 +101,17 @@ MQ_MIX = (("dense", 5, 101), ("tree", 5, 101), ("sparse", 5, 103), ("dense", 4, 
 
 def multi_query(args, threads, log):
     """Several registered queries of uneven size over the same update stream,
-    each with its own device query (index, stream).  `plain`: per batch the
-    queries' Stage 2 runs one after another; `served`: the queries' Stage 2
-    runs concurrently, one host thread per query.  Both consume identical
-    batches from separate device queries, so every per-query count must agree."""
-    import threading
+    each with its own device query (index, stream).  Per batch:
+      plain           the queries' Stage 2 runs one after another;
+      served_threads  the queries' Stage 2 runs concurrently, one host thread
+                      per query (separate streams, one DFS launch each);
+      served          index maintenance and masks run concurrently per query,
+                      then ONE final-DFS launch counts every query from a
+                      shared work queue (cgbdsm_matching_batch), then the
+                      commits.
+    All modes consume identical batches from separate device queries, so
+    every per-query count must agree."""
+    from concurrent.futures import ThreadPoolExecutor
 
     from cgbdsm_amd.engine import Engine
     from cgbdsm_amd.host import HostSession, synth_spec
@@ -122,7 +128,8 @@ def multi_query(args, threads, log):
     warm = max(0, min(args.warmup, nb - 1))
     steps = max(1, min(args.steps, nb - warm))
     eng = Engine(0)
-    sets = {"plain": [], "served": []}
+    modes = ("plain", "served_threads", "served")
+    sets = {m: [] for m in modes}
     for s in sessions:
         keys, dv, card, avg = s.initial_keys()
         plan = s.make_plan(card, avg)
@@ -137,41 +144,57 @@ def multi_query(args, threads, log):
                 q.enqueue(kb)
             sets[name].append(q)
     log(f"multi-query: {n} queries {mix}, {warm} warmup + {steps} timed batches")
+    pool = ThreadPoolExecutor(max_workers=n)
 
-    def one(q, out, k):
+    def one(q):
         q.stage_next()
-        c, _, _ = q.run_batch()
-        out[k] = c
+        return q.run_batch()[0]
+
+    def prepare(q):
+        q.stage_next()
+        q.transfer_update_all()
+        if not q.transfer_update_index():
+            return False
+        q.build_local_index()
+        return True
 
     res = {}
     for name, qs in sets.items():
         lat, counts = [], []
         for b in range(warm + steps):
-            out = [0] * n
             t0 = time.perf_counter()
             if name == "plain":
-                for k, q in enumerate(qs):
-                    one(q, out, k)
+                out = [one(q) for q in qs]
+            elif name == "served_threads":
+                out = list(pool.map(one, qs))
             else:
-                ths = [threading.Thread(target=one, args=(q, out, k)) for k, q in enumerate(qs)]
-                for t in ths:
-                    t.start()
-                for t in ths:
-                    t.join()
+                live = list(pool.map(prepare, qs))
+                out = [0] * n
+                ks = [k for k in range(n) if live[k]]
+                if ks:
+                    for k, (c, _) in zip(ks, eng.matching_batch([qs[k] for k in ks])):
+                        out[k] = c
+                    for k in ks:
+                        qs[k].batch_update_global_index()
+                for q in qs:
+                    q.sync()
             dt = 1e3 * (time.perf_counter() - t0)
             if b >= warm:
                 lat.append(dt)
             counts.append(out)
         res[name] = (lat, counts)
         log(f"multi-query {name}: {statistics.mean(lat):.1f} ms per batch")
+    pool.shutdown()
     pairs = args.batch_edges
     out = {"queries": n, "mix": mix, "timed_batches": steps,
-           "counts_equal": res["plain"][1] == res["served"][1],
+           "counts_equal": all(res[m][1] == res["plain"][1] for m in modes),
            "matches": int(sum(sum(c) for c in res["plain"][1][warm:]))}
     for name, (lat, _) in res.items():
         out[name] = {"update_edges_per_s": round(pairs * len(lat) / (1e-3 * sum(lat)), 3),
                      "p50_batch_ms": round(percentile(lat, 50), 4), "p99_batch_ms": round(percentile(lat, 99), 4)}
     out["served_vs_plain"] = round(out["served"]["update_edges_per_s"] / out["plain"]["update_edges_per_s"], 4)
+    out["served_threads_vs_plain"] = round(out["served_threads"]["update_edges_per_s"] /
+                                           out["plain"]["update_edges_per_s"], 4)
     for qs in sets.values():
         for q in qs:
             q.close()

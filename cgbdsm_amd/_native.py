@@ -11,6 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 MAX_QV, MAX_QE = 12, 42
 MAX_DIR = 2 * MAX_QE
+MAX_BATCH_QUERIES = 256
 
 
 class Plan(C.Structure):
@@ -134,6 +135,7 @@ def dev_lib():
         L.cgbdsm_transfer_update_index.argtypes = [P, C.POINTER(I)]
         L.cgbdsm_build_local_index.argtypes = [P]
         L.cgbdsm_matching.argtypes = [P, C.POINTER(C.c_ulonglong), C.c_void_p]
+        L.cgbdsm_matching_batch.argtypes = [C.POINTER(P), I, C.POINTER(C.c_ulonglong), C.POINTER(C.c_void_p)]
         L.cgbdsm_batch_update_global_index.argtypes = [P]
         L.cgbdsm_set_match_params.argtypes = [P, U64, U64, U32]
         L.cgbdsm_index_export.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
@@ -157,7 +159,7 @@ DEV_SYMBOLS = [
     "cgbdsm_last_error", "cgbdsm_version", "cgbdsm_ctx_create", "cgbdsm_ctx_destroy",
     "cgbdsm_query_create", "cgbdsm_query_destroy", "cgbdsm_query_set_plan", "cgbdsm_index_init",
     "cgbdsm_batch_stage", "cgbdsm_transfer_update_all", "cgbdsm_transfer_update_index",
-    "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_batch_update_global_index",
+    "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_matching_batch", "cgbdsm_batch_update_global_index",
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
     "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
     "cgbdsm_profile", "cgbdsm_query_set_layout",

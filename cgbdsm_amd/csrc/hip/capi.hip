@@ -83,6 +83,11 @@ struct cgbdsm_query {
         uint32_t dv = 0;
     };
     std::deque<Resident> fifo;
+    // final-DFS job table (one entry per query of a launch)
+    DBuf<DfsJob> d_jobs;
+    DBuf<unsigned long long> d_rowbase;
+    std::vector<DfsJob> h_jobs;
+    std::vector<unsigned long long> h_rowbase;
     // GPMA layout state (packed-memory array of all main-index keys)
     int layout = CGBDSM_LAYOUT_GAMMA;
     bool inited = false;
@@ -395,18 +400,20 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
     Q.seq = q->steal_seq;
     Q.cap = q->steal_cap;
     Q.donate_min = donate ? q->donate_min : 0;
-    Q.shortest = q->dfs_shortest ? 1 : 0;
-    Q.suffix = q->dfs_suffix ? 1 : 0;
     Q.abort = q->d_abort;
     Q.poll_ticks = std::max<long long>(1, q->ctx->wall_khz / 20);  // 50 us
     Q.host_ticks = (long long)q->ctx->wall_khz * 2;              // 2 ms
     return Q;
 }
 
-// Final DFS over the frontier; waits for it while honouring the deadline:
-// past the deadline the host raises the abort flag the kernel polls, and the
-// batch fails with CGBDSM_ERR_OVERFLOW (its count would be incomplete).
-void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
+// Final DFS over the rows of the jobs in q->h_jobs (job j owns rows
+// [h_rowbase[j], h_rowbase[j+1])), on q's stream with q's work queue; waits
+// for it while honouring q's deadline: past the deadline the host raises the
+// abort flag the kernel polls, and the call fails with CGBDSM_ERR_OVERFLOW
+// (its counts would be incomplete).
+void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
+    const uint64_t n = q->h_rowbase.back();
+    const uint32_t depth = q->h_jobs.empty() ? 0 : q->h_jobs[0].D;
     SetCounter(q, &Counters::flags, 0);
     LaunchQueueSync(q->d_cnt, q->s);
     FillU64((uint64_t*)&q->d_cnt->q_rows.v, 0, 1, q->s);
@@ -415,9 +422,15 @@ void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, u
     *(volatile uint32_t*)q->h_abort = 0;
     std::atomic_thread_fence(std::memory_order_seq_cst);
     if (!n) return;
+    Ensure(q, q->d_jobs, q->h_jobs.size());
+    Ensure(q, q->d_rowbase, q->h_rowbase.size());
+    CGD_HIP(hipMemcpyAsync(q->d_jobs.p, q->h_jobs.data(), q->h_jobs.size() * sizeof(DfsJob), hipMemcpyHostToDevice,
+                           q->s));
+    CGD_HIP(hipMemcpyAsync(q->d_rowbase.p, q->h_rowbase.data(), q->h_rowbase.size() * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, q->s));
     const int k = tuples ? -1 : KBegin(q, CGBDSM_K_DFS);
-    LaunchDfs(MakeTables(q), rows, n, depth, q->d_cnt, MakeSteal(q, tuples == nullptr), tuples, tup_cap,
-              q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+    LaunchDfs(q->d_jobs.p, (uint32_t)q->h_jobs.size(), q->d_rowbase.p, n, q->d_cnt, MakeSteal(q, tuples == nullptr),
+              tuples, tup_cap, q->ctx->num_cus * DfsBlocksPerCU(), q->s);
     KEnd(q, k);
     CGD_HIP(hipGetLastError());
     if (q->deadline_ms > 0) {
@@ -451,6 +464,17 @@ void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, u
             std::this_thread::sleep_for(std::chrono::microseconds(200));
         }
     }
+}
+
+DfsJob MakeJob(cgbdsm_query* q, const uint32_t* rows, uint32_t depth) {
+    return DfsJob{MakeTables(q), rows, q->d_cnt, depth,
+                  (q->dfs_shortest ? kJobShortest : 0u) | (q->dfs_suffix ? kJobSuffix : 0u)};
+}
+
+void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
+    q->h_jobs.assign(1, MakeJob(q, rows, depth));
+    q->h_rowbase.assign({0ull, (unsigned long long)n});
+    RunDfsJobs(q, tuples, tup_cap);
 }
 
 float Elapsed(hipEvent_t a, hipEvent_t b) {
@@ -707,6 +731,8 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
                         &q->pma.leaf2, &q->pma.keep, &q->pma.pos, &q->pma.chosen})
             Release(q, *b);
         Release(q, q->pma.meta);
+        Release(q, q->d_jobs);
+        Release(q, q->d_rowbase);
         for (auto& r : q->fifo) {
             Release(q, r.ins);
             Release(q, r.del);
@@ -1123,79 +1149,143 @@ cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
     });
 }
 
+// Seeds and BFS levels of one query's matching (phase 4): returns the rows
+// left for the final DFS, their depth and the buffer holding them.
+struct Frontier {
+    uint64_t n;
+    uint32_t depth, cur;
+};
+
+Frontier BuildFrontier(cgbdsm_query* q) {
+    const EiList L = ActiveEdges(q);
+    for (auto f : {&Counters::rows_out, &Counters::flags, &Counters::tuples})
+        SetCounter(q, f, 0);
+    FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
+    const Tables T = MakeTables(q);
+    const bool fixed_cap = q->frontier_cap_rows != 0;
+    uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
+    Ensure(q, q->front[0], cap_rows * 2);
+    // Step 1: seeds (retry once with a buffer of the exact size on overflow).
+    uint64_t n = 0;
+    for (int attempt = 0;; attempt++) {
+        const int ks = KBegin(q, CGBDSM_K_SEED);
+        LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
+        KEnd(q, ks);
+        ReadCounters(q);
+        n = q->h_cnt->rows_out;
+        if (!(q->h_cnt->flags & kFlagFrontier)) break;
+        if (attempt) Throw(CGBDSM_ERR_OVERFLOW, "seed frontier overflow");
+        Ensure(q, q->front[0], n * 2 + 64);
+        SetCounter(q, &Counters::rows_out, 0);
+        SetCounter(q, &Counters::flags, 0);
+    }
+    uint32_t depth = 2, cur = 0;
+    // Step 2: BFS levels while the frontier is small.
+    while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
+        const uint64_t want_rows =
+            fixed_cap ? q->frontier_cap_rows : std::min<uint64_t>(std::max<uint64_t>(n * 64, 1u << 20), 1ull << 28);
+        const uint64_t want_words = want_rows * (depth + 1);
+        Ensure(q, q->front[cur ^ 1], want_words);
+        SetCounter(q, &Counters::rows_out, 0);
+        SetCounter(q, &Counters::flags, 0);
+        const int ke = KBegin(q, CGBDSM_K_EXTEND);
+        LaunchExtend(T, q->front[cur].p, n, depth, q->front[cur ^ 1].p, q->front[cur ^ 1].cap / (depth + 1),
+                     q->d_cnt, q->s);
+        KEnd(q, ke);
+        ReadCounters(q);
+        KAdd(q, CGBDSM_K_EXTEND, 0, n);
+        if (q->h_cnt->flags & kFlagFrontier) break;  // keep this level, count it by DFS
+        n = q->h_cnt->rows_out;
+        cur ^= 1;
+        depth++;
+    }
+    return Frontier{n, depth, cur};
+}
+
+void ResetDfsCounters(cgbdsm_query* q) {
+    for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
+                   &Counters::idle_ticks})
+        SetCounter(q, f, 0);
+}
+
+// Statistics of the final-DFS launch whose queue lived in `leader`.
+void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
+    ReadCounters(leader);
+    const Counters* c = leader->h_cnt;
+    if (c->flags & (kFlagSteal | kFlagAborted))
+        Throw(CGBDSM_ERR_OVERFLOW, "final DFS did not complete (flags " + std::to_string(c->flags) + ")");
+    if (rows) KAdd(leader, CGBDSM_K_DFS, 4ull * (c->scanned + c->probes), rows);
+}
+
+// Per-query results of a completed final DFS; `launch` holds the counters of
+// the launch's work queue (the query itself for a single-query launch).
+void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* launch, unsigned long long* num_matches,
+                    uint64_t* per_edge) {
+    const Counters* c = launch->h_cnt;
+    q->st.last_dfs_scanned = c->scanned;
+    q->st.last_dfs_probes = c->probes;
+    q->st.last_dfs_donated = c->donated;
+    q->st.last_dfs_busy = c->wave_ticks ? 1.0 - (double)c->idle_ticks / (double)c->wave_ticks : 0.0;
+    unsigned long long total = 0;
+    for (uint32_t e = 0; e < q->qe; e++) {
+        total += q->h_cnt->per_edge[e];
+        if (per_edge) per_edge[e] = q->h_cnt->per_edge[e];
+    }
+    if (num_matches) *num_matches += total;
+    q->last_depth = f.depth;
+    q->last_rows = f.n;
+    q->last_buf = f.cur;
+    q->st.last_frontier_rows = f.n;
+    q->st.last_frontier_depth = f.depth;
+    q->st.last_matches = total;
+    q->phase = 5;
+}
+
 cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge) {
     return Guard([&] {
         if (!q || q->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
-        const EiList L = ActiveEdges(q);
-        for (auto f : {&Counters::rows_out, &Counters::flags, &Counters::tuples})
-            SetCounter(q, f, 0);
-        FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
-        const Tables T = MakeTables(q);
-        const bool fixed_cap = q->frontier_cap_rows != 0;
-        uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
-        Ensure(q, q->front[0], cap_rows * 2);
-        // Step 1: seeds (retry once with a buffer of the exact size on overflow).
-        uint64_t n = 0;
-        for (int attempt = 0;; attempt++) {
-            const int ks = KBegin(q, CGBDSM_K_SEED);
-            LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
-            KEnd(q, ks);
-            ReadCounters(q);
-            n = q->h_cnt->rows_out;
-            if (!(q->h_cnt->flags & kFlagFrontier)) break;
-            if (attempt) Throw(CGBDSM_ERR_OVERFLOW, "seed frontier overflow");
-            Ensure(q, q->front[0], n * 2 + 64);
-            SetCounter(q, &Counters::rows_out, 0);
-            SetCounter(q, &Counters::flags, 0);
-        }
-        uint32_t depth = 2, cur = 0;
-        // Step 2: BFS levels while the frontier is small.
-        while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
-            const uint64_t want_rows =
-                fixed_cap ? q->frontier_cap_rows : std::min<uint64_t>(std::max<uint64_t>(n * 64, 1u << 20), 1ull << 28);
-            const uint64_t want_words = want_rows * (depth + 1);
-            Ensure(q, q->front[cur ^ 1], want_words);
-            SetCounter(q, &Counters::rows_out, 0);
-            SetCounter(q, &Counters::flags, 0);
-            const int ke = KBegin(q, CGBDSM_K_EXTEND);
-            LaunchExtend(T, q->front[cur].p, n, depth, q->front[cur ^ 1].p, q->front[cur ^ 1].cap / (depth + 1),
-                         q->d_cnt, q->s);
-            KEnd(q, ke);
-            ReadCounters(q);
-            KAdd(q, CGBDSM_K_EXTEND, 0, n);
-            if (q->h_cnt->flags & kFlagFrontier) break;  // keep this level, count it by DFS
-            n = q->h_cnt->rows_out;
-            cur ^= 1;
-            depth++;
-        }
+        const Frontier f = BuildFrontier(q);
         // Step 5: final DFS count from the current level.
-        for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
-                       &Counters::idle_ticks})
-            SetCounter(q, f, 0);
-        RunDfs(q, q->front[cur].p, n, depth, nullptr, 0);
-        ReadCounters(q);
-        if (q->h_cnt->flags & (kFlagSteal | kFlagAborted))
-            Throw(CGBDSM_ERR_OVERFLOW, "final DFS did not complete (flags " + std::to_string(q->h_cnt->flags) + ")");
-        if (n) KAdd(q, CGBDSM_K_DFS, 4ull * (q->h_cnt->scanned + q->h_cnt->probes), n);
-        q->st.last_dfs_scanned = q->h_cnt->scanned;
-        q->st.last_dfs_probes = q->h_cnt->probes;
-        q->st.last_dfs_donated = q->h_cnt->donated;
-        q->st.last_dfs_busy = q->h_cnt->wave_ticks
-                                  ? 1.0 - (double)q->h_cnt->idle_ticks / (double)q->h_cnt->wave_ticks
-                                  : 0.0;
-        unsigned long long total = 0;
-        for (uint32_t e = 0; e < q->qe; e++) {
-            total += q->h_cnt->per_edge[e];
-            if (per_edge) per_edge[e] = q->h_cnt->per_edge[e];
+        ResetDfsCounters(q);
+        RunDfs(q, q->front[f.cur].p, f.n, f.depth, nullptr, 0);
+        CheckDfs(q, f.n);
+        FinishMatching(q, f, q, num_matches, per_edge);
+    });
+}
+
+cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned long long* num_matches,
+                                    uint64_t* const* per_edge) {
+    return Guard([&] {
+        if (!qs || nq <= 0 || nq > CGBDSM_MAX_BATCH_QUERIES) Throw(CGBDSM_ERR_INVALID, "bad query batch");
+        for (int i = 0; i < nq; i++) {
+            if (!qs[i] || qs[i]->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
+            if (qs[i]->ctx != qs[0]->ctx) Throw(CGBDSM_ERR_INVALID, "batched queries must share one context");
+            for (int k = 0; k < i; k++)
+                if (qs[k] == qs[i]) Throw(CGBDSM_ERR_INVALID, "query listed twice in a batch");
         }
-        if (num_matches) *num_matches += total;
-        q->last_depth = depth;
-        q->last_rows = n;
-        q->last_buf = cur;
-        q->st.last_frontier_rows = n;
-        q->st.last_frontier_depth = depth;
-        q->st.last_matches = total;
-        q->phase = 5;
+        // Seeds and BFS levels, one query after another (small kernels).
+        std::vector<Frontier> fr(nq);
+        for (int i = 0; i < nq; i++) fr[i] = BuildFrontier(qs[i]);
+        // One final-DFS launch for all of them, on the first query's stream
+        // and work queue.  Every frontier is complete (BuildFrontier read its
+        // counters back), so the other streams hold no pending work the
+        // launch depends on.
+        cgbdsm_query* lead = qs[0];
+        lead->h_jobs.clear();
+        lead->h_rowbase.assign(1, 0ull);
+        for (int i = 0; i < nq; i++) {
+            if (!fr[i].n) continue;
+            lead->h_jobs.push_back(MakeJob(qs[i], qs[i]->front[fr[i].cur].p, fr[i].depth));
+            lead->h_rowbase.push_back(lead->h_rowbase.back() + fr[i].n);
+        }
+        ResetDfsCounters(lead);
+        RunDfsJobs(lead, nullptr, 0);
+        CheckDfs(lead, lead->h_rowbase.back());
+        for (int i = 0; i < nq; i++) {
+            if (qs[i] != lead) ReadCounters(qs[i]);
+            FinishMatching(qs[i], fr[i], lead, num_matches ? &num_matches[i] : nullptr,
+                           per_edge ? per_edge[i] : nullptr);
+        }
     });
 }
 

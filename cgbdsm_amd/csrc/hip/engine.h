@@ -91,8 +91,6 @@ struct StealQueue {
     const uint32_t* abort;    // host-mapped coherent flag, non-zero asks kernels to stop
     long long poll_ticks;     // wall-clock ticks between a wave's reads of shared queue words
     long long host_ticks;     // wall-clock ticks between a wave's reads of the host flag
-    uint32_t shortest;        // scan the shortest backward-neighbour list when lists are sets
-    uint32_t suffix;          // count the product suffix of the order in closed form
 };
 
 // Per-query parameter block in device memory (read through scalar loads).

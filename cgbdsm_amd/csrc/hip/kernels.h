@@ -52,9 +52,23 @@ void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, Counters* cnt
 void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);
 void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out,
                   uint64_t cap_rows, Counters* cnt, hipStream_t s);
-// Final DFS over n rows of `depth` words; counts per edge tag into cnt->per_edge.
-// tuples != nullptr: also writes full embeddings (qv words + tag word) up to tup_cap.
-void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
+// One query's share of a final-DFS launch: its index tables, its frontier
+// rows (depth D, word 0 = ei tag | v0), its counters (per-edge counts,
+// duplicate flag of its update lists) and its counting options.
+enum : uint32_t {
+    kJobShortest = 1u,  // scan the shortest backward-neighbour list when lists are sets
+    kJobSuffix = 2u,    // count the product suffix of the order in closed form
+};
+struct DfsJob {
+    Tables T;
+    const uint32_t* rows;
+    Counters* cnt;
+    uint32_t D;
+    uint32_t opts;  // kJob* bits
+};
+// Final DFS over the rows of njobs queries (row_base[j] = first global row of
+// job j, row_base[njobs] = all rows); `cnt` holds the shared work queue.
+void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
 uint32_t DfsBlocksPerCU();
 

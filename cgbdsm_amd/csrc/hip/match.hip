@@ -22,8 +22,12 @@
 #include "wave.h"
 
 // Register budget of the final DFS (occupancy against spills), a build-time
-// tuning knob: -DCGD_DFS_WAVES_PER_EU=N asks for N resident waves per SIMD.
-#if defined(CGD_DFS_WAVES_PER_EU) && CGD_DFS_WAVES_PER_EU > 0
+// tuning knob: -DCGD_DFS_WAVES_PER_EU=N asks for N resident waves per SIMD
+// (3: about 168 VGPRs with a handful of spills; 0: no cap).
+#ifndef CGD_DFS_WAVES_PER_EU
+#define CGD_DFS_WAVES_PER_EU 3
+#endif
+#if CGD_DFS_WAVES_PER_EU > 0
 #define CGD_DFS_ATTR __attribute__((amdgpu_waves_per_eu(CGD_DFS_WAVES_PER_EU)))
 #else
 #define CGD_DFS_ATTR
@@ -99,6 +103,53 @@ __device__ void FillPlanLds(const Tables& T, PlanLds* L) {
         L->nbn = nbn;
     }
     __syncthreads();
+}
+
+// The same tables filled by one wavefront into its own copy (the final DFS
+// switches between queries per wavefront).
+__device__ void FillPlanLdsWave(const Tables& T, PlanLds* L) {
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t t = LaneId();
+    for (uint32_t i = t; i < (uint32_t)(kMaxQV * kMaxQV); i += kWave) {
+        const uint32_t d = i / kMaxQV, k = i % kMaxQV;
+        uint16_t v = 0;
+        if (d < p.qv && k < (uint32_t)(p.go_bni_offs[d + 1] - p.go_bni_offs[d])) {
+            const uint32_t bq = p.go_bni[p.go_bni_offs[d] + k];
+            v = (uint16_t)(p.eidx[p.go_vs[bq] * kMaxQV + p.go_vs[d]] | (bq << 8));
+        }
+        L->pair[d][k] = v;
+    }
+    for (uint32_t e = t; e < (uint32_t)kMaxQE; e += kWave) {
+        uint64_t a0 = 0, a1 = 0;
+        uint32_t ones = 0, own = 0;
+        if (e < p.qe) {
+            for (uint32_t dir = 0; dir < p.ndir; dir++)
+                if (p.dir_to_edge[dir] < e) {
+                    if (dir < 64) a0 |= 1ull << dir;
+                    else a1 |= 1ull << (dir - 64);
+                }
+            for (uint32_t q = 0; q < p.qv; q++)
+                if (T.P->all_ones[e][q]) ones |= 1u << q;
+            own = (uint32_t)T.P->sa[e] | ((uint32_t)T.P->sb[e] << 8);
+        }
+        L->app[e][0] = a0;
+        L->app[e][1] = a1;
+        L->ones[e] = ones;
+        L->own[e] = own;
+    }
+    if (t < (uint32_t)kMaxQV) L->dep_mask[t] = T.P->dep_mask[t];
+    if (t == 0) {
+        uint64_t vs = 0, cls = 0, nbn = 0;
+        for (uint32_t d = 0; d < p.qv; d++) {
+            vs |= (uint64_t)p.go_vs[d] << (4 * d);
+            cls |= (uint64_t)p.go_lcls[d] << (4 * d);
+            nbn |= (uint64_t)(p.go_bni_offs[d + 1] - p.go_bni_offs[d]) << (4 * d);
+        }
+        L->vs = vs;
+        L->cls = cls;
+        L->nbn = nbn;
+    }
+    WaveSync();
 }
 
 // Uniform view of the plan for one edge tag.
@@ -652,32 +703,65 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
 //    non-adjacent) is never enumerated: a partial match of the positions
 //    before it adds SuffixCount.
 template <bool kTuples>
-__global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const uint32_t* __restrict__ rows, uint64_t n_rows,
-                                                 uint32_t D, uint32_t grab, Counters* __restrict__ cnt, StealQueue Q,
-                                                 uint32_t* __restrict__ tuples, uint64_t tup_cap) {
-    // Per-edge counts gather in LDS and reach global memory once per block:
-    // a device-scope atomic per item on one address would serialize.
-    __shared__ unsigned long long s_pe[kMaxQE];
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) s_pe[t] = 0;
+__global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __restrict__ jobs, uint32_t njobs,
+                                                              const unsigned long long* __restrict__ row_base,
+                                                              uint32_t grab, Counters* __restrict__ cnt, StealQueue Q,
+                                                              uint32_t* __restrict__ tuples, uint64_t tup_cap) {
+    // Per-edge counts gather in LDS per wave and reach the query's counters
+    // when the wave changes query or ends: a device-scope atomic per item on
+    // one address would serialize.
+    __shared__ unsigned long long s_pe[kWpb][kMaxQE];
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
     __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
     __shared__ uint32_t s_surv[kWpb][kMaxQV][kWave];
+    __shared__ PlanLds s_plan[kWpb];
+    __shared__ uint32_t s_csize[kWpb][kMaxQV];
+    __shared__ uint32_t s_stab[kWpb][kStabCap];
     const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
     uint32_t* matched = s_matched[wv];
     DfsLevel* lvl = s_lvl[wv];
-    __shared__ PlanLds s_plan;
-    FillPlanLds(T, &s_plan);
-    Ctx X = MakeCtx(T, &s_plan);
-    const uint32_t qv = X.qv, last = qv - 1;
-    const uint32_t tag_shift = T.P->tag_shift, v_mask = T.P->v_mask;
-    __shared__ uint32_t s_csize[kWpb][kMaxQV];
-    __shared__ uint32_t s_stab[kWpb][kStabCap];
-    StabCache sc{s_stab[threadIdx.x / kWave], 0, 0, 0, T.P->stab_pos, T.P->stab_var_k, T.P->stab_deps};
-    // Every list is a set when the update lists hold no duplicate value.
-    const bool lists_are_sets = Uniform64(cnt->u_dups) == 0;
-    const bool sets = Q.shortest && lists_are_sets;
-    const uint32_t cps = T.P->cp_start;
-    const bool cart = !kTuples && lists_are_sets && Q.suffix && cps < qv;
+    unsigned long long* pe = s_pe[wv];
+    for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave) pe[t] = 0;
+    const uint64_t n_rows = row_base[njobs];
+    // The current query of this wavefront and what depends on it.
+    uint32_t job = 0xFFFFFFFFu, D = 0, qv = 0, last = 0, tag_shift = 0, v_mask = 0, cps = 0;
+    Tables T = jobs[0].T;
+    const uint32_t* rows = nullptr;
+    Counters* jc = nullptr;
+    Ctx X = MakeCtx(T, &s_plan[wv]);
+    StabCache sc{s_stab[wv], 0, 0, 0, 0xFFu, 0, 0};
+    bool sets = false, cart = false;
+    auto flush_pe = [&]() {
+        if (!jc) return;
+        for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave)
+            if (pe[t]) {
+                atomicAdd(&jc->per_edge[t], pe[t]);
+                pe[t] = 0;
+            }
+        WaveSync();
+    };
+    auto use_job = [&](uint32_t j) {
+        if (j == job) return;
+        flush_pe();
+        job = j;
+        T = jobs[j].T;
+        rows = jobs[j].rows;
+        jc = jobs[j].cnt;
+        D = jobs[j].D;
+        FillPlanLdsWave(T, &s_plan[wv]);
+        X = MakeCtx(T, &s_plan[wv]);
+        qv = X.qv;
+        last = qv - 1;
+        tag_shift = T.P->tag_shift;
+        v_mask = T.P->v_mask;
+        sc = StabCache{s_stab[wv], 0, 0, 0, T.P->stab_pos, T.P->stab_var_k, T.P->stab_deps};
+        // Every list is a set when the update lists hold no duplicate value.
+        const bool lists_are_sets = Uniform64(jc->u_dups) == 0;
+        const uint32_t opts = jobs[j].opts;
+        sets = (opts & kJobShortest) && lists_are_sets;
+        cps = T.P->cp_start;
+        cart = !kTuples && lists_are_sets && (opts & kJobSuffix) && cps < qv;
+    };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
     uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
@@ -731,9 +815,18 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
             if (primary) {
                 const uint64_t r = r_next++;
                 from_row = true;
-                const uint32_t w0 = Uniform(rows[r * D]);
+                // the query this row belongs to
+                uint32_t jlo = 0, jhi = njobs;
+                while (jhi - jlo > 1) {
+                    const uint32_t m = (jlo + jhi) >> 1;
+                    if (row_base[m] <= r) jlo = m;
+                    else jhi = m;
+                }
+                use_job(jlo);
+                const uint64_t lr = r - row_base[jlo];
+                const uint32_t w0 = Uniform(rows[lr * D]);
                 ei = w0 >> tag_shift;
-                if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : rows[r * D + lane];
+                if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : rows[lr * D + lane];
                 D0 = D;
                 got = true;
             }
@@ -764,7 +857,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                 D0 = Uniform(GetAgent(&it->depth));
                 const uint32_t tag = Uniform(GetAgent(&it->ei));
                 ei = tag & 0xFFu;
-                sel0 = tag >> 8;
+                sel0 = (tag >> 8) & 0xFFu;
+                use_job(tag >> 16);
                 lo0 = Uniform(GetAgent(&it->lo));
                 hi0 = Uniform(GetAgent(&it->hi));
                 if (lane < D0) matched[lane] = GetAgent(&it->v[lane]);
@@ -837,7 +931,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                                 const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                                 StealItem* it = &Q.items[slot];
                                 PutAgent(&it->depth, d + 1);
-                                PutAgent(&it->ei, ei | (kNoSel << 8));
+                                PutAgent(&it->ei, ei | (kNoSel << 8) | (job << 16));
                                 for (uint32_t f = 0; f < d; f++) PutAgent(&it->v[f], matched[f]);
                                 PutAgent(&it->v[d], s_surv[wv][d][ccount - k + lane]);
                                 PutAgent(&it->lo, 0);
@@ -857,7 +951,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                         matched[d] = x;
                         lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel};
                     }
-                    cvalid &= ~Uniform(s_plan.dep_mask[d]);
+                    cvalid &= ~Uniform(s_plan[wv].dep_mask[d]);
                     if ((sc.deps >> d) & 1u) sc.state = 0;
                     WaveSync();
                     d++;
@@ -905,7 +999,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
                         const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                         StealItem* it = &Q.items[slot];
                         PutAgent(&it->depth, d);
-                        PutAgent(&it->ei, ei | (sel << 8));
+                        PutAgent(&it->ei, ei | (sel << 8) | (job << 16));
                         for (uint32_t f = 0; f < d; f++) PutAgent(&it->v[f], matched[f]);
                         const uint32_t plo = off + (lane + 1) * piece;
                         PutAgent(&it->lo, plo);
@@ -955,7 +1049,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
             WaveSync();
         }
         if (lane == 0) {
-            if (count) atomicAdd(&s_pe[ei], count);
+            if (count) pe[ei] += count;
             // Rows retire from `pending` a whole grab at a time, stolen items one by one.
             if (!from_row) atomicSub(&cnt->q_pending.v, 1ull);
             else if (r_next >= r_end) atomicSub(&cnt->q_pending.v, (unsigned long long)(r_end - r_first));
@@ -976,9 +1070,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(Tables T, const ui
         atomicAdd(&cnt->probes, p_tot);
         if (donated) atomicAdd(&cnt->donated, donated);
     }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x)
-        if (s_pe[t]) atomicAdd(&cnt->per_edge[t], s_pe[t]);
+    flush_pe();
 }
 
 // Before a DFS launch: tickets taken but never filled in the previous launch
@@ -1025,17 +1117,17 @@ uint32_t DfsBlocksPerCU() {
 
 void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>(cnt); }
 
-void LaunchDfs(const Tables& T, const uint32_t* rows, uint64_t n, uint32_t depth, Counters* cnt,
+void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
-    if (!n) return;
-    const unsigned g = Grid1(n, kWpb, grid_blocks);
+    if (!n_rows || !njobs) return;
+    const unsigned g = Grid1(n_rows, kWpb, grid_blocks);
     // Rows per cursor atomic (4 measured best on dense-6 streams: larger grabs
     // leave a tail that stealing has to repair).
     const uint32_t grab = 4;
     if (tuples)
-        KDfs<true><<<g, kThreads, 0, s>>>(T, rows, n, depth, grab, cnt, Q, tuples, tup_cap);
+        KDfs<true><<<g, kThreads, 0, s>>>(jobs, njobs, row_base, grab, cnt, Q, tuples, tup_cap);
     else
-        KDfs<false><<<g, kThreads, 0, s>>>(T, rows, n, depth, grab, cnt, Q, nullptr, 0);
+        KDfs<false><<<g, kThreads, 0, s>>>(jobs, njobs, row_base, grab, cnt, Q, nullptr, 0);
 }
 
 }  // namespace cgd

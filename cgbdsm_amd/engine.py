@@ -50,6 +50,18 @@ class Engine:
     def query(self, tables):
         return DeviceQuery(self, tables)
 
+    def matching_batch(self, queries):
+        """Matching of several queries of this engine (each after
+        build_local_index) with one final-DFS launch; returns a list of
+        (count, per_edge) in the order given."""
+        n = len(queries)
+        hs = (C.c_void_p * n)(*[q.h.value for q in queries])
+        counts = (C.c_ulonglong * n)()
+        pers = [np.zeros(N.MAX_QE, dtype=np.uint64) for _ in queries]
+        pp = (C.c_void_p * n)(*[p.ctypes.data for p in pers])
+        _check(self.L.cgbdsm_matching_batch(hs, n, counts, pp))
+        return [(int(counts[i]), pers[i][: q.qe].copy()) for i, q in enumerate(queries)]
+
     def close(self):
         if self.h:
             _check(self.L.cgbdsm_ctx_destroy(self.h))

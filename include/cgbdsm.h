@@ -30,6 +30,7 @@ extern "C" {
 #define CGBDSM_MAX_QV 12
 #define CGBDSM_MAX_QE 42
 #define CGBDSM_MAX_DIR (2 * CGBDSM_MAX_QE)
+#define CGBDSM_MAX_BATCH_QUERIES 256
 #define CGBDSM_NO_EDGE 0xFFu
 
 typedef enum {
@@ -190,6 +191,20 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q);
  * positive matches to *num_matches; per_edge[qe] (optional) receives the
  * count attributed to each query edge (the 5-bit edge tag). */
 cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge);
+
+/* cgbdsm_matching for nq registered queries of one context at once: each
+ * query's seeds and BFS levels run on its own stream, then ONE persistent
+ * final-DFS launch counts all of them from a shared device work queue (the
+ * first query's), so that the queries' uneven tails fill each other's idle
+ * compute units.  Adds query i's matches to num_matches[i]; per_edge[i]
+ * (optional array, entries optional) receives its per-edge counts.  Counts
+ * are identical to nq separate cgbdsm_matching calls.  Every query must be
+ * in the state cgbdsm_matching expects; the deadline and the statistics of
+ * the launch are the first query's (last_dfs_* are stored in each query).
+ * No reference counterpart: the reference runs one query per process
+ * (main_calig.cpp:29-35).  nq <= CGBDSM_MAX_BATCH_QUERIES. */
+cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned long long* num_matches,
+                                    uint64_t* const* per_edge);
 
 /* Post-match M <- M u Upd: CaLiGHelperGamma::BatchUpdateGlobalIndex
  * (index/calig_helper_gamma.h:98, index/calig_helper_gamma.cu:539-567). */

@@ -1,0 +1,130 @@
+"""Batched matching: several registered queries over one update stream are
+counted by ONE final-DFS launch (cgbdsm_matching_batch); every query's
+count and per-edge split must equal the CPU oracle's, batch by batch."""
+import numpy as np
+import pytest
+
+from cgbdsm_amd.engine import CgbdsmError, Engine
+from cgbdsm_amd.host import HostSession, synth_spec
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+# (query kind, query size, query seed) over the same data graph and stream
+MIX = [("dense", 5, 2), ("tree", 5, 3), ("sparse", 6, 4), ("dense", 4, 5), ("tree", 6, 6), ("dense", 6, 7)]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _open(eng, mix, scale=11, batch_edges=300, donate_min=None):
+    out = []
+    for kind, size, qseed in mix:
+        spec = synth_spec(scale, 8, 4, seed=2, query_size=size, query_kind=kind, query_seed=qseed)
+        host = HostSession(spec=spec, batch_edges=batch_edges, threads=4)
+        keys, dv, card, avg = host.initial_keys()
+        q = eng.query(host.query_tables)
+        if donate_min is not None:
+            q.set_match_params(donate_min=donate_min)
+        q.index_init(keys, dv)
+        plan = host.make_plan(card, avg)
+        q.set_plan(plan)
+        idx = O.Index(plan.ndir, host.ebits)
+        idx.set_dv(dv)
+        idx.init(keys)
+        out.append((host, q, idx, plan))
+    return out
+
+
+@pytest.mark.parametrize("donate_min", [None, 1, 0])
+def test_matching_batch_parity(eng, donate_min):
+    print(f"case batch donate_min={donate_min}", flush=True)
+    sess = _open(eng, MIX, donate_min=donate_min)
+    nb = min(h.num_batches for h, _, _, _ in sess)
+    assert nb >= 3
+    totals = [0] * len(sess)
+    for b in range(nb):
+        live = []
+        for k, (host, q, idx, plan) in enumerate(sess):
+            kb = host.next_batch()
+            q.stage(kb)
+            idx.set_dv(kb.dv)
+            q.transfer_update_all()
+            idx.insert(kb.ins)
+            idx.remove(kb.dele)
+            nonempty = q.transfer_update_index()
+            idx.rebuild_update(kb.upd)
+            if nonempty:
+                q.build_local_index()
+                live.append((k, kb))
+        # vary the leader: rotate the order of the batch's queries
+        live = live[b % max(1, len(live)):] + live[: b % max(1, len(live))]
+        got = eng.matching_batch([sess[k][1] for k, _ in live])
+        for (k, kb), (cnt, per) in zip(live, got):
+            host, q, idx, plan = sess[k]
+            ocnt, oper = idx.match(plan, kb.edge_ok, threads=4)
+            print(f"batch {b} query {k} ({MIX[k]}): {cnt} vs oracle {ocnt}", flush=True)
+            assert cnt == ocnt and np.array_equal(per, oper), (b, k, cnt, ocnt, list(per), list(oper))
+            st = q.stats()
+            assert st["last_matches"] == cnt
+            totals[k] += cnt
+        for k, kb in live:
+            sess[k][1].batch_update_global_index()
+            sess[k][2].insert(kb.upd)
+    print(f"totals {totals}", flush=True)
+    assert sum(totals) > 0
+    for _, q, _, _ in sess:
+        q.close()
+
+
+def test_matching_batch_single_and_tuples(eng):
+    """A batch of one query equals cgbdsm_matching; the embeddings of a query
+    counted in a batch are available afterwards."""
+    print("case batch single", flush=True)
+    sess = _open(eng, [("dense", 4, 9)], scale=9, batch_edges=150)
+    host, q, idx, plan = sess[0]
+    for _ in range(host.num_batches):
+        kb = host.next_batch()
+        q.stage(kb)
+        idx.set_dv(kb.dv)
+        q.transfer_update_all()
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        ne = q.transfer_update_index()
+        idx.rebuild_update(kb.upd)
+        if ne:
+            q.build_local_index()
+            [(cnt, per)] = eng.matching_batch([q])
+            ocnt, oper, otup, otei = idx.match(plan, kb.edge_ok, tuples_cap=1 << 16)
+            assert cnt == ocnt and np.array_equal(per, oper)
+            n, tup, tei = q.tuples(1 << 16)
+            assert n == ocnt
+            a = np.concatenate([tup, tei[:, None].astype(np.uint32)], axis=1)
+            bb = np.concatenate([otup, otei[:, None].astype(np.uint32)], axis=1)
+            assert np.array_equal(a[np.lexsort(a.T[::-1])], bb[np.lexsort(bb.T[::-1])])
+            q.batch_update_global_index()
+        idx.insert(kb.upd)
+    q.close()
+
+
+def test_matching_batch_rejects(eng):
+    print("case batch rejects", flush=True)
+    sess = _open(eng, [("tree", 4, 11), ("tree", 4, 12)], scale=8, batch_edges=None)
+    (h0, q0, _, _), (h1, q1, _, _) = sess
+    with pytest.raises(CgbdsmError):
+        eng.matching_batch([q0])             # not after build_local_index
+    for host, q, _, _ in sess:
+        q.stage(host.next_batch())
+        q.transfer_update_all()
+        if q.transfer_update_index():
+            q.build_local_index()
+    with pytest.raises(CgbdsmError):
+        eng.matching_batch([q0, q0])         # listed twice
+    with pytest.raises(CgbdsmError):
+        eng.matching_batch([])
+    for _, q, _, _ in sess:
+        q.close()
