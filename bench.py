@@ -99,6 +99,36 @@ MQ_MIX = (("dense", 5, 101), ("tree", 5, 101), ("sparse", 5, 103), ("dense", 4, 
           ("tree", 4, 100), ("sparse", 4, 104), ("dense", 6, 103), ("tree", 6, 104))
 
 
+TUNE_GRABS = (1, 2, 4, 8, 16)
+TUNE_BPC = (0, 2)
+
+
+def tuned_batch(eng, qs, log):
+    """Stage 2 of the staged batch of each query in `qs` with the final-DFS
+    launch shape autotuned on it (cgbdsm_amd.tune); returns the chosen shape,
+    the DFS time per candidate and the batch's counts."""
+    from cgbdsm_amd.tune import autotune
+
+    live = []
+    for q in qs:
+        q.transfer_update_all()
+        if q.transfer_update_index():
+            q.build_local_index()
+            live.append(q)
+    res = {"rows_per_grab": None, "blocks_per_cu": None, "dfs_ms": {}, "counts": [0] * len(qs)}
+    if live:
+        best, table, got = autotune(eng, live, grabs=TUNE_GRABS, blocks_per_cu=TUNE_BPC, log=log)
+        res.update(rows_per_grab=best[0], blocks_per_cu=best[1],
+                   dfs_ms={f"{g}/{b}": round(ms, 3) for (g, b), ms in table.items()})
+        for q, (c, _) in zip(live, got):
+            res["counts"][qs.index(q)] = c
+        for q in live:
+            q.batch_update_global_index()
+    for q in qs:
+        q.sync()
+    return res
+
+
 def multi_query(args, threads, log):
     """Several registered queries of uneven size over the same update stream,
     each with its own device query (index, stream).  Per batch:
@@ -115,6 +145,7 @@ def multi_query(args, threads, log):
 
     from cgbdsm_amd.engine import Engine
     from cgbdsm_amd.host import HostSession, synth_spec
+    from cgbdsm_amd.tune import autotune
 
     n = min(args.multi_queries, len(MQ_MIX))
     mix, sessions = [], []
@@ -145,6 +176,8 @@ def multi_query(args, threads, log):
             sets[name].append(q)
     log(f"multi-query: {n} queries {mix}, {warm} warmup + {steps} timed batches")
     pool = ThreadPoolExecutor(max_workers=n)
+    tuned = None
+    phases = []
 
     def one(q):
         q.stage_next()
@@ -156,6 +189,7 @@ def multi_query(args, threads, log):
         if not q.transfer_update_index():
             return False
         q.build_local_index()
+        q.build_frontier()
         return True
 
     res = {}
@@ -169,15 +203,29 @@ def multi_query(args, threads, log):
                 out = list(pool.map(one, qs))
             else:
                 live = list(pool.map(prepare, qs))
+                t1 = time.perf_counter()
                 out = [0] * n
                 ks = [k for k in range(n) if live[k]]
+                if ks and b == 0 and warm > 0 and not args.no_autotune:
+                    tuning = autotune(eng, [qs[k] for k in ks], grabs=TUNE_GRABS, blocks_per_cu=TUNE_BPC, log=log)
+                    tuned = {"rows_per_grab": tuning[0][0], "blocks_per_cu": tuning[0][1]}
+                    t1 = time.perf_counter()
                 if ks:
+                    lead = qs[ks[0]]
+                    if b >= warm:
+                        lead.profile(True)
+                        lead.kernel_times(reset=True)
                     for k, (c, _) in zip(ks, eng.matching_batch([qs[k] for k in ks])):
                         out[k] = c
-                    for k in ks:
-                        qs[k].batch_update_global_index()
+                    t2 = time.perf_counter()
+                    list(pool.map(lambda k: qs[k].batch_update_global_index(), ks))
                 for q in qs:
                     q.sync()
+                if b >= warm and ks:
+                    kt = lead.kernel_times()
+                    lead.profile(False)
+                    phases.append((t1 - t0, t2 - t1, time.perf_counter() - t2, kt["dfs"]["ms"],
+                                   lead.stats()["last_dfs_busy"]))
             dt = 1e3 * (time.perf_counter() - t0)
             if b >= warm:
                 lat.append(dt)
@@ -195,6 +243,15 @@ def multi_query(args, threads, log):
     out["served_vs_plain"] = round(out["served"]["update_edges_per_s"] / out["plain"]["update_edges_per_s"], 4)
     out["served_threads_vs_plain"] = round(out["served_threads"]["update_edges_per_s"] /
                                            out["plain"]["update_edges_per_s"], 4)
+    out["served_dfs_tuning"] = tuned
+    if phases:
+        # wall ms per batch: index maintenance + masks of all queries
+        # (concurrent), batched matching (seeds + one DFS launch), commits
+        out["served_phase_ms"] = {k: round(1e3 * statistics.mean(p[i] for p in phases), 3)
+                                  for i, k in enumerate(("prepare", "matching", "commit"))}
+        out["served_phase_ms"]["dfs_kernel"] = round(statistics.mean(p[3] for p in phases), 3)
+        out["served_dfs_busy"] = round(statistics.mean(p[4] for p in phases), 4)
+        log(f"multi-query served phases: {out['served_phase_ms']}")
     for qs in sets.values():
         for q in qs:
             q.close()
@@ -288,6 +345,10 @@ def main():
     ap.add_argument("--query-seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--multi-only", action="store_true",
+                    help="run only the multi-query comparison (plain / served_threads / served)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default final-DFS launch shape instead of tuning it on the first warmup batch")
     ap.add_argument("--deadline-ms", type=float, default=60_000)
     ap.add_argument("--donate-min", type=int, default=4, help="survivors left before a DFS wavefront donates (0: off)")
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
@@ -314,6 +375,11 @@ def main():
     threads = min(16, os.cpu_count() or 1)
     if args.index_stress:
         print(json.dumps(index_stress(args, threads, log)), flush=True)
+        return
+    if args.multi_only:
+        out = multi_query(args, threads, log)
+        out["metric"] = "update edges/s, served multi-query vs plain (exact counts)"
+        print(json.dumps(out), flush=True)
         return
     t_setup = time.perf_counter()
     spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
@@ -353,8 +419,13 @@ def main():
         enq_ms.append(1e3 * (time.perf_counter() - t_e))
 
     counts, lat_ms = [], []
+    tuning = None
     for i in range(warm):
         q.stage_next()
+        if i == 0 and not args.no_autotune:
+            tuning = tuned_batch(eng, [q], log)
+            counts.append(tuning.pop("counts")[0])
+            continue
         c, _, _ = q.run_batch()
         counts.append(c)
     q.profile(True)
@@ -434,7 +505,7 @@ def main():
         "kernels_ms": {k: round(v["ms"], 4) for k, v in kt.items()},
         "kernels_gbs": {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 3) for k, v in kt.items() if v["ms"] > 0 and v["bytes"]},
         "dfs_totals": dfs,
-        "roofline": roofline, "cpu_baseline": cpu, "multi_query": mq,
+        "roofline": roofline, "cpu_baseline": cpu, "multi_query": mq, "dfs_tuning": tuning,
     }
     print(json.dumps(line), flush=True)
 

@@ -146,6 +146,8 @@ def dev_lib():
         L.cgbdsm_sync.argtypes = [P]
         L.cgbdsm_set_deadline_ms.argtypes = [P, C.c_double]
         L.cgbdsm_set_dfs_options.argtypes = [P, C.c_int, C.c_int]
+        L.cgbdsm_set_dfs_tuning.argtypes = [P, U32, U32]
+        L.cgbdsm_build_frontier.argtypes = [P]
         L.cgbdsm_query_set_layout.argtypes = [P, C.c_int]
         L.cgbdsm_profile.argtypes = [P, I]
         L.cgbdsm_kernel_times.argtypes = [P, C.c_void_p, I]
@@ -162,6 +164,6 @@ DEV_SYMBOLS = [
     "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_matching_batch", "cgbdsm_batch_update_global_index",
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
     "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
-    "cgbdsm_profile", "cgbdsm_query_set_layout",
+    "cgbdsm_profile", "cgbdsm_query_set_layout", "cgbdsm_set_dfs_tuning", "cgbdsm_build_frontier",
     "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
 ]

@@ -30,6 +30,13 @@ struct cgbdsm_ctx {
     int wall_khz = 100000;  // device wall clock (wall_clock64) rate
 };
 
+// Rows left for the final DFS after seeds and BFS levels: count, depth and
+// which of the two frontier buffers holds them.
+struct Frontier {
+    uint64_t n = 0;
+    uint32_t depth = 0, cur = 0;
+};
+
 struct cgbdsm_query {
     cgbdsm_ctx* ctx = nullptr;
     hipStream_t s = nullptr;
@@ -49,11 +56,17 @@ struct cgbdsm_query {
     DBuf<uint8_t> scan_scratch;
     DBuf<uint64_t> masks;
     DBuf<uint32_t> front[2];
+    DBuf<unsigned long long> tag_hist;  // 2 * kMaxQE counters of the seed grouping
     uint64_t n_ins = 0, n_del = 0, n_upd = 0, u_rows_upper = 0;
     uint8_t edge_ok[kMaxQE] = {};
     uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
     uint32_t donate_min = 4;
     bool dfs_shortest = true, dfs_suffix = true;
+    // Final-DFS launch shape: frontier rows taken per cursor atomic, and
+    // resident blocks per CU (0: as many as fit).  4 rows measured best on
+    // dense 6-vertex streams (larger grabs leave a tail that stealing has to
+    // repair); cgbdsm_set_dfs_tuning changes both.
+    uint32_t dfs_grab = 4, dfs_bpc = 0;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;
@@ -63,6 +76,10 @@ struct cgbdsm_query {
     int phase = 0;  // 0 idle, 1 staged, 2 applied, 3 U built, 4 masks, 5 matched
     bool u_nonempty = false;
     uint32_t last_depth = 0, last_buf = 0;
+    // Frontier of the current batch (valid from cgbdsm_build_frontier or the
+    // first count until the next build_local_index).
+    Frontier fr;
+    bool fr_ready = false;
     uint64_t last_rows = 0;
     cgbdsm_stats st{};
     hipEvent_t ev[8] = {};
@@ -429,8 +446,9 @@ void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
     CGD_HIP(hipMemcpyAsync(q->d_rowbase.p, q->h_rowbase.data(), q->h_rowbase.size() * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, q->s));
     const int k = tuples ? -1 : KBegin(q, CGBDSM_K_DFS);
+    const uint32_t bpc = q->dfs_bpc ? std::min(q->dfs_bpc, DfsBlocksPerCU()) : DfsBlocksPerCU();
     LaunchDfs(q->d_jobs.p, (uint32_t)q->h_jobs.size(), q->d_rowbase.p, n, q->d_cnt, MakeSteal(q, tuples == nullptr),
-              tuples, tup_cap, q->ctx->num_cus * DfsBlocksPerCU(), q->s);
+              tuples, tup_cap, q->ctx->num_cus * bpc, q->dfs_grab, q->s);
     KEnd(q, k);
     CGD_HIP(hipGetLastError());
     if (q->deadline_ms > 0) {
@@ -733,6 +751,7 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
         Release(q, q->pma.meta);
         Release(q, q->d_jobs);
         Release(q, q->d_rowbase);
+        Release(q, q->tag_hist);
         for (auto& r : q->fifo) {
             Release(q, r.ins);
             Release(q, r.del);
@@ -1121,6 +1140,7 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q) {
             KAdd(q, CGBDSM_K_MASKS, q->h_cnt->bytes + (tiles + 2ull * L.n) * ((q->dv + 7) / 8), tiles);
         }
         q->phase = 4;
+        q->fr_ready = false;
     });
 }
 
@@ -1131,6 +1151,14 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit, u
         q->bfs_row_limit = bfs_row_limit;
         q->frontier_cap_rows = frontier_cap;
         q->donate_min = donate_min;
+    });
+}
+
+cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu) {
+    return Guard([&] {
+        if (!q || rows_per_grab > 1024) Throw(CGBDSM_ERR_INVALID, "bad DFS tuning");
+        q->dfs_grab = rows_per_grab ? rows_per_grab : 4;
+        q->dfs_bpc = blocks_per_cu;
     });
 }
 
@@ -1149,18 +1177,14 @@ cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
     });
 }
 
-// Seeds and BFS levels of one query's matching (phase 4): returns the rows
-// left for the final DFS, their depth and the buffer holding them.
-struct Frontier {
-    uint64_t n;
-    uint32_t depth, cur;
-};
-
-Frontier BuildFrontier(cgbdsm_query* q) {
+// Seeds and BFS levels of one query's matching (phase 4): the rows left for
+// the final DFS, their depth and the buffer holding them, kept in q->fr.
+// Reads the counters back, so q's stream is idle on return.
+const Frontier& BuildFrontier(cgbdsm_query* q) {
+    if (q->fr_ready) return q->fr;
     const EiList L = ActiveEdges(q);
     for (auto f : {&Counters::rows_out, &Counters::flags, &Counters::tuples})
         SetCounter(q, f, 0);
-    FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, q->s);
     const Tables T = MakeTables(q);
     const bool fixed_cap = q->frontier_cap_rows != 0;
     uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
@@ -1180,6 +1204,16 @@ Frontier BuildFrontier(cgbdsm_query* q) {
         SetCounter(q, &Counters::flags, 0);
     }
     uint32_t depth = 2, cur = 0;
+    // Seeds come out grouped by owner; group them by edge tag.
+    if (n > 1 && q->qe > 1) {
+        Ensure(q, q->front[1], n * 2);
+        Ensure(q, q->tag_hist, 2 * kMaxQE);
+        FillU64((uint64_t*)q->tag_hist.p, 0, 2 * kMaxQE, q->s);
+        const int kg = KBegin(q, CGBDSM_K_SEED);
+        LaunchGroupByTag(q->front[0].p, n, q->hplan.tag_shift, q->tag_hist.p, q->front[1].p, q->s);
+        KEnd(q, kg);
+        cur = 1;
+    }
     // Step 2: BFS levels while the frontier is small.
     while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
         const uint64_t want_rows =
@@ -1199,7 +1233,16 @@ Frontier BuildFrontier(cgbdsm_query* q) {
         cur ^= 1;
         depth++;
     }
-    return Frontier{n, depth, cur};
+    q->fr = Frontier{n, depth, cur};
+    q->fr_ready = true;
+    return q->fr;
+}
+
+// Zeroes the per-edge counts (and the frontier/tuple flags) of `q` on the
+// stream `s` that will run the final DFS.
+void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
+    FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, s);
+    FillU64((uint64_t*)&q->d_cnt->tuples, 0, 1, s);
 }
 
 void ResetDfsCounters(cgbdsm_query* q) {
@@ -1241,11 +1284,19 @@ void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* laun
     q->phase = 5;
 }
 
+cgbdsm_status cgbdsm_build_frontier(cgbdsm_query* q) {
+    return Guard([&] {
+        if (!q || q->phase != 4) Throw(CGBDSM_ERR_INVALID, "build_frontier must follow build_local_index");
+        BuildFrontier(q);
+    });
+}
+
 cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge) {
     return Guard([&] {
-        if (!q || q->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
+        if (!q || (q->phase != 4 && q->phase != 5)) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
         const Frontier f = BuildFrontier(q);
         // Step 5: final DFS count from the current level.
+        ResetMatchCounters(q, q->s);
         ResetDfsCounters(q);
         RunDfs(q, q->front[f.cur].p, f.n, f.depth, nullptr, 0);
         CheckDfs(q, f.n);
@@ -1258,12 +1309,14 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
     return Guard([&] {
         if (!qs || nq <= 0 || nq > CGBDSM_MAX_BATCH_QUERIES) Throw(CGBDSM_ERR_INVALID, "bad query batch");
         for (int i = 0; i < nq; i++) {
-            if (!qs[i] || qs[i]->phase != 4) Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
+            if (!qs[i] || (qs[i]->phase != 4 && qs[i]->phase != 5))
+                Throw(CGBDSM_ERR_INVALID, "matching must follow build_local_index");
             if (qs[i]->ctx != qs[0]->ctx) Throw(CGBDSM_ERR_INVALID, "batched queries must share one context");
             for (int k = 0; k < i; k++)
                 if (qs[k] == qs[i]) Throw(CGBDSM_ERR_INVALID, "query listed twice in a batch");
         }
-        // Seeds and BFS levels, one query after another (small kernels).
+        // Seeds and BFS levels, one query after another, for queries whose
+        // frontier cgbdsm_build_frontier has not built already.
         std::vector<Frontier> fr(nq);
         for (int i = 0; i < nq; i++) fr[i] = BuildFrontier(qs[i]);
         // One final-DFS launch for all of them, on the first query's stream
@@ -1278,6 +1331,9 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
             lead->h_jobs.push_back(MakeJob(qs[i], qs[i]->front[fr[i].cur].p, fr[i].depth));
             lead->h_rowbase.push_back(lead->h_rowbase.back() + fr[i].n);
         }
+        // Per-query counts are zeroed on the launch's stream; every other
+        // query's stream is idle (BuildFrontier ended by reading its counters).
+        for (int i = 0; i < nq; i++) ResetMatchCounters(qs[i], lead->s);
         ResetDfsCounters(lead);
         RunDfsJobs(lead, nullptr, 0);
         CheckDfs(lead, lead->h_rowbase.back());

@@ -50,6 +50,10 @@ void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, Counters* cnt
 
 // match.hip
 void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);
+// Depth-2 rows (2 words, word 0 = tag << tag_shift | v0) of `in` copied to
+// `out` grouped by tag; hist = 2 * kMaxQE zeroed counters (scratch).
+void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t tag_shift, unsigned long long* hist, uint32_t* out,
+                      hipStream_t s);
 void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out,
                   uint64_t cap_rows, Counters* cnt, hipStream_t s);
 // One query's share of a final-DFS launch: its index tables, its frontier
@@ -69,7 +73,9 @@ struct DfsJob {
 // Final DFS over the rows of njobs queries (row_base[j] = first global row of
 // job j, row_base[njobs] = all rows); `cnt` holds the shared work queue.
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
-               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s);
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, uint32_t grab,
+               hipStream_t s);
+// Resident KDfs blocks per CU at full occupancy.
 uint32_t DfsBlocksPerCU();
 
 void LaunchCountDups(const uint64_t* k, uint64_t n, Counters* cnt, hipStream_t s);

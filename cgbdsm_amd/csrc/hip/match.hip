@@ -3,8 +3,10 @@
 //
 //   KSeed    depth-2 rows [(ei << tag_shift) | v0, v1] for every edge tag ei
 //            whose batch lists exist (writeInitialResultsAllBit,
-//            kernels/enumeration.cu:727-809).  One wavefront per 64 owners =
-//            one support-mask word; wave-wide scan + one atomic reserves rows.
+//            kernels/enumeration.cu:727-809), all tags in one pass: one
+//            wavefront per 64 owners reads the shared first-pair lists once,
+//            spreads their entries over lanes; scan + one atomic per 64
+//            entries reserves rows.
 //   KExtend  one BFS level, one wavefront per row, 64 candidates per step
 //            (extendBFSAllBit with write_res, kernels/enumeration.cu:811-1155).
 //   KDfs     the final count: persistent wavefronts pull rows from a global
@@ -239,52 +241,170 @@ __device__ __forceinline__ bool Accept(const Ctx& X, const Tables& T, uint32_t d
     return true;
 }
 
+// Depth-2 seeds of every active edge tag in one pass.  Position 0 -> 1 of
+// the global order is one query direction `idx` for every tag, so the list
+// of an owner v0 is shared: M(v0, idx) for tags other than idx's own edge
+// (plus U(v0, idx) for tags after that edge), U(v0, idx) alone for the own
+// edge.  A wavefront takes 64 owners (one support-mask word per tag), reads
+// their list records once, and spreads the concatenated list entries over
+// its lanes (an owner with a long list no longer stalls the other 63); each
+// entry then emits one row per tag whose masks hold both ends.
 __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t* __restrict__ rows, uint64_t cap_rows,
                                                   Counters* __restrict__ cnt) {
-    __shared__ PlanLds s_plan;
-    FillPlanLds(T, &s_plan);
-    Ctx X = MakeCtx(T, &s_plan);
-    const uint32_t ei = EL.ei[blockIdx.y];
-    SetTag(X, ei);
-    const uint32_t q0 = Vs(X, 0), q1 = Vs(X, 1);
-    const uint32_t idx = Pair(X, 1, 0) & 0xFFu;
-    const uint32_t lane = LaneId();
-    const uint32_t tag = ei << T.P->tag_shift;
-    const bool all0 = (X.ones >> q0) & 1u;
-    const uint64_t* tile0 = T.masks + ((uint64_t)ei * kMaxQV + q0) * T.W;
+    __shared__ const uint64_t* s_tile1[kMaxQE];
+    __shared__ uint32_t s_tag[kMaxQE];
+    __shared__ uint32_t s_incl[kWpb][kWave], s_lenm[kWpb][kWave], s_offu[kWpb][kWave];
+    __shared__ uint64_t s_m0[kWpb][kWave], s_offm[kWpb][kWave];
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t q0 = p.go_vs[0], q1 = p.go_vs[1];
+    const uint32_t idx = p.eidx[q0 * kMaxQV + q1];
+    const uint32_t own = p.dir_to_edge[idx];
+    const uint32_t n = EL.n;
+    const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
+    // Per slot k of the active-tag list (bit k of the masks below).
+    uint64_t ones0 = 0, ones1 = 0, use_m = 0, use_u = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t ei = EL.ei[k];
+        if (T.P->all_ones[ei][q0]) ones0 |= 1ull << k;
+        if (T.P->all_ones[ei][q1]) ones1 |= 1ull << k;
+        if (ei != own) use_m |= 1ull << k;
+        if (ei >= own) use_u |= 1ull << k;
+    }
+    const uint32_t tag_shift = T.P->tag_shift;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        s_tile1[k] = T.masks + ((uint64_t)EL.ei[k] * kMaxQV + q1) * T.W;
+        s_tag[k] = (uint32_t)EL.ei[k] << tag_shift;
+    }
+    __syncthreads();
+    uint32_t* incl_l = s_incl[wv];
     const uint64_t nw = (uint64_t)gridDim.x * kWpb;
-    for (uint64_t w = (uint64_t)blockIdx.x * kWpb + threadIdx.x / kWave; w < T.W; w += nw) {
-        const uint32_t v = (uint32_t)(w * 64 + lane);
-        const uint64_t word = all0 ? ~0ull : tile0[w];
-        const bool ok = v < T.dv && ((word >> lane) & 1ull);
-        if (!__ballot(ok)) continue;
-        List L{nullptr, nullptr, 0, 0};
-        uint32_t c = 0;
-        if (ok) {
-            L = Resolve(X, T, idx, v);
-            for (uint32_t t = 0; t < L.bl + L.al; t++) c += TileBit(X, T, q1, At(L, t)) ? 1u : 0u;
+    for (uint64_t w = (uint64_t)blockIdx.x * kWpb + wv; w < T.W; w += nw) {
+        const uint32_t v = (uint32_t)(w * kWave + lane);
+        uint64_t m0 = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint64_t word =
+                ((ones0 >> k) & 1ull) ? ~0ull : T.masks[((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w];
+            m0 |= ((word >> lane) & 1ull) << k;
         }
-        const uint32_t incl = WaveInclusiveScan(c);
-        const uint32_t total = __shfl(incl, 63, kWave);
-        if (total == 0) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)total);
-        base = __shfl(base, 0, kWave);
-        if (base + total > cap_rows) {
-            if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
-            continue;
+        if (v >= T.dv) m0 = 0;
+        if (!__ballot(m0 != 0)) continue;
+        const uint64_t r = (uint64_t)v * T.ndir + idx;
+        uint32_t lm = 0, lu = 0, ou = 0;
+        uint64_t om = 0;
+        if (m0 & use_m) {
+            const MRec mr = T.m_rec[r];
+            om = mr.off;
+            lm = mr.len;
         }
-        uint64_t pos = base + incl - c;
-        if (ok) {
-            for (uint32_t t = 0; t < L.bl + L.al; t++) {
-                const uint32_t nb = At(L, t);
-                if (TileBit(X, T, q1, nb)) {
-                    rows[2 * pos] = tag | v;
-                    rows[2 * pos + 1] = nb;
-                    pos++;
+        if (m0 & use_u) {
+            const URec ur = T.u_rec[r];
+            ou = ur.off;
+            lu = ur.len;
+        }
+        const uint32_t incl = WaveInclusiveScan(lm + lu);
+        const uint32_t total = Uniform(__shfl(incl, kWave - 1, kWave));
+        if (!total) continue;
+        incl_l[lane] = incl;
+        s_lenm[wv][lane] = lm;
+        s_offu[wv][lane] = ou;
+        s_m0[wv][lane] = m0;
+        s_offm[wv][lane] = om;
+        WaveSync();
+        for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+            const uint32_t j = c0 + lane;
+            uint64_t ok = 0;
+            uint32_t v0 = 0, nb = 0;
+            if (j < total) {
+                // owner lane: first lane whose inclusive count exceeds j
+                uint32_t lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                    const uint32_t m = (lo + hi) >> 1;
+                    if (incl_l[m] > j) hi = m;
+                    else lo = m + 1;
+                }
+                const uint32_t o = j - (lo ? incl_l[lo - 1] : 0u);
+                const uint32_t lmo = s_lenm[wv][lo];
+                uint64_t cand;
+                if (o < lmo) {
+                    nb = T.pool[s_offm[wv][lo] + o];
+                    cand = s_m0[wv][lo] & use_m;
+                } else {
+                    nb = T.u_vals[s_offu[wv][lo] + (o - lmo)];
+                    cand = s_m0[wv][lo] & use_u;
+                }
+                v0 = (uint32_t)(w * kWave + lo);
+                const uint64_t bit1 = 1ull << (nb & 63u);
+                for (uint64_t cm = cand; cm; cm &= cm - 1) {
+                    const uint32_t k = (uint32_t)__builtin_ctzll(cm);
+                    if (((ones1 >> k) & 1ull) || (s_tile1[k][nb >> 6] & bit1)) ok |= 1ull << k;
                 }
             }
+            const uint32_t nr = (uint32_t)__popcll(ok);
+            const uint32_t incl2 = WaveInclusiveScan(nr);
+            const uint32_t tot2 = Uniform(__shfl(incl2, kWave - 1, kWave));
+            if (!tot2) continue;
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)tot2);
+            base = Uniform64(__shfl(base, 0, kWave));
+            if (base + tot2 > cap_rows) {
+                if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
+                continue;
+            }
+            uint64_t pos = base + incl2 - nr;
+            for (; ok; ok &= ok - 1, pos++) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(ok);
+                rows[2 * pos] = s_tag[k] | v0;
+                rows[2 * pos + 1] = nb;
+            }
         }
+        WaveSync();
+    }
+}
+
+// Counting sort of depth-2 rows by edge tag (KSeed emits the tags of one
+// owner together; the final DFS runs faster over tag-grouped rows, whose
+// waves then share one tag's support masks in L2).  Order inside a tag is
+// arbitrary.  hist[0..kMaxQE) = rows per tag, hist[kMaxQE..) = cursors.
+__global__ void __launch_bounds__(kThreads) KTagHist(const uint32_t* __restrict__ in, uint64_t n, uint32_t tag_shift,
+                                                     uint64_t per_block, unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t h[kMaxQE];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * per_block, hi = lo + per_block < n ? lo + per_block : n;
+    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[2 * r] >> tag_shift], 1u);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x)
+        if (h[t]) atomicAdd(&hist[t], (unsigned long long)h[t]);
+}
+
+__global__ void __launch_bounds__(kWave) KTagScan(unsigned long long* __restrict__ hist) {
+    const uint32_t lane = LaneId();
+    const unsigned long long c = lane < (uint32_t)kMaxQE ? hist[lane] : 0ull;
+    const unsigned long long incl = WaveInclusiveScan(c);
+    if (lane < (uint32_t)kMaxQE) hist[kMaxQE + lane] = incl - c;
+}
+
+__global__ void __launch_bounds__(kThreads) KTagScatter(const uint32_t* __restrict__ in, uint64_t n, uint32_t tag_shift,
+                                                        uint64_t per_block, unsigned long long* __restrict__ hist,
+                                                        uint32_t* __restrict__ out) {
+    __shared__ uint32_t h[kMaxQE];
+    __shared__ unsigned long long base[kMaxQE];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * per_block, hi = lo + per_block < n ? lo + per_block : n;
+    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[2 * r] >> tag_shift], 1u);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) {
+        base[t] = h[t] ? atomicAdd(&hist[kMaxQE + t], (unsigned long long)h[t]) : 0ull;
+        h[t] = 0;
+    }
+    __syncthreads();
+    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+        const uint32_t w0 = in[2 * r], w1 = in[2 * r + 1];
+        const uint32_t k = w0 >> tag_shift;
+        const uint64_t pos = base[k] + atomicAdd(&h[k], 1u);
+        out[2 * pos] = w0;
+        out[2 * pos + 1] = w1;
     }
 }
 
@@ -1093,7 +1213,17 @@ unsigned Grid1(uint64_t n, unsigned per, unsigned cap) {
 
 void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s) {
     if (!L.n || !T.W) return;
-    KSeed<<<dim3(Grid1(T.W, kWpb, 4096), L.n), kThreads, 0, s>>>(T, L, rows, cap_rows, cnt);
+    KSeed<<<Grid1(T.W, kWpb, 8192), kThreads, 0, s>>>(T, L, rows, cap_rows, cnt);
+}
+
+void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t tag_shift, unsigned long long* hist, uint32_t* out,
+                      hipStream_t s) {
+    if (!n) return;
+    const uint64_t per_block = 4096;
+    const unsigned g = (unsigned)((n + per_block - 1) / per_block);
+    KTagHist<<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist);
+    KTagScan<<<1, kWave, 0, s>>>(hist);
+    KTagScatter<<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist, out);
 }
 
 void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out, uint64_t cap_rows,
@@ -1118,12 +1248,11 @@ uint32_t DfsBlocksPerCU() {
 void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>(cnt); }
 
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
-               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, hipStream_t s) {
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, uint32_t grab,
+               hipStream_t s) {
     if (!n_rows || !njobs) return;
     const unsigned g = Grid1(n_rows, kWpb, grid_blocks);
-    // Rows per cursor atomic (4 measured best on dense-6 streams: larger grabs
-    // leave a tail that stealing has to repair).
-    const uint32_t grab = 4;
+    grab = grab < 1 ? 1 : grab;
     if (tuples)
         KDfs<true><<<g, kThreads, 0, s>>>(jobs, njobs, row_base, grab, cnt, Q, tuples, tup_cap);
     else

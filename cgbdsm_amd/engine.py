@@ -137,6 +137,10 @@ class DeviceQuery:
     def build_local_index(self):
         _check(self.L.cgbdsm_build_local_index(self.h))
 
+    def build_frontier(self):
+        """Seeds and BFS levels of the next matching (optional; see cgbdsm_build_frontier)."""
+        _check(self.L.cgbdsm_build_frontier(self.h))
+
     def matching(self):
         n = C.c_ulonglong(0)
         per = np.zeros(N.MAX_QE, dtype=np.uint64)
@@ -164,6 +168,11 @@ class DeviceQuery:
     def set_dfs_options(self, shortest_list=True, product_suffix=True):
         """Count-mode refinements of the final DFS (both on by default)."""
         _check(self.L.cgbdsm_set_dfs_options(self.h, int(bool(shortest_list)), int(bool(product_suffix))))
+
+    def set_dfs_tuning(self, rows_per_grab=4, blocks_per_cu=0):
+        """Launch shape of the final DFS (counts are identical for every setting)."""
+        _check(self.L.cgbdsm_set_dfs_tuning(self.h, int(rows_per_grab), int(blocks_per_cu)))
+        self.dfs_tuning = (int(rows_per_grab), int(blocks_per_cu))
 
     def export(self, which):
         n = C.c_uint64()

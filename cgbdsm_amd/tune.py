@@ -1,0 +1,57 @@
+"""Autotuner of the final-DFS launch shape.
+
+The knobs are the frontier rows a wavefront takes per cursor atomic (the
+stealing chunk of the row phase) and the resident blocks per compute unit
+(wavefronts in flight, each with its LDS stack of up to 12 levels).  The
+tuner counts ONE batch once per candidate (cgbdsm_matching may count the
+same batch again: the index is only read), times each final-DFS launch with
+the HIP events the engine records on the query's stream, requires every
+candidate to produce identical per-query, per-edge counts, and leaves the
+fastest setting applied to the queries (event timing is switched off
+again on the leading query).  No reference counterpart: the
+reference's launch shape is fixed (utils/config.h GRID_DIM / BLOCK_DIM).
+"""
+import numpy as np
+
+GRABS = (1, 2, 4, 8, 16)
+BLOCKS_PER_CU = (0,)
+
+
+def _count(engine, queries):
+    if len(queries) == 1:
+        c, per = queries[0].matching()
+        return [(c, per)]
+    return engine.matching_batch(queries)
+
+
+def autotune(engine, queries, grabs=GRABS, blocks_per_cu=BLOCKS_PER_CU, repeats=1, log=None):
+    """`queries`: device queries after build_local_index on the same batch
+    (the first one leads a batched launch).  Returns (best, table, counts):
+    best = (rows_per_grab, blocks_per_cu), table = {setting: DFS ms},
+    counts = the batch's [(count, per_edge)] (identical for every setting)."""
+    lead = queries[0]
+    lead.profile(True)
+    table, ref = {}, None
+    for bpc in blocks_per_cu:
+        for g in grabs:
+            for q in queries:
+                q.set_dfs_tuning(g, bpc)
+            best_ms = None
+            for _ in range(max(1, repeats)):
+                lead.kernel_times(reset=True)
+                got = _count(engine, queries)
+                ms = lead.kernel_times()["dfs"]["ms"]
+                best_ms = ms if best_ms is None else min(best_ms, ms)
+                if ref is None:
+                    ref = got
+                elif any(a[0] != b[0] or not np.array_equal(a[1], b[1]) for a, b in zip(got, ref)):
+                    raise RuntimeError(f"DFS tuning {(g, bpc)} changed the counts")
+            table[(g, bpc)] = best_ms
+            if log:
+                log(f"autotune rows_per_grab={g} blocks_per_cu={bpc}: DFS {best_ms:.2f} ms")
+    best = min(table, key=table.get)
+    for q in queries:
+        q.set_dfs_tuning(*best)
+    lead.kernel_times(reset=True)
+    lead.profile(False)
+    return best, table, ref

@@ -189,11 +189,23 @@ cgbdsm_status cgbdsm_build_local_index(cgbdsm_query* q);
 /* Merged matching of all query edges: GPUGraphLoader::MatchingBitAll
  * (graph/graph_gpu.h:180, graph/graph_gpu.cu:1261-1480).  Adds the batch's
  * positive matches to *num_matches; per_edge[qe] (optional) receives the
- * count attributed to each query edge (the 5-bit edge tag). */
+ * count attributed to each query edge (the 5-bit edge tag).  May be called
+ * again before cgbdsm_batch_update_global_index: it counts the same batch
+ * again (the index is only read), e.g. to time other launch shapes. */
 cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge);
 
+/* Optional first half of cgbdsm_matching: the depth-2 seeds and the BFS
+ * levels (writeInitialResultsAllBit and the write_res levels of
+ * extendBFSAllBit, graph/graph_gpu.cu:1261-1340) on the query's stream,
+ * leaving the frontier for the final DFS of the next cgbdsm_matching or
+ * cgbdsm_matching_batch call.  Lets a serving loop build the frontiers of
+ * several queries concurrently (one host thread per query) before one
+ * batched DFS launch.  Returns with the query's stream idle. */
+cgbdsm_status cgbdsm_build_frontier(cgbdsm_query* q);
+
 /* cgbdsm_matching for nq registered queries of one context at once: each
- * query's seeds and BFS levels run on its own stream, then ONE persistent
+ * query's seeds and BFS levels run on its own stream (unless
+ * cgbdsm_build_frontier built them already), then ONE persistent
  * final-DFS launch counts all of them from a shared device work queue (the
  * first query's), so that the queries' uneven tails fill each other's idle
  * compute units.  Adds query i's matches to num_matches[i]; per_edge[i]
@@ -230,6 +242,14 @@ cgbdsm_status cgbdsm_set_match_params(cgbdsm_query* q, uint64_t bfs_row_limit,
  *   sizes, inclusion-exclusion within a label class) once per partial
  *   match of the positions before it. */
 cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int product_suffix);
+
+/* Launch shape of the final DFS (autotuner knobs; counts are identical for
+ * every setting).  rows_per_grab = frontier rows a wavefront takes per
+ * cursor atomic (0: default 4, at most 1024); blocks_per_cu = resident
+ * blocks per compute unit (0: as many as fit; larger values are clamped).
+ * For cgbdsm_matching_batch the first query's setting applies.  No
+ * reference counterpart (fixed GRID_DIM/BLOCK_DIM, utils/config.h). */
+cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu);
 
 /* Cooperative deadline for the final DFS of cgbdsm_matching (default 120 s,
  * 0 = none).

@@ -60,6 +60,8 @@ def test_matching_batch_parity(eng, donate_min):
             idx.rebuild_update(kb.upd)
             if nonempty:
                 q.build_local_index()
+                if (k + b) % 2 == 0:
+                    q.build_frontier()   # prebuilt frontier for half of them
                 live.append((k, kb))
         # vary the leader: rotate the order of the batch's queries
         live = live[b % max(1, len(live)):] + live[: b % max(1, len(live))]
@@ -126,5 +128,55 @@ def test_matching_batch_rejects(eng):
         eng.matching_batch([q0, q0])         # listed twice
     with pytest.raises(CgbdsmError):
         eng.matching_batch([])
+    for _, q, _, _ in sess:
+        q.close()
+
+
+def test_dfs_tuning_keeps_counts(eng):
+    """Every final-DFS launch shape the autotuner tries counts the same batch
+    identically (checked by the tuner itself) and equal to the oracle; a
+    second cgbdsm_matching on the same batch counts it again."""
+    from cgbdsm_amd.tune import autotune
+
+    print("case dfs tuning", flush=True)
+    sess = _open(eng, [("dense", 5, 2), ("sparse", 6, 4), ("tree", 5, 3)], scale=11, batch_edges=300)
+    tuned = 0
+    for b in range(2):
+        live = []
+        for k, (host, q, idx, plan) in enumerate(sess):
+            kb = host.next_batch()
+            q.stage(kb)
+            idx.set_dv(kb.dv)
+            q.transfer_update_all()
+            idx.insert(kb.ins)
+            idx.remove(kb.dele)
+            ne = q.transfer_update_index()
+            idx.rebuild_update(kb.upd)
+            if ne:
+                q.build_local_index()
+                live.append((k, kb))
+        if not live:
+            continue
+        qs = [sess[k][1] for k, _ in live]
+        # one query alone (b == 0), then all of them in one launch
+        sel = qs[:1] if b == 0 else qs
+        best, table, got = autotune(eng, sel, grabs=(1, 4, 16, 64), blocks_per_cu=(0, 1))
+        assert len(table) == 8 and best in table
+        tuned += 1
+        for (k, kb), (cnt, per) in zip(live, got):
+            host, q, idx, plan = sess[k]
+            ocnt, oper = idx.match(plan, kb.edge_ok, threads=4)
+            assert cnt == ocnt and np.array_equal(per, oper), (b, k, cnt, ocnt)
+        if b == 0:
+            c2, per2 = qs[0].matching()        # recount of the same batch
+            assert c2 == got[0][0] and np.array_equal(per2, got[0][1])
+            for q in qs[1:]:
+                q.matching()
+        for k, kb in live:
+            sess[k][1].batch_update_global_index()
+            sess[k][2].insert(kb.upd)
+    assert tuned > 0
+    with pytest.raises(CgbdsmError):
+        sess[0][1].set_dfs_tuning(2048, 0)
     for _, q, _, _ in sess:
         q.close()
