@@ -36,7 +36,8 @@ class Stats(C.Structure):
         "batches", "dv_count", "main_entries", "pool_used", "pool_cap", "last_ins", "last_del",
         "last_upd", "last_rows_touched", "last_idx_bytes", "last_frontier_rows",
         "last_frontier_depth", "last_matches", "compactions", "last_dfs_scanned", "last_dfs_probes",
-        "last_dfs_donated", "gpma_rebuilds", "gpma_windows", "gpma_top_level")] + [
+        "last_dfs_donated", "gpma_rebuilds", "gpma_windows", "gpma_top_level", "last_dfs_steps",
+        "last_dfs_step_lanes")] + [
         (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit",
                                   "last_dfs_busy")]
 

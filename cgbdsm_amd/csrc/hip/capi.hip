@@ -811,6 +811,14 @@ cgbdsm_status cgbdsm_query_set_plan(cgbdsm_query* q, const cgbdsm_plan* plan) {
             if (worst <= 3) break;
         }
         q->hplan.cp_start = cps;
+        q->hplan.sfx_batch = 0;
+        if (cps < qv) {
+            bool singles = true;
+            for (uint32_t j = cps; j < qv; j++)
+                for (uint32_t i = j + 1; i < qv; i++)
+                    if (plan->go_lcls[i] == plan->go_lcls[j]) singles = false;
+            q->hplan.sfx_batch = singles ? 1u : 0u;
+        }
         // The suffix position with the most backward neighbours (two or more)
         // keeps the intersection of its lists from all but the deepest one in
         // LDS; it only changes when one of those shallower positions does.
@@ -1247,7 +1255,7 @@ void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
 
 void ResetDfsCounters(cgbdsm_query* q) {
     for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
-                   &Counters::idle_ticks})
+                   &Counters::idle_ticks, &Counters::steps, &Counters::step_lanes})
         SetCounter(q, f, 0);
 }
 
@@ -1268,6 +1276,8 @@ void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* laun
     q->st.last_dfs_scanned = c->scanned;
     q->st.last_dfs_probes = c->probes;
     q->st.last_dfs_donated = c->donated;
+    q->st.last_dfs_steps = c->steps;
+    q->st.last_dfs_step_lanes = c->step_lanes;
     q->st.last_dfs_busy = c->wave_ticks ? 1.0 - (double)c->idle_ticks / (double)c->wave_ticks : 0.0;
     unsigned long long total = 0;
     for (uint32_t e = 0; e < q->qe; e++) {

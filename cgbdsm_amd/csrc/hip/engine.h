@@ -48,11 +48,12 @@ struct alignas(128) Counters {
     unsigned long long bytes;         // algorithmic bytes of the last index pass
     unsigned long long rows_touched;  // lists rewritten by the last index pass
     unsigned long long u_rows;        // touched rows of the current U
-    unsigned long long unused_q[3];
+    unsigned long long step_lanes;    // candidates covered by those steps (active lanes)
+    unsigned long long unused_q[2];
     unsigned long long per_edge[kMaxQE];
     unsigned long long card[kMaxDir], nodes[kMaxDir];
     unsigned long long tuples;
-    unsigned long long idle_unused;
+    unsigned long long steps;         // 64-lane candidate scan steps of the final DFS
     unsigned long long scanned;       // candidate list entries read by the final DFS
     unsigned long long probes;        // binary-search probes of the final DFS
     unsigned long long donated;       // partial matches handed to the stealing queue
@@ -106,6 +107,9 @@ struct DevPlan {
     uint32_t stab_pos;                 // suffix position whose stable lists are cached (0xFF: none)
     uint32_t stab_var_k;               // its backward neighbour at the deepest position
     uint32_t stab_deps;                // positions the cached intersection depends on
+    uint32_t sfx_batch;                // every label class of the product suffix has one member:
+                                       // the DFS counts the suffixes of a whole step of
+                                       // siblings at the position before it at once
     uint8_t maxbn[kMaxQV];             // largest backward-neighbour position of each position
     uint32_t dep_mask[kMaxQV];         // positions whose candidate set depends on position x
 };

@@ -595,6 +595,56 @@ __device__ __forceinline__ bool StabContains(const StabCache& sc, uint32_t c) {
     return LdsContains(sc.buf, sc.la, c) || LdsContains(sc.buf + sc.la, sc.len - sc.la, c);
 }
 
+// Fills the stable-intersection cache for position jf (its shallower
+// backward neighbours' lists intersected, support-mask bit applied): scans
+// the shortest of those lists and checks the others; state 2 when the
+// result does not fit the buffer.
+__device__ __forceinline__ void StabBuild(const Ctx& X, const Tables& T, uint32_t jf, const uint32_t* matched,
+                                          StabCache& sc, unsigned long long& scanned, unsigned long long& probes) {
+    const uint32_t lane = LaneId();
+    const uint32_t vk = sc.vk;
+    const uint32_t nbn = Nbn(X, jf);
+    // Build S from the shortest stable list, checking the others.
+    uint32_t best = 0xFFFFFFFFu, bk = 0;
+    for (uint32_t k = 0; k < nbn; k++) {
+        if (k == vk) continue;
+        const List L = BnList(X, T, jf, k, matched);
+        const uint32_t n = Uniform(L.bl + L.al);
+        if (n < best) {
+            best = n;
+            bk = k;
+        }
+    }
+    const List S0 = BnList(X, T, jf, bk, matched);
+    uint32_t len = 0, la = 0;
+    bool big = false;
+    for (uint32_t off = 0; off < best; off += kWave) {
+        const uint32_t i = off + lane;
+        bool ok = false;
+        uint32_t c = 0;
+        if (i < best) {
+            scanned++;
+            c = At(S0, i);
+            ok = TileBit(X, T, Vs(X, jf), c);
+            for (uint32_t k = 0; k < nbn && ok; k++)
+                if (k != vk && k != bk) ok = InList(BnList(X, T, jf, k, matched), c, probes);
+        }
+        const uint64_t m = __ballot(ok);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (len + cnt > kStabCap) {
+            big = true;
+            break;
+        }
+        if (ok) sc.buf[len + RankBelow(m)] = c;
+        len += cnt;
+        la += (uint32_t)__popcll(__ballot(ok && i < S0.bl));
+    }
+    WaveSync();
+    sc.len = len;
+    sc.la = la;
+    sc.state = big ? 2u : 1u;
+}
+
 // Size of C_B = the intersection of the candidate sets of the positions in
 // B (bit mask, all at or after j0): every backward-neighbour list of every
 // position of B, and the support-mask bit of each.  `raw` leaves the
@@ -614,48 +664,7 @@ __device__ __forceinline__ void SetSize(const Ctx& X, const Tables& T, uint32_t 
     }
     if (sc.pos < kMaxQV && B == (1u << sc.pos)) {
         const uint32_t vk = sc.vk;
-        const uint32_t nbn = Nbn(X, jf);
-        if (sc.state == 0) {
-            // Build S from the shortest stable list, checking the others.
-            uint32_t best = 0xFFFFFFFFu, bk = 0;
-            for (uint32_t k = 0; k < nbn; k++) {
-                if (k == vk) continue;
-                const List L = BnList(X, T, jf, k, matched);
-                const uint32_t n = Uniform(L.bl + L.al);
-                if (n < best) {
-                    best = n;
-                    bk = k;
-                }
-            }
-            const List S0 = BnList(X, T, jf, bk, matched);
-            uint32_t len = 0, la = 0;
-            bool big = false;
-            for (uint32_t off = 0; off < best; off += kWave) {
-                const uint32_t i = off + lane;
-                bool ok = false;
-                uint32_t c = 0;
-                if (i < best) {
-                    scanned++;
-                    c = At(S0, i);
-                    ok = TileBit(X, T, Vs(X, jf), c);
-                    for (uint32_t k = 0; k < nbn && ok; k++)
-                        if (k != vk && k != bk) ok = InList(BnList(X, T, jf, k, matched), c, probes);
-                }
-                const uint64_t m = __ballot(ok);
-                const uint32_t cnt = (uint32_t)__popcll(m);
-                if (len + cnt > kStabCap) {
-                    big = true;
-                    break;
-                }
-                if (ok) sc.buf[len + RankBelow(m)] = c;
-                len += cnt;
-                la += (uint32_t)__popcll(__ballot(ok && i < S0.bl));
-            }
-            WaveSync();
-            sc.len = len;
-            sc.la = la;
-            sc.state = big ? 2u : 1u;
-        }
+        if (sc.state == 0) StabBuild(X, T, jf, matched, sc, scanned, probes);
         if (sc.state == 1) {
             const List V = BnList(X, T, jf, vk, matched);
             const uint32_t nv = Uniform(V.bl + V.al);
@@ -808,6 +817,165 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
     return result;
 }
 
+// Per-wave LDS of BatchSuffix: per sibling lane, its varying list and the
+// running candidate count.
+struct BxLds {
+    const uint32_t* base[kWave];
+    const uint32_t* app[kWave];
+    uint32_t bl[kWave], al[kWave], incl[kWave], cnt[kWave];  // al bit 31: scan the source, probe the list
+};
+
+// Product-suffix counts of all survivors of one 64-candidate step at the
+// last position p before the suffix (lane s holds sibling x_s when bit s of
+// m is set), summed.  Every label class of the suffix has one member, so a
+// sibling's count is the product over suffix positions j of |C_j| - hits_j.
+// Positions whose backward neighbours lie before p have sibling-independent
+// sizes (cached per wave while those positions stay); the others are
+// counted for all siblings at once: their lists N_{p->j}(x_s) are laid end
+// to end over the 64 lanes, each entry tested against j's other lists (or
+// the cached stable intersection) and its support mask, and counted per
+// sibling in LDS.  Equal to SuffixCount per sibling (identical counts).
+__device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_t p, const uint32_t* matched,
+                                          uint64_t m, uint32_t x, uint32_t* csize, uint32_t& cvalid, StabCache& sc,
+                                          BxLds& B, unsigned long long& scanned, unsigned long long& probes) {
+    const uint32_t lane = LaneId();
+    const uint32_t qv = X.qv;
+    const uint32_t pcls = Cls(X, p);
+    unsigned long long prod = ((m >> lane) & 1ull) ? 1ull : 0ull;
+    for (uint32_t j = p + 1; j < qv; j++) {
+        if (!__ballot(prod != 0)) break;
+        const uint32_t nbn = Nbn(X, j);
+        uint32_t kp = kNoSkip;
+        for (uint32_t k = 0; k < nbn; k++)
+            if ((Pair(X, j, k) >> 8) == p) kp = k;
+        const uint32_t cls = Cls(X, j), qj = Vs(X, j);
+        const bool same = pcls == cls;  // x_s itself may fall in C_j
+        if (kp == kNoSkip) {
+            const bool cached = (cvalid >> j) & 1u;
+            uint32_t raw = cached ? Uniform(csize[j]) : 0u, hits = 0;
+            SetSize(X, T, 1u << j, p, matched, !cached, raw, hits, sc, scanned, probes);
+            if (!cached) {
+                if (lane == 0) csize[j] = raw;
+                cvalid |= 1u << j;
+            }
+            long long val = (long long)raw - (long long)hits;
+            if (same && prod && val > 0) {
+                bool in = TileBit(X, T, qj, x);
+                for (uint32_t k = 0; k < nbn && in; k++) in = InList(BnList(X, T, j, k, matched), x, probes);
+                val -= in ? 1 : 0;
+            }
+            prod = val > 0 ? prod * (unsigned long long)val : 0ull;
+            continue;
+        }
+        // j's list from p, per sibling
+        const uint32_t dir = Pair(X, j, kp) & 0xFFu;
+        List Ls{nullptr, nullptr, 0, 0};
+        if (prod) Ls = Resolve(X, T, dir, x);
+        const uint32_t len = Ls.bl + Ls.al;
+        const bool ones = (X.ones >> qj) & 1u;
+        uint32_t raw;
+        if (nbn == 1 && ones) {
+            raw = len;
+        } else {
+            bool use_sc = false;
+            if (nbn > 1 && sc.pos == j && sc.vk == kp) {
+                if (sc.state == 0) StabBuild(X, T, j, matched, sc, scanned, probes);
+                use_sc = sc.state == 1;
+            }
+            // Sibling-independent source: the cached intersection S, or j's
+            // shortest list from a position before p.
+            uint32_t kmin = kNoSkip, nmin = 0xFFFFFFFFu;
+            if (use_sc) {
+                nmin = sc.len;
+            } else {
+                for (uint32_t k = 0; k < nbn; k++) {
+                    if (k == kp) continue;
+                    const List L = BnList(X, T, j, k, matched);
+                    const uint32_t n = Uniform(L.bl + L.al);
+                    if (n < nmin) {
+                        nmin = n;
+                        kmin = k;
+                    }
+                }
+            }
+            const List Lmin = kmin != kNoSkip ? BnList(X, T, j, kmin, matched) : List{nullptr, nullptr, 0, 0};
+            // Per sibling, scan the shorter side: the source (probing the
+            // sibling's list by binary search) or the sibling's list.
+            bool from_src = false;
+            if (prod && nmin != 0xFFFFFFFFu) {
+                const uint32_t depth_v = 32u - (uint32_t)__builtin_clz(len | 1u);
+                from_src = use_sc ? (uint64_t)nmin * (depth_v + 1) < len : nmin < len;
+            }
+            const uint32_t ne = prod ? (from_src ? nmin : len) : 0u;
+            const uint32_t incl = WaveInclusiveScan(ne);
+            const uint32_t total = Uniform(__shfl(incl, kWave - 1, kWave));
+            B.base[lane] = Ls.base;
+            B.app[lane] = Ls.app;
+            B.bl[lane] = Ls.bl;
+            B.al[lane] = from_src ? (Ls.al | 0x80000000u) : Ls.al;
+            B.incl[lane] = incl;
+            B.cnt[lane] = 0;
+            WaveSync();
+            for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+                const uint32_t e = c0 + lane;
+                if (e < total) {
+                    uint32_t lo = 0, hi = kWave - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (B.incl[mid] > e) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const uint32_t o = e - (lo ? B.incl[lo - 1] : 0u);
+                    const uint32_t alw = B.al[lo];
+                    scanned++;
+                    bool ok;
+                    if (alw & 0x80000000u) {
+                        const uint32_t c = use_sc ? sc.buf[o] : At(Lmin, o);
+                        const List Lo{B.base[lo], B.app[lo], B.bl[lo], alw & 0x7FFFFFFFu};
+                        ok = InList(Lo, c, probes);
+                        if (ok && !use_sc) {
+                            ok = TileBit(X, T, qj, c);
+                            for (uint32_t k = 0; k < nbn && ok; k++)
+                                if (k != kp && k != kmin) ok = InList(BnList(X, T, j, k, matched), c, probes);
+                        }
+                    } else {
+                        const uint32_t c = o < B.bl[lo] ? B.base[lo][o] : B.app[lo][o - B.bl[lo]];
+                        if (use_sc) {
+                            ok = StabContains(sc, c);
+                        } else {
+                            ok = TileBit(X, T, qj, c);
+                            for (uint32_t k = 0; k < nbn && ok; k++)
+                                if (k != kp) ok = InList(BnList(X, T, j, k, matched), c, probes);
+                        }
+                    }
+                    if (ok) atomicAdd(&B.cnt[lo], 1u);
+                }
+            }
+            WaveSync();
+            raw = B.cnt[lane];
+        }
+        // Positions before the suffix with j's label class that lie in C_j.
+        uint32_t hits = 0;
+        if (prod && raw) {
+            auto member = [&](uint32_t y) -> bool {
+                if (!TileBit(X, T, qj, y)) return false;
+                if (!InList(Ls, y, probes)) return false;
+                for (uint32_t k = 0; k < nbn; k++)
+                    if (k != kp && !InList(BnList(X, T, j, k, matched), y, probes)) return false;
+                return true;
+            };
+            for (uint32_t i = 0; i < p; i++)
+                if (Cls(X, i) == cls && member(matched[i])) hits++;
+            if (same && member(x)) hits++;
+        }
+        const long long val = (long long)raw - (long long)hits;
+        prod = val > 0 ? prod * (unsigned long long)val : 0ull;
+    }
+    unsigned long long sum = prod;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+    return Uniform64(sum);
+}
+
 // Final DFS.  Work items are frontier rows (taken `grab` at a time from a
 // global cursor) and, once those run out, partial matches stolen from the
 // global queue.  A wavefront that is descending at a shallow level while
@@ -837,6 +1005,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     __shared__ PlanLds s_plan[kWpb];
     __shared__ uint32_t s_csize[kWpb][kMaxQV];
     __shared__ uint32_t s_stab[kWpb][kStabCap];
+    __shared__ BxLds s_bx[kWpb];
     const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
     uint32_t* matched = s_matched[wv];
     DfsLevel* lvl = s_lvl[wv];
@@ -850,7 +1019,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     Counters* jc = nullptr;
     Ctx X = MakeCtx(T, &s_plan[wv]);
     StabCache sc{s_stab[wv], 0, 0, 0, 0xFFu, 0, 0};
-    bool sets = false, cart = false;
+    bool sets = false, cart = false, bsfx = false;
     auto flush_pe = [&]() {
         if (!jc) return;
         for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave)
@@ -881,9 +1050,10 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         sets = (opts & kJobShortest) && lists_are_sets;
         cps = T.P->cp_start;
         cart = !kTuples && lists_are_sets && (opts & kJobSuffix) && cps < qv;
+        bsfx = cart && T.P->sfx_batch;
     };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
-    unsigned long long my_scanned = 0, my_probes = 0, donated = 0;
+    unsigned long long my_scanned = 0, my_probes = 0, donated = 0, steps = 0, step_lanes = 0;
     uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
     bool primary = true, idle = false, stop = false, have_ticket = false, from_row = false;
     uint32_t iter = 0, backoff = 1;
@@ -1008,7 +1178,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         unsigned long long count = 0;
         uint32_t d = D0;
         for (;;) {
-            if ((++iter & 31u) == 0 && poll()) {
+            // (every iteration at the batched level, whose steps are long:
+            // donors must see idle wavefronts early)
+            if (((++iter & 31u) == 0 || (bsfx && d + 1 == cps)) && poll()) {
                 stop = true;
                 break;
             }
@@ -1095,7 +1267,13 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 continue;
             }
             // Split the rest of this level's candidate range with idle wavefronts.
-            if (Q.donate_min && idl_hint > 0 && limit - off >= 2 * kRangeMin) {
+            // At the position before a batched product suffix every candidate
+            // carries a whole suffix count and its survivors are not descended
+            // into, so its ranges are split down to one step each (smaller
+            // pieces would repeat the per-item set-up of the suffix sizes).
+            const bool blev = bsfx && d + 1 == cps;
+            const uint32_t rmin = blev ? kWave : kRangeMin;
+            if (Q.donate_min && idl_hint > 0 && limit - off >= 2 * rmin) {
                 unsigned long long h = 0, t = 0;
                 if (lane == 0) {
                     h = LoadAgent(&cnt->q_head.v);
@@ -1105,7 +1283,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 t = __shfl(t, 0, kWave);
                 const uint32_t rem = limit - off;
                 uint32_t parts = (uint32_t)(idl_hint < 15 ? idl_hint + 1 : 16);
-                if (parts > rem / kRangeMin) parts = rem / kRangeMin;
+                if (parts > rem / rmin) parts = rem / rmin;
                 if (parts >= 2 && (long long)(t - h) + (long long)parts <= (long long)(Q.cap / 2)) {
                     const uint32_t piece = ((rem + parts - 1) / parts + kWave - 1) & ~(uint32_t)(kWave - 1);
                     const uint32_t k = (rem - 1) / piece;  // donated pieces after the kept one
@@ -1136,12 +1314,23 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             const uint32_t i = off + lane;
             uint32_t c = 0;
             bool ok = false;
+            steps++;
+            step_lanes += limit - off < kWave ? limit - off : kWave;
             if (i < limit) {
                 c = At(L, i);
                 my_scanned++;
                 ok = AcceptSel(X, T, d, matched, c, sel, d, my_probes);
             }
             const uint64_t m = __ballot(ok);
+            // Last position before the product suffix: count the suffix of
+            // every survivor of this step at once instead of descending.
+            if (blev) {
+                if (m) count += BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv], my_scanned,
+                                            my_probes);
+                if (lane == 0) lvl[d].off = off + kWave;
+                WaveSync();
+                continue;
+            }
             if (d == last) {
                 count += (unsigned long long)__popcll(m);
                 if (kTuples && ok) {
@@ -1188,6 +1377,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     if (lane == 0) {
         atomicAdd(&cnt->scanned, s_tot);
         atomicAdd(&cnt->probes, p_tot);
+        atomicAdd(&cnt->steps, steps);
+        atomicAdd(&cnt->step_lanes, step_lanes);
         if (donated) atomicAdd(&cnt->donated, donated);
     }
     flush_pe();

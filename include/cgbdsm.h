@@ -102,6 +102,10 @@ typedef struct {
     uint64_t gpma_rebuilds;       /* GPMA layout: growths by full rebuild         */
     uint64_t gpma_windows;        /* GPMA layout: windows rebalanced              */
     uint64_t gpma_top_level;      /* GPMA layout: highest window level + 1        */
+    uint64_t last_dfs_steps;      /* 64-lane candidate steps of the final DFS's
+                                     descent loop (suffix sizing excluded)        */
+    uint64_t last_dfs_step_lanes; /* candidates those steps covered: lane
+                                     utilisation = step_lanes / (64 * steps)      */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
     double   last_dfs_busy;       /* share of final-DFS wave time spent working   */
 } cgbdsm_stats;
