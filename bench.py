@@ -259,6 +259,97 @@ def multi_query(args, threads, log):
     return out
 
 
+def serve_bench(args, threads, log):
+    """Serving loop (cgbdsm_amd.serve) on one update stream: batch sizes drawn
+    lognormal (median --serve-median edges, sigma 1, capped at 2 M), the
+    8-query mix, half registered at the start and the rest arriving during
+    the run.  Open loop: batches arrive as a Poisson process at --serve-rate
+    batches/s, registrations at a third of that rate; closed loop: every
+    batch is waiting at the start (capacity).  Each run in both modes: served
+    (concurrent index maintenance + one batched DFS launch) and plain (the
+    live queries' Stage 2 one after another); every count must agree."""
+    import random
+
+    import numpy as np
+
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import HostSession, synth_spec
+    from cgbdsm_amd.serve import Registration, Server
+
+    rng = random.Random(7)
+    n = min(args.multi_queries or 8, len(MQ_MIX))
+    sessions = []
+    for kind, size, seed in MQ_MIX[:n]:
+        spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                          initial_frac=args.initial_frac, stream_edges=args.stream_edges, query_size=size,
+                          query_kind=kind, query_seed=seed)
+        sessions.append((f"{size}-{kind}-s{seed}", HostSession(spec=spec, threads=threads)))
+    left = sessions[0][1].stream_edges_left()
+    sizes = []
+    while left > 0 and len(sizes) < args.serve_batches:
+        k = min(left, max(1, min(2_000_000, int(round(args.serve_median * float(np.exp(rng.gauss(0.0, 1.0))))))))
+        sizes.append(k)
+        left -= k
+    regs = []
+    for name, h in sessions:
+        keys, dv, card, avg = h.initial_keys()
+        plan = h.make_plan(card, avg)
+        batches = [h.next_batch(k) for k in sizes]
+        regs.append(Registration(name, h.query_tables, plan, keys, dv, batches))
+    pairs = [int(b.pairs) for b in regs[0].batches]
+    log(f"serve: {len(sizes)} batches (median {int(np.median(pairs))} edges, max {max(pairs)}), {n} queries")
+    t, arr = 0.0, []
+    for p in pairs:
+        arr.append((t, p))
+        t += rng.expovariate(args.serve_rate)
+    t, reg_t = 0.0, []
+    for i in range(n):
+        if i < n // 2:
+            reg_t.append(0.0)
+        else:
+            t += rng.expovariate(args.serve_rate / 3.0)
+            reg_t.append(t)
+    out = {"metric": "serving loop: sustained update edges/s and p50/p99 per-batch latency, served vs plain "
+                     "(exact counts)",
+           "batches": len(sizes), "batch_edges": pairs, "queries": [r.name for r in regs],
+           "registration_s": [round(x, 4) for x in reg_t], "rate_batches_per_s": args.serve_rate}
+    counts, tuning = {}, None
+    for loop in ("open", "closed"):
+        for mode in ("plain", "served"):
+            mix = [Registration(r.name, r.tables, r.plan, r.keys, r.dv, r.batches, arrive_s=(a if loop == "open"
+                                                                                              else 0.0))
+                   for r, a in zip(regs, reg_t)]
+            arrivals = arr if loop == "open" else [(0.0, p) for p in pairs]
+            eng = Engine(0)
+            # the launch shape is tuned once (first served batch of the open
+            # loop) and kept, as a deployment would
+            srv = Server(eng, mode=mode, threads=min(16, threads), deadline_ms=args.deadline_ms, tuning=tuning)
+            res = srv.run(mix, arrivals)
+            if mode == "served":
+                tuning = res["dfs_tuning"]
+            srv.close()
+            eng.close()
+            key = f"{loop}_{mode}"
+            counts[key] = res["counts"]
+            out[key] = {"update_edges_per_s": round(res["update_edges_per_s"], 3),
+                        "p50_batch_ms": round(res["p50_ms"], 3), "p99_batch_ms": round(res["p99_ms"], 3),
+                        "joined_batch": {k: v.get("joined_batch") for k, v in res["admissions"].items()}}
+            log(f"serve {key}: {res['update_edges_per_s']:.0f} update edges/s, p50 {res['p50_ms']:.1f} ms, "
+                f"p99 {res['p99_ms']:.1f} ms")
+    # a query's count for a batch is the same in every run that served it
+    agree = True
+    for name in out["queries"]:
+        merged = {}
+        for c in counts.values():
+            for b, v in c[name].items():
+                if merged.setdefault(b, v) != v:
+                    agree = False
+    out["counts_agree"] = agree
+    out["closed_served_vs_plain"] = round(out["closed_served"]["update_edges_per_s"] /
+                                          out["closed_plain"]["update_edges_per_s"], 4)
+    return out
+
+
 def index_stress(args, threads, log):
     """Index maintenance alone (apply, update index, commit; no matching),
     GAMMA lists vs the GPMA array on the same stream; the final main index
@@ -356,6 +447,11 @@ def main():
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
     ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
                     help="main-index layout (both give identical lists; GPMA maintains them faster)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving loop, open and closed loop, served vs plain (prints its own JSON line)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="batch arrivals per second (open loop)")
+    ap.add_argument("--serve-median", type=int, default=50_000, help="median batch size in edges")
+    ap.add_argument("--serve-batches", type=int, default=24, help="batches drawn from the stream at most")
     ap.add_argument("--index-stress", action="store_true",
                     help="index maintenance only, GAMMA vs GPMA (prints its own JSON line)")
     ap.add_argument("--multi-queries", type=int, default=8,
@@ -375,6 +471,9 @@ def main():
     threads = min(16, os.cpu_count() or 1)
     if args.index_stress:
         print(json.dumps(index_stress(args, threads, log)), flush=True)
+        return
+    if args.serve:
+        print(json.dumps(serve_bench(args, threads, log)), flush=True)
         return
     if args.multi_only:
         out = multi_query(args, threads, log)

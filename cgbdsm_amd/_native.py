@@ -106,6 +106,9 @@ def host_lib():
         L.cgh_num_threads.argtypes = [P]
         L.cgh_num_threads.restype = U32
         L.cgh_next_batch.argtypes = [P, C.POINTER(Batch)]
+        L.cgh_next_batch_edges.argtypes = [P, U64, C.POINTER(Batch)]
+        L.cgh_stream_left.argtypes = [P]
+        L.cgh_stream_left.restype = U64
         L.cgh_old_ids.argtypes = [P, C.POINTER(C.POINTER(U32)), C.POINTER(U32)]
         L.cgh_export_graph.argtypes = [P, C.c_void_p, C.c_void_p, U64, C.c_void_p, C.POINTER(U32),
                                        C.POINTER(U64)]
@@ -127,6 +130,7 @@ def dev_lib():
         L.cgbdsm_version.restype = C.c_char_p
         L.cgbdsm_ctx_create.argtypes = [I, U64, C.POINTER(P)]
         L.cgbdsm_ctx_destroy.argtypes = [P]
+        L.cgbdsm_ctx_memory.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
         L.cgbdsm_query_create.argtypes = [P, C.POINTER(Plan), C.POINTER(P)]
         L.cgbdsm_query_destroy.argtypes = [P]
         L.cgbdsm_query_set_plan.argtypes = [P, C.POINTER(Plan)]
@@ -159,7 +163,7 @@ def dev_lib():
 
 
 DEV_SYMBOLS = [
-    "cgbdsm_last_error", "cgbdsm_version", "cgbdsm_ctx_create", "cgbdsm_ctx_destroy",
+    "cgbdsm_last_error", "cgbdsm_version", "cgbdsm_ctx_create", "cgbdsm_ctx_destroy", "cgbdsm_ctx_memory",
     "cgbdsm_query_create", "cgbdsm_query_destroy", "cgbdsm_query_set_plan", "cgbdsm_index_init",
     "cgbdsm_batch_stage", "cgbdsm_transfer_update_all", "cgbdsm_transfer_update_index",
     "cgbdsm_build_local_index", "cgbdsm_matching", "cgbdsm_matching_batch", "cgbdsm_batch_update_global_index",

@@ -685,6 +685,14 @@ cgbdsm_status cgbdsm_ctx_destroy(cgbdsm_ctx* ctx) {
     return Guard([&] { delete ctx; });
 }
 
+cgbdsm_status cgbdsm_ctx_memory(cgbdsm_ctx* ctx, uint64_t* used_bytes, uint64_t* budget_bytes) {
+    return Guard([&] {
+        if (!ctx) Throw(CGBDSM_ERR_INVALID, "null context");
+        if (used_bytes) *used_bytes = ctx->used.load();
+        if (budget_bytes) *budget_bytes = ctx->budget;
+    });
+}
+
 cgbdsm_status cgbdsm_query_create(cgbdsm_ctx* ctx, const cgbdsm_plan* plan, cgbdsm_query** out) {
     return Guard([&] {
         if (!ctx || !plan || !out) Throw(CGBDSM_ERR_INVALID, "null argument");

@@ -63,8 +63,11 @@ uint64_t Session::BatchPairs(uint32_t b) const {
 }
 
 const KeyBatch& Session::RunBatch(uint32_t b, S1Timing* t) {
-    uint32_t start = b * batch_ints_;
-    uint32_t count = std::min(batch_ints_, calig_->UpdateSize() - start);
+    const uint32_t start = b * batch_ints_;
+    return RunRange(start, std::min(batch_ints_, calig_->UpdateSize() - start), t);
+}
+
+const KeyBatch& Session::RunRange(uint32_t start, uint32_t count, S1Timing* t) {
     auto t0 = Clock::now();
     calig_->UpdateIndex(start, count);
     auto t1 = Clock::now();
@@ -96,6 +99,7 @@ const KeyBatch& Session::RunBatch(uint32_t b, S1Timing* t) {
 struct cgh_session {
     std::unique_ptr<cg::Session> s;
     uint32_t next_batch = 0;
+    uint64_t cursor = 0;  // stream ints consumed by Stage 1 so far
     bool have_keys = false;
 };
 
@@ -210,6 +214,7 @@ int cgh_next_batch(cgh_session* s, cgh_batch* out) {
         uint32_t b = s->next_batch++;
         cg::S1Timing t;
         const cg::KeyBatch& kb = s->s->RunBatch(b, &t);
+        s->cursor = std::min<uint64_t>((uint64_t)(b + 1) * s->s->batch_ints(), s->s->num_updates());
         out->ins = kb.ins.data();
         out->n_ins = kb.ins.size();
         out->del = kb.del.data();
@@ -220,6 +225,37 @@ int cgh_next_batch(cgh_session* s, cgh_batch* out) {
         out->dv = kb.dv;
         out->batch = b;
         out->pairs = s->s->BatchPairs(b);
+        out->ms_update_index = t.update_index_ms;
+        out->ms_construct_update = t.construct_update_ms;
+        out->ms_prepare_all = t.prepare_all_ms;
+        out->ms_prepare_index = t.prepare_index_ms;
+    })
+}
+
+uint64_t cgh_stream_left(cgh_session* s) { return s->s->num_updates() - std::min<uint64_t>(s->cursor, s->s->num_updates()); }
+
+int cgh_next_batch_edges(cgh_session* s, uint64_t n_edges, cgh_batch* out) {
+    CGH_TRY({
+        if (!s->have_keys) throw cg::Error("cgh_initial_keys must run before the first batch");
+        const uint64_t total = s->s->num_updates();
+        if (s->cursor >= total) throw cg::Error("no stream edges left");
+        if (n_edges == 0) throw cg::Error("empty batch");
+        const uint64_t start = s->cursor;
+        const uint64_t count = std::min<uint64_t>(2 * n_edges, total - start);
+        cg::S1Timing t;
+        const cg::KeyBatch& kb = s->s->RunRange((uint32_t)start, (uint32_t)count, &t);
+        s->cursor = start + count;
+        const uint32_t b = s->next_batch++;
+        out->ins = kb.ins.data();
+        out->n_ins = kb.ins.size();
+        out->del = kb.del.data();
+        out->n_del = kb.del.size();
+        out->upd = kb.upd.data();
+        out->n_upd = kb.upd.size();
+        out->edge_ok = kb.edge_ok.data();
+        out->dv = kb.dv;
+        out->batch = b;
+        out->pairs = count / 2;
         out->ms_update_index = t.update_index_ms;
         out->ms_construct_update = t.construct_update_ms;
         out->ms_prepare_all = t.prepare_all_ms;

@@ -31,6 +31,9 @@ public:
     // Stage 1 of batch `b` (main_calig.cpp:220-262): updateIndex,
     // ConstructUpdate, mergeDeltas, PrepareUpdateAll, PrepareUpdateIndex.
     const KeyBatch& RunBatch(uint32_t b, S1Timing* t);
+    // Stage 1 over stream ints [start, start + count) (batches of any size;
+    // callers keep them contiguous and in order).
+    const KeyBatch& RunRange(uint32_t start, uint32_t count, S1Timing* t);
     // The same pieces one at a time, for callers that time them separately.
     uint64_t BatchPairs(uint32_t b) const;
 

@@ -50,6 +50,12 @@ class Engine:
     def query(self, tables):
         return DeviceQuery(self, tables)
 
+    def memory(self):
+        """(device bytes held by this engine's queries, budget or 0)."""
+        used, budget = C.c_uint64(), C.c_uint64()
+        _check(self.L.cgbdsm_ctx_memory(self.h, C.byref(used), C.byref(budget)))
+        return int(used.value), int(budget.value)
+
     def matching_batch(self, queries):
         """Matching of several queries of this engine (each after
         build_local_index) with one final-DFS launch; returns a list of

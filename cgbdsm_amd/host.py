@@ -113,10 +113,18 @@ class HostSession:
         _check(L.cgh_make_plan(self._h, c, a, C.byref(p)), L)
         return p
 
-    def next_batch(self):
+    def stream_edges_left(self):
+        return int(self._L.cgh_stream_left(self._h)) // 2
+
+    def next_batch(self, n_edges=None):
+        """Stage 1 of the next batch: the session's fixed batch size, or the
+        next `n_edges` stream edges (batches of any size)."""
         L = self._L
         b = N.Batch()
-        _check(L.cgh_next_batch(self._h, C.byref(b)), L)
+        if n_edges is None:
+            _check(L.cgh_next_batch(self._h, C.byref(b)), L)
+        else:
+            _check(L.cgh_next_batch_edges(self._h, int(n_edges), C.byref(b)), L)
         return KeyBatch(b.batch, _arr(b.ins, b.n_ins, np.uint64), _arr(b.dele, b.n_del, np.uint64),
                         _arr(b.upd, b.n_upd, np.uint64), _arr(b.edge_ok, self.qe, np.uint8), b.dv, b.pairs,
                         {"updateIndex": b.ms_update_index, "ConstructUpdate": b.ms_construct_update,

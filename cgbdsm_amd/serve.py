@@ -1,0 +1,288 @@
+"""Continuous-batching serving loop over one device engine.
+
+Requests arrive open-loop: update batches of one shared edge stream, and
+query registrations (with an optional departure batch).  The loop keeps
+every admitted query's index (main index, GPMA array, update lists) resident
+in device memory and, per update batch:
+
+  * index maintenance, masks and frontiers of all live queries run
+    concurrently (one host thread and one HIP stream per query);
+  * ONE persistent final-DFS launch counts every live query from a shared
+    device work queue (cgbdsm_matching_batch), launch shape autotuned on the
+    first batch;
+  * the commits run concurrently.
+
+A registration is admitted when a live slot is free and the engine's device
+memory budget has room for its footprint next to the footprints of the
+queries already admitted (and next to the bytes cgbdsm_ctx_memory reports
+in use; a load that still exceeds the budget fails with CGBDSM_ERR_OOM and
+is retried): otherwise it waits until a query leaves.  An admitted query is brought up to the stream position on
+its own thread and stream while the live queries keep being served: its
+index is loaded and the batches it missed are applied (index maintenance
+only, no matching); it joins the live set at the next batch boundary it
+has reached.  A query's count for a batch depends only on its index before
+and after that batch, so the counts of a query served in the mix equal the
+counts of the same query run alone (tests/test_gpu_serve.py checks this).
+
+The reference has no serving loop (one query per process, batches read from
+a file, main_calig.cpp:178-338); Stage 1 (CaLiG) stays per query on the host
+and is supplied as per-batch key sets, as the reference's S1 thread
+supplies them to its S2 loop.
+"""
+import statistics
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+from .engine import CgbdsmError
+
+OOM = 2
+
+
+@dataclass
+class Registration:
+    """One query and its Stage-1 output: initial index keys and one key batch
+    per stream batch (KeyBatch list, index = stream batch number)."""
+    name: str
+    tables: object
+    plan: object
+    keys: object
+    dv: int
+    batches: list
+    arrive_s: float = 0.0            # arrival time, seconds after the loop starts
+    leave_after: int = None          # last stream batch served (None: to the end)
+    footprint: int = None            # device bytes to reserve at admission (None: estimated)
+
+    def need(self):
+        # index keys as lists, records, masks and frontier buffers: a coarse
+        # bound when the caller has no measured footprint
+        return self.footprint if self.footprint is not None else 64 * len(self.keys) + (32 << 20)
+
+
+@dataclass
+class _Query:
+    reg: Registration
+    dq: object = None
+    applied: int = 0                 # stream batches applied to its index
+    joined_at: int = None            # first batch counted in the live set
+    ready: bool = False              # caught up, waiting to join
+    failed: object = None
+    cond: threading.Condition = field(default_factory=threading.Condition)
+
+
+class Server:
+    def __init__(self, engine, *, mode="served", layout="gpma", deadline_ms=120000.0, autotune=True, max_live=128,
+                 threads=16, tuning=None, log=None):
+        """tuning: a DFS launch shape (rows_per_grab, blocks_per_cu) found
+        earlier; otherwise (autotune) it is tuned on the first batch served."""
+        if mode not in ("served", "plain"):
+            raise ValueError("mode is 'served' or 'plain'")
+        self.eng, self.mode, self.layout = engine, mode, layout
+        self.deadline_ms, self.autotune, self.max_live = deadline_ms, autotune, max_live
+        self.pool = ThreadPoolExecutor(max_workers=threads)
+        self.log = log or (lambda m: None)
+        self.lock = threading.Lock()
+        self.b_next = 0              # next stream batch the live set will count
+        self.tuning = tuple(tuning) if tuning else None
+
+    # ------------------------------------------------------------ catch-up
+    def _open(self, q):
+        r = q.reg
+        dq = self.eng.query(r.tables)
+        try:
+            dq.set_layout(self.layout)
+            dq.index_init(r.keys, r.dv)
+            dq.set_plan(r.plan)
+            dq.set_deadline(self.deadline_ms)
+            if self.tuning:
+                dq.set_dfs_tuning(*self.tuning)
+        except Exception:
+            dq.close()
+            raise
+        q.dq = dq
+
+    def _apply(self, q, b):
+        """Index maintenance of stream batch b without matching."""
+        dq = q.dq
+        dq.stage(q.reg.batches[b])
+        dq.transfer_update_all()
+        dq.transfer_update_index()
+        dq.batch_update_global_index()
+        dq.sync()
+
+    def _catch_up(self, q):
+        try:
+            self._open(q)
+            while True:
+                with self.lock:
+                    target = self.b_next
+                if q.applied < target:
+                    self._apply(q, q.applied)
+                    q.applied += 1
+                    continue
+                with q.cond:
+                    q.ready = True
+                    # wait until joined, or until the live set moved on
+                    q.cond.wait_for(lambda: q.joined_at is not None or self.b_next > q.applied, timeout=1.0)
+                    if q.joined_at is not None:
+                        return
+                    q.ready = False
+        except Exception as e:  # admission failure (OOM) or device error
+            q.failed = e
+            if q.dq is not None:
+                q.dq.close()
+                q.dq = None
+            with q.cond:
+                q.ready = True
+                q.cond.notify_all()
+
+    # ------------------------------------------------------------ one batch
+    def _prepare(self, q, b):
+        dq = q.dq
+        dq.stage(q.reg.batches[b])
+        dq.transfer_update_all()
+        if not dq.transfer_update_index():
+            return False
+        dq.build_local_index()
+        dq.build_frontier()
+        return True
+
+    def _plain_one(self, q, b):
+        dq = q.dq
+        dq.stage(q.reg.batches[b])
+        c, _, _ = dq.run_batch()
+        return c
+
+    def _serve_batch(self, live, b):
+        out = {}
+        if self.mode == "plain":
+            for q in live:
+                out[q.reg.name] = self._plain_one(q, b)
+            return out
+        ok = list(self.pool.map(lambda q: self._prepare(q, b), live))
+        ks = [q for q, f in zip(live, ok) if f]
+        for q, f in zip(live, ok):
+            if not f:
+                out[q.reg.name] = 0
+        if ks:
+            qs = [q.dq for q in ks]
+            if self.autotune and self.tuning is None:
+                from .tune import autotune
+                best, _, _ = autotune(self.eng, qs, grabs=(2, 4, 8), blocks_per_cu=(0,))
+                self.tuning = best
+                for q in live:
+                    q.dq.set_dfs_tuning(*best)
+            for q, (c, _) in zip(ks, self.eng.matching_batch(qs)):
+                out[q.reg.name] = c
+            list(self.pool.map(lambda q: q.dq.batch_update_global_index(), ks))
+        for q in live:
+            q.dq.sync()
+        return out
+
+    # ------------------------------------------------------------ the loop
+    def run(self, registrations, arrivals):
+        """registrations: Registration list; arrivals: [(arrival_s, pairs)] per
+        stream batch, in stream order (seconds after the loop starts, which is
+        after the queries registered at time <= 0 are loaded).  Returns counts
+        {name: {batch: count}}, per-batch latency (ms, arrival -> every live
+        count available), and admission records."""
+        pending = sorted(registrations, key=lambda r: r.arrive_s)
+        waiting, catching, live = [], [], []
+        counts = {r.name: {} for r in registrations}
+        lat, admitted, rejected_oom = [], {}, 0
+        blocked = False  # a load failed for memory: wait for a departure
+        # Queries registered before the stream starts (arrive_s <= 0) are
+        # loaded first, within the budget, and are live from batch 0.
+        while pending and pending[0].arrive_s <= 0 and len(live) < self.max_live:
+            used, budget = self.eng.memory()
+            reserved = sum(q.reg.need() for q in live)
+            if budget and max(used, reserved) + pending[0].need() > budget:
+                break
+            q = _Query(pending.pop(0))
+            self._open(q)
+            q.joined_at = 0
+            live.append(q)
+            admitted[q.reg.name] = {"arrived_s": q.reg.arrive_s, "admitted_s": 0.0, "at_batch": 0,
+                                    "joined_batch": 0}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        n = len(arrivals)
+        t_first = arrivals[0][0] if arrivals else 0.0
+        while self.b_next < n:
+            b = self.b_next
+            # registrations that have arrived wait for admission
+            while pending and pending[0].arrive_s <= now():
+                waiting.append(pending.pop(0))
+            # admission: a free slot and room in the memory budget
+            used, budget = self.eng.memory()
+            while waiting and not blocked and len(live) + len(catching) < self.max_live:
+                # reservations: every admitted query holds its footprint
+                reserved = sum(q.reg.need() for q in live + catching)
+                if budget and max(used, reserved) + waiting[0].need() > budget:
+                    blocked = True  # no room: wait for a departure
+                    break
+                r = waiting.pop(0)
+                q = _Query(r)
+                catching.append(q)
+                admitted[r.name] = {"arrived_s": r.arrive_s, "admitted_s": now(), "at_batch": b}
+                threading.Thread(target=self._catch_up, args=(q,), daemon=True).start()
+            # joins at this batch boundary; failed loads go back to the queue
+            for q in list(catching):
+                with q.cond:
+                    if not q.ready:
+                        continue
+                    if q.failed is not None:
+                        catching.remove(q)
+                        if isinstance(q.failed, CgbdsmError) and q.failed.code == OOM:
+                            rejected_oom += 1
+                            blocked = True
+                            waiting.insert(0, q.reg)
+                            admitted.pop(q.reg.name, None)
+                            continue
+                        raise q.failed
+                    if q.applied == b:
+                        q.joined_at = b
+                        catching.remove(q)
+                        live.append(q)
+                        admitted[q.reg.name]["joined_batch"] = b
+                    q.cond.notify_all()
+            # the batch itself
+            t_arr = arrivals[b][0]
+            if now() < t_arr:
+                time.sleep(min(t_arr - now(), 0.001))
+                continue
+            res = self._serve_batch(live, b) if live else {}
+            t_done = now()
+            for name, c in res.items():
+                counts[name][b] = c
+            lat.append(1e3 * (t_done - t_arr))
+            with self.lock:
+                self.b_next = b + 1
+            for q in catching:
+                with q.cond:
+                    q.cond.notify_all()
+            # departures after this batch free their memory
+            for q in [q for q in live if q.reg.leave_after is not None and q.reg.leave_after <= b]:
+                live.remove(q)
+                q.dq.close()
+                q.dq = None
+                blocked = False
+        t_end = now()
+        for q in live + catching:
+            if q.dq is not None:
+                q.dq.close()
+        edges = sum(p for _, p in arrivals)
+        return {
+            "counts": counts,
+            "latency_ms": lat,
+            "admissions": admitted,
+            "oom_retries": rejected_oom,
+            "update_edges_per_s": edges / max(t_end - t_first, 1e-9),
+            "p50_ms": statistics.median(lat) if lat else 0.0,
+            "p99_ms": sorted(lat)[min(len(lat) - 1, int(0.99 * len(lat)))] if lat else 0.0,
+            "dfs_tuning": self.tuning,
+        }
+
+    def close(self):
+        self.pool.shutdown()

@@ -124,6 +124,10 @@ const char* cgbdsm_version(void);
  * (main_calig.cpp:64, graph/graph_gpu.cu:58-102). budget_bytes = 0: no cap. */
 cgbdsm_status cgbdsm_ctx_create(int device, uint64_t budget_bytes, cgbdsm_ctx** out);
 cgbdsm_status cgbdsm_ctx_destroy(cgbdsm_ctx* ctx);
+/* Device bytes held by the context's queries and its budget (0: none); a
+ * serving loop admits work against them.  No reference counterpart (its
+ * pool is a fixed 256 MB, utils/config.h:19). */
+cgbdsm_status cgbdsm_ctx_memory(cgbdsm_ctx* ctx, uint64_t* used_bytes, uint64_t* budget_bytes);
 
 /* Register a query: GPUGraphLoader ctor + LoadQuery + AllocRelations
  * (graph/graph_gpu.h:152-169, graph/graph_gpu.cu:156-197).  Only the query

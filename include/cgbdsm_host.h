@@ -80,6 +80,12 @@ uint32_t cgh_num_batches(cgh_session* s);
 uint32_t cgh_num_threads(cgh_session* s);
 /* Run Stage 1 for the next batch; pointers stay valid until the next call. */
 int cgh_next_batch(cgh_session* s, cgh_batch* out);
+/* Stage 1 for a batch of the next n_edges stream edges (fewer at the end of
+ * the stream): batches of any size, e.g. a serving loop's arrivals.  The
+ * stream is consumed in order by both calls. */
+int cgh_next_batch_edges(cgh_session* s, uint64_t n_edges, cgh_batch* out);
+/* Stream ints (two per edge) not yet consumed by Stage 1. */
+uint64_t cgh_stream_left(cgh_session* s);
 
 /* Compressed id -> original vertex id. */
 int cgh_old_ids(cgh_session* s, const uint32_t** ids, uint32_t* n);
