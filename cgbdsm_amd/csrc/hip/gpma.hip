@@ -144,6 +144,19 @@ __global__ void KPmaLevelUp(uint64_t* __restrict__ S, uint32_t nleaves, uint32_t
         cur[w] = prev[2 * w] + prev[2 * w + 1];
 }
 
+// The upper levels of the sum tree (at most 2048 windows each) in one
+// block, level after level: one launch instead of one per level.
+__global__ void __launch_bounds__(1024) KPmaLevelsTop(uint64_t* __restrict__ S, uint32_t nleaves, uint32_t l0,
+                                                      uint32_t height) {
+    for (uint32_t l = l0; l <= height; l++) {
+        const uint64_t* prev = S + LevelOff(nleaves, l - 1);
+        uint64_t* cur = S + LevelOff(nleaves, l);
+        const uint32_t nw = nleaves >> l;
+        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) cur[w] = prev[2 * w] + prev[2 * w + 1];
+        __syncthreads();
+    }
+}
+
 // Each leaf with inserts takes the smallest window around it that stays
 // under its level's density bound; no window fits: the array must grow.
 __global__ void KPmaChoose(const uint32_t* __restrict__ ins_cnt, const uint64_t* __restrict__ S, uint32_t nleaves,
@@ -401,8 +414,10 @@ void LaunchPmaWindows(const uint32_t* cnt, uint32_t* ins_cnt, const uint32_t* in
     FillU32(ins_cnt, 0, nleaves, s);
     if (n_upper) KPmaInsCount<<<GridFor64(n_upper), 256, 0, s>>>(ins_leaf, n_ins_dev, ins_cnt);
     KPmaLevel0<<<GridFor64(nleaves), 256, 0, s>>>(cnt, ins_cnt, nleaves, S);
-    for (uint32_t l = 1; l <= height; l++)
+    uint32_t l = 1;
+    for (; l <= height && (nleaves >> l) > 2048; l++)
         KPmaLevelUp<<<GridFor64(nleaves >> l), 256, 0, s>>>(S, nleaves, l);
+    if (l <= height) KPmaLevelsTop<<<1, 1024, 0, s>>>(S, nleaves, l, height);
     // Clear the chosen flags with a kernel (no memset node if the caller captures).
     FillU32((uint32_t*)chosen, 0, (2ull * nleaves + 3) / 4, s);
     KPmaChoose<<<GridFor64(nleaves), 256, 0, s>>>(ins_cnt, S, nleaves, height, chosen, counters);

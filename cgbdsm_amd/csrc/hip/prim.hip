@@ -81,10 +81,45 @@ __global__ void __launch_bounds__(kScanThreads) KScanApply(const T* in, T* out, 
     }
 }
 
+// Whole scan in one block, tile after tile with a running carry (in place
+// allowed: a tile is read into registers before it is written).  For short
+// inputs one launch instead of three: the per-batch index kernels run dozens
+// of such scans per query.
+template <class T>
+__global__ void __launch_bounds__(kScanThreads) KScanSingle(const T* in, T* out, uint64_t n, uint64_t* total) {
+    T carry = 0;
+    for (uint64_t base = 0; base < n; base += kScanTile) {
+        const uint64_t b0 = base + (uint64_t)threadIdx.x * kScanItems;
+        T v[kScanItems];
+        T sum = 0;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            v[i] = b0 + i < n ? in[b0 + i] : 0;
+            sum += v[i];
+        }
+        T tot;
+        T ex = BlockExclusiveScan(sum, &tot) + carry;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            if (b0 + i < n) out[b0 + i] = ex;
+            ex += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = (uint64_t)carry;
+}
+
+constexpr uint64_t kScanSingleMax = 16 * kScanTile;  // 32768 elements
+
 template <class T>
 void ScanImpl(const T* in, T* out, uint64_t n, uint64_t* total_dev, void* scratch, hipStream_t s) {
     if (n == 0) {
         if (total_dev) FillU64(total_dev, 0, 1, s);
+        return;
+    }
+    if (n <= kScanSingleMax) {
+        KScanSingle<T><<<1, kScanThreads, 0, s>>>(in, out, n, total_dev);
+        CGD_HIP(hipGetLastError());
         return;
     }
     const uint64_t nt = (n + kScanTile - 1) / kScanTile;
