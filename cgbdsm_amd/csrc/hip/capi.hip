@@ -1263,7 +1263,7 @@ void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
 
 void ResetDfsCounters(cgbdsm_query* q) {
     for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
-                   &Counters::idle_ticks, &Counters::steps, &Counters::step_lanes})
+                   &Counters::idle_ticks, &Counters::steps, &Counters::step_lanes, &Counters::batched})
         SetCounter(q, f, 0);
 }
 
@@ -1286,6 +1286,7 @@ void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* laun
     q->st.last_dfs_donated = c->donated;
     q->st.last_dfs_steps = c->steps;
     q->st.last_dfs_step_lanes = c->step_lanes;
+    q->st.last_dfs_batched = c->batched;
     q->st.last_dfs_busy = c->wave_ticks ? 1.0 - (double)c->idle_ticks / (double)c->wave_ticks : 0.0;
     unsigned long long total = 0;
     for (uint32_t e = 0; e < q->qe; e++) {

@@ -1053,7 +1053,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         bsfx = cart && T.P->sfx_batch;
     };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
-    unsigned long long my_scanned = 0, my_probes = 0, donated = 0, steps = 0, step_lanes = 0;
+    unsigned long long my_scanned = 0, my_probes = 0, donated = 0, steps = 0, step_lanes = 0, batched = 0;
     uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
     bool primary = true, idle = false, stop = false, have_ticket = false, from_row = false;
     uint32_t iter = 0, backoff = 1;
@@ -1325,8 +1325,11 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             // Last position before the product suffix: count the suffix of
             // every survivor of this step at once instead of descending.
             if (blev) {
-                if (m) count += BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv], my_scanned,
-                                            my_probes);
+                if (m) {
+                    count += BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv], my_scanned,
+                                         my_probes);
+                    batched++;
+                }
                 if (lane == 0) lvl[d].off = off + kWave;
                 WaveSync();
                 continue;
@@ -1379,6 +1382,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         atomicAdd(&cnt->probes, p_tot);
         atomicAdd(&cnt->steps, steps);
         atomicAdd(&cnt->step_lanes, step_lanes);
+        if (batched) atomicAdd(&cnt->batched, batched);
         if (donated) atomicAdd(&cnt->donated, donated);
     }
     flush_pe();

@@ -106,6 +106,8 @@ typedef struct {
                                      descent loop (suffix sizing excluded)        */
     uint64_t last_dfs_step_lanes; /* candidates those steps covered: lane
                                      utilisation = step_lanes / (64 * steps)      */
+    uint64_t last_dfs_batched;    /* steps whose product suffixes were counted
+                                     for all their survivors at once              */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
     double   last_dfs_busy;       /* share of final-DFS wave time spent working   */
 } cgbdsm_stats;

@@ -53,7 +53,7 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
     idx.init(keys)
     if check_index:
         assert np.array_equal(q.export(0), idx.export(0)), "initial M differs"
-    res = {"batches": 0, "total": 0, "mismatches": 0, "counts": [], "per_edge": [], "skipped": 0}
+    res = {"batches": 0, "total": 0, "mismatches": 0, "counts": [], "per_edge": [], "skipped": 0, "dfs_batched": 0}
     nb = host.num_batches if max_batches is None else min(max_batches, host.num_batches)
     for _ in range(nb):
         kb = host.next_batch()
@@ -88,6 +88,7 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
                 if log:
                     log(f"batch {kb.index}: masks differ {bad[:5]}")
         cnt, per = q.matching()
+        res["dfs_batched"] += q.stats()["last_dfs_batched"]
         if tuples_cap:
             ocnt, oper, otup, otei = idx.match(plan, kb.edge_ok, masks=om, tuples_cap=tuples_cap)
             n, tup, tei = q.tuples(tuples_cap)

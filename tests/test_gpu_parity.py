@@ -140,6 +140,53 @@ def test_gpma_layout_parity(eng, case):
         assert st["gpma_top_level"] >= 2      # windows above the leaf level
 
 
+# Few labels: suffix positions share label classes with earlier positions
+# (hits), stable intersections outgrow the LDS cache, single-list positions
+# with real masks - the batched product-suffix count against the oracle.
+SUFFIX_STREAMS = [
+    ("sfx-s12-l2-dense5", 12, 8, 2, 5, "dense", 3, 400, None),
+    ("sfx-s12-l3-dense6", 12, 10, 3, 6, "dense", 5, 400, None),
+    ("sfx-s13-l2-tree5", 13, 8, 2, 5, "tree", 6, 800, None),
+    ("sfx-s12-l2-sparse6", 12, 8, 2, 6, "sparse", 4, 40, 2),
+    ("sfx-s12-l4-dense6", 12, 8, 4, 6, "dense", 22, 400, None),
+    ("sfx-s12-l4-sparse6", 12, 8, 4, 6, "sparse", 21, 400, None),
+    ("sfx-s12-l4-tree6", 12, 8, 4, 6, "tree", 20, 400, None),
+]
+
+
+@pytest.mark.parametrize("case", SUFFIX_STREAMS, ids=[c[0] for c in SUFFIX_STREAMS])
+def test_batched_suffix_parity(eng, case):
+    """DFS from the depth-2 seeds (no BFS levels), so that the position
+    before the product suffix is scanned by the DFS itself."""
+    name, scale, deg, labels, qs, kind, seed, be, maxb = case
+    print(f"case {name}", flush=True)
+    spec = synth_spec(scale, deg, labels, seed=seed, query_size=qs, query_kind=kind, query_seed=seed)
+    host = HostSession(spec=spec, batch_edges=be, threads=4)
+    res = parity.run_stream(host, eng, check_masks=False, max_batches=maxb, match_params={"bfs_row_limit": 0},
+                            log=_log)
+    print(f"{name}: total {res['total']}, batched suffix steps {res['dfs_batched']}", flush=True)
+    assert res["batches"] > 0
+    assert res["mismatches"] == 0
+    if name in ("sfx-s12-l2-dense5", "sfx-s13-l2-tree5", "sfx-s12-l4-dense6", "sfx-s12-l4-sparse6",
+                "sfx-s12-l4-tree6"):
+        assert res["dfs_batched"] > 0   # every class of the suffix has one member
+
+
+def test_batched_suffix_is_exercised(eng):
+    """With the product suffix off the batched path is not taken; counts are
+    the same either way."""
+    spec = synth_spec(12, 8, 2, seed=3, query_size=5, query_kind="dense", query_seed=3)
+    mp = {"bfs_row_limit": 0}
+    on = parity.run_stream(HostSession(spec=spec, batch_edges=400, threads=4), eng, check_index=False,
+                           check_masks=False, match_params=mp, log=_log)
+    off = parity.run_stream(HostSession(spec=spec, batch_edges=400, threads=4), eng, check_index=False,
+                            check_masks=False, match_params=mp, dfs_options={"product_suffix": False}, log=_log)
+    assert on["mismatches"] == 0 and off["mismatches"] == 0
+    assert on["counts"] == off["counts"]
+    assert off["dfs_batched"] == 0
+    assert on["dfs_batched"] > 0
+
+
 def test_rejects_out_of_range_keys(eng):
     spec = synth_spec(8, 4, 2, seed=9, query_size=4, query_kind="tree", query_seed=9)
     host = HostSession(spec=spec, batch_edges=None, threads=2)
