@@ -106,6 +106,8 @@ def host_lib():
         L.cgh_num_threads.argtypes = [P]
         L.cgh_num_threads.restype = U32
         L.cgh_next_batch.argtypes = [P, C.POINTER(Batch)]
+        L.cgh_generate_binary.argtypes = [C.POINTER(SynthSpec), C.c_char_p, C.c_char_p]
+        L.cgh_export_updates.argtypes = [P, C.c_void_p, U64, C.POINTER(U64)]
         L.cgh_next_batch_edges.argtypes = [P, U64, C.POINTER(Batch)]
         L.cgh_stream_left.argtypes = [P]
         L.cgh_stream_left.restype = U64

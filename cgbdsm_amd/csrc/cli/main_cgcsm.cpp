@@ -62,7 +62,7 @@ std::string BaseName(const char* p) {
 }
 
 int Usage(const std::string& name) {
-    std::cerr << "usage: " << name << " <query> <data.graph> <update> <gpu_id> [batch_size]\n"
+    std::cerr << "usage: " << name << " <query> <data.graph | binary CSR prefix> <update> <gpu_id> [batch_size]\n"
               << "  batch_size counts ints of the update stream (two per edge); default: one batch\n";
     return 2;
 }

@@ -1,6 +1,8 @@
 #include "calig.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <random>
 #include <cstring>
 #include <fstream>
 
@@ -130,6 +132,65 @@ void CaLiG::InputG(const std::string& path) {
         }
     }
     SetData(labels, edges);
+}
+
+void CaLiG::InputGBin(const std::string& prefix) {
+    uint64_t m = 0;
+    uint32_t n = 0, vsize = 0, esize = 0, vlbl = 0, elbl = 0, maxdeg = 0, feat = 0, nvcls = 0, necls = 0;
+    {
+        std::ifstream meta(prefix + ".meta.txt");
+        if (!meta) throw Error("cannot open " + prefix + ".meta.txt");
+        meta >> n >> m >> vsize >> esize >> vlbl >> elbl >> maxdeg >> feat >> nvcls >> necls;
+        if (!meta) throw Error("bad meta file " + prefix + ".meta.txt");
+    }
+    if (vsize != 4 || esize != 8) throw Error("binary CSR needs 4-byte vertex and 8-byte edge ids");
+    if (nvcls == 0) throw Error("binary CSR meta names no vertex label classes");
+    std::vector<uint64_t> offs((size_t)n + 1);
+    std::vector<uint32_t> adj(m);
+    {
+        std::ifstream f(prefix + ".vertex.bin", std::ios::binary);
+        if (!f.read(reinterpret_cast<char*>(offs.data()), (std::streamsize)(offs.size() * sizeof(uint64_t))))
+            throw Error("short read of " + prefix + ".vertex.bin");
+    }
+    {
+        std::ifstream f(prefix + ".edge.bin", std::ios::binary);
+        if (m && !f.read(reinterpret_cast<char*>(adj.data()), (std::streamsize)(m * sizeof(uint32_t))))
+            throw Error("short read of " + prefix + ".edge.bin");
+    }
+    if (offs[n] != m) throw Error("binary CSR: row_ptr[n] differs from the edge count");
+    for (uint32_t v = 0; v < n; v++)
+        if (offs[v] > offs[v + 1]) throw Error("binary CSR: row_ptr not monotone");
+    // Labels: one rand() draw per vertex in id order, from the C library's
+    // initial state (the reference draws them first thing in its process).
+    std::vector<int> labels(n);
+    std::srand(1);
+    for (uint32_t v = 0; v < n; v++) labels[v] = std::rand() % (int)nvcls;
+    // Every tenth undirected edge by a multiplicative hash of (min, max) goes
+    // to the update stream as (larger id, smaller id); the rest is the graph.
+    auto streamed = [](uint32_t a, uint32_t b) {
+        const uint64_t key = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
+        return (key * 0x9e3779b97f4a7c15ULL) % 10ULL == 0;
+    };
+    std::vector<std::pair<uint32_t, uint32_t>> edges, upd;
+    edges.reserve(m / 2);
+    for (uint32_t v = 0; v < n; v++)
+        for (uint64_t i = offs[v]; i < offs[v + 1]; i++) {
+            const uint32_t u = adj[i];
+            if (u >= n) throw Error("binary CSR: neighbour id out of range");
+            if (streamed(v, u)) {
+                if (v < u) upd.emplace_back(u, v);
+            } else if (v < u) {
+                edges.emplace_back(v, u);
+            }
+        }
+    std::vector<size_t> order(upd.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::mt19937 rng(42);
+    std::shuffle(order.begin(), order.end(), rng);
+    std::vector<std::pair<uint32_t, uint32_t>> shuffled(upd.size());
+    for (size_t i = 0; i < order.size(); i++) shuffled[i] = upd[order[i]];
+    SetData(labels, edges);
+    SetUpdates(shuffled);
 }
 
 void CaLiG::SetData(const std::vector<int>& labels, const std::vector<std::pair<uint32_t, uint32_t>>& edges) {

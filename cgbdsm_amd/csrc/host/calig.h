@@ -48,6 +48,12 @@ public:
     void InputQ(const std::string& path);
     void InputG(const std::string& path);
     void InputUpdate(const std::string& path, int max_num = 0);
+    // Binary CSR prefix (index/calig.cpp:218-323): <prefix>.meta.txt,
+    // .vertex.bin (u64 row_ptr[n+1]), .edge.bin (u32 adj[m]).  Labels are
+    // drawn with the C library's rand() from its initial state, every tenth
+    // edge (by a multiplicative hash of the pair) becomes the update stream,
+    // shuffled by std::mt19937(42).
+    void InputGBin(const std::string& prefix);
     // In-memory equivalents (labels are the raw integer labels).
     void SetQuery(const std::vector<int>& labels, const std::vector<std::pair<uint32_t, uint32_t>>& edges);
     void SetData(const std::vector<int>& labels, const std::vector<std::pair<uint32_t, uint32_t>>& edges);
@@ -61,6 +67,7 @@ public:
     BatchDeltas MergeDeltas();                                 // :1071-1088
 
     uint32_t UpdateSize() const { return (uint32_t)update_.size(); }
+    const std::vector<int>& UpdateStream() const { return update_; }
     uint32_t NumVertices() const { return (uint32_t)glabel_.size(); }
     int Threads() const { return pool_.size(); }
     ThreadPool& Pool() { return pool_; }

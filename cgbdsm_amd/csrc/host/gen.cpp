@@ -215,6 +215,37 @@ void WriteStream(const std::string& path, const SynthGraph& g) {
     fclose(f);
 }
 
+void WriteBinaryCsr(const std::string& prefix, const SynthGraph& g, uint32_t num_labels) {
+    std::vector<std::vector<uint32_t>> adj(g.n);
+    for (const auto* part : {&g.initial, &g.stream})
+        for (auto& [a, b] : *part) {
+            if (a == b || a >= g.n || b >= g.n) continue;
+            adj[a].push_back(b);
+            adj[b].push_back(a);
+        }
+    std::vector<uint64_t> offs((size_t)g.n + 1, 0);
+    uint32_t maxdeg = 0;
+    for (uint32_t v = 0; v < g.n; v++) {
+        auto& s = adj[v];
+        std::sort(s.begin(), s.end());
+        s.erase(std::unique(s.begin(), s.end()), s.end());
+        offs[v + 1] = offs[v] + s.size();
+        maxdeg = std::max<uint32_t>(maxdeg, (uint32_t)s.size());
+    }
+    FILE* f = fopen((prefix + ".meta.txt").c_str(), "w");
+    if (!f) throw Error("cannot write " + prefix + ".meta.txt");
+    fprintf(f, "%u %llu 4 8 4 0 %u 0 %u 0\n", g.n, (unsigned long long)offs[g.n], maxdeg, num_labels);
+    fclose(f);
+    f = fopen((prefix + ".vertex.bin").c_str(), "wb");
+    if (!f) throw Error("cannot write " + prefix + ".vertex.bin");
+    fwrite(offs.data(), sizeof(uint64_t), offs.size(), f);
+    fclose(f);
+    f = fopen((prefix + ".edge.bin").c_str(), "wb");
+    if (!f) throw Error("cannot write " + prefix + ".edge.bin");
+    for (uint32_t v = 0; v < g.n; v++) fwrite(adj[v].data(), sizeof(uint32_t), adj[v].size(), f);
+    fclose(f);
+}
+
 void WriteQueryFile(const std::string& path, const SampledQuery& q) {
     FILE* f = fopen(path.c_str(), "w");
     if (!f) throw Error("cannot write " + path);

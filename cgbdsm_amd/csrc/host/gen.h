@@ -55,6 +55,9 @@ SampledQuery SampleQuery(const SynthGraph& g, uint32_t size, QueryKind kind, uin
 
 void WriteDataGraph(const std::string& path, const SynthGraph& g);
 void WriteStream(const std::string& path, const SynthGraph& g);
+// The whole graph (initial and stream edges) as the binary CSR prefix read
+// by CaLiG::InputGBin; num_labels is written as the vertex label classes.
+void WriteBinaryCsr(const std::string& prefix, const SynthGraph& g, uint32_t num_labels);
 void WriteQueryFile(const std::string& path, const SampledQuery& q);
 
 }  // namespace cg

@@ -17,8 +17,15 @@ Session::Session(const std::string& qp, const std::string& dp, const std::string
                  int threads, uint32_t ebits) {
     calig_ = std::make_unique<CaLiG>(threads);
     calig_->InputQ(qp);
-    calig_->InputG(dp);
-    calig_->InputUpdate(up, 0);
+    // main_calig.cpp:93-101: a data path ending in ".graph" is text (with the
+    // update file); anything else is a binary CSR prefix whose update stream
+    // is split off the graph itself.
+    if (dp.size() >= 6 && dp.compare(dp.size() - 6, 6, ".graph") == 0) {
+        calig_->InputG(dp);
+        calig_->InputUpdate(up, 0);
+    } else {
+        calig_->InputGBin(dp);
+    }
     query_ = LoadQuery(qp);
     Setup(batch_ints, ebits);
 }
@@ -145,6 +152,16 @@ int cgh_generate_files(const cgh_synth_spec* spec, const char* qp, const char* d
     })
 }
 
+int cgh_generate_binary(const cgh_synth_spec* spec, const char* qp, const char* prefix) {
+    CGH_TRY({
+        cg::ThreadPool pool(cg::DefaultThreads());
+        cg::SynthGraph g = cg::GenerateRmat(ToRmat(spec), &pool);
+        cg::SampledQuery q = cg::SampleQuery(g, spec->query_size, (cg::QueryKind)spec->query_kind, spec->query_seed);
+        cg::WriteQueryFile(qp, q);
+        cg::WriteBinaryCsr(prefix, g, spec->num_labels);
+    })
+}
+
 int cgh_session_open(const char* qp, const char* dp, const char* up, uint32_t batch_ints, int threads,
                      uint32_t ebits, cgh_session** out) {
     CGH_TRY({
@@ -229,6 +246,15 @@ int cgh_next_batch(cgh_session* s, cgh_batch* out) {
         out->ms_construct_update = t.construct_update_ms;
         out->ms_prepare_all = t.prepare_all_ms;
         out->ms_prepare_index = t.prepare_index_ms;
+    })
+}
+
+int cgh_export_updates(cgh_session* s, uint32_t* out, uint64_t cap, uint64_t* n_out) {
+    CGH_TRY({
+        const auto& u = s->s->calig().UpdateStream();
+        *n_out = u.size();
+        if (out)
+            for (uint64_t i = 0; i < u.size() && i < cap; i++) out[i] = (uint32_t)u[i];
     })
 }
 

@@ -34,6 +34,13 @@ def generate_files(spec, query_path, data_path, update_path):
                                 str(update_path).encode()), L)
 
 
+def generate_binary(spec, query_path, prefix):
+    """Query text file + the graph as a binary CSR prefix (update stream
+    split off by the reader)."""
+    L = N.host_lib()
+    _check(L.cgh_generate_binary(C.byref(spec), str(query_path).encode(), str(prefix).encode()), L)
+
+
 @dataclass
 class KeyBatch:
     index: int
@@ -129,6 +136,15 @@ class HostSession:
                         _arr(b.upd, b.n_upd, np.uint64), _arr(b.edge_ok, self.qe, np.uint8), b.dv, b.pairs,
                         {"updateIndex": b.ms_update_index, "ConstructUpdate": b.ms_construct_update,
                          "PrepareUpdateAll": b.ms_prepare_all, "PrepareUpdateIndex": b.ms_prepare_index})
+
+    def update_stream(self):
+        """The loaded update stream as (n, 2) original-id pairs, in order."""
+        L = self._L
+        n = C.c_uint64()
+        _check(L.cgh_export_updates(self._h, None, 0, C.byref(n)), L)
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        _check(L.cgh_export_updates(self._h, out.ctypes.data, n.value, C.byref(n)), L)
+        return out[: n.value].reshape(-1, 2)
 
     def old_ids(self):
         ptr = C.POINTER(C.c_uint32)()

@@ -56,6 +56,12 @@ const char* cgh_last_error(void);
 /* Write query / data-graph / update-stream text files for a synthetic spec. */
 int cgh_generate_files(const cgh_synth_spec* spec, const char* query_path, const char* data_path,
                        const char* update_path);
+/* The same graph (initial and stream edges together) as a binary CSR prefix
+ * (<prefix>.meta.txt, .vertex.bin, .edge.bin) plus the query text file.  A
+ * data path that does not end in ".graph" is read as such a prefix
+ * (main_calig.cpp:93-101, index/calig.cpp:218-323): labels from rand(),
+ * the update stream split off by a hash of each edge. */
+int cgh_generate_binary(const cgh_synth_spec* spec, const char* query_path, const char* prefix);
 
 /* Load files (CaLiG::inputQ/inputG/inputUpdate + CPUGraphLoader), then
  * constructCand + staticFilter (main_calig.cpp:88-116).  batch_ints counts
@@ -86,6 +92,9 @@ int cgh_next_batch(cgh_session* s, cgh_batch* out);
 int cgh_next_batch_edges(cgh_session* s, uint64_t n_edges, cgh_batch* out);
 /* Stream ints (two per edge) not yet consumed by Stage 1. */
 uint64_t cgh_stream_left(cgh_session* s);
+/* The whole update stream as loaded (ints, two per edge, in stream order);
+ * *n_out = its length, up to cap ints copied. */
+int cgh_export_updates(cgh_session* s, uint32_t* out, uint64_t cap, uint64_t* n_out);
 
 /* Compressed id -> original vertex id. */
 int cgh_old_ids(cgh_session* s, const uint32_t** ids, uint32_t* n);

@@ -108,3 +108,24 @@ def test_cli_counts_and_output_contract(tmp_path, name, batch_ints):
         assert want == closed
     tb = re.search(r"^Total batches: (\d+)$", out, re.M)
     assert tb and int(tb.group(1)) == nb
+
+
+@pytest.mark.gpu
+def test_cli_binary_prefix(tmp_path):
+    """A data path without ".graph" is a binary CSR prefix whose update stream
+    is split off the graph (main_calig.cpp:93-101); the count matches the
+    oracle pipeline and the closed form."""
+    from cgbdsm_amd.host import generate_binary
+
+    print("case cli binary prefix", flush=True)
+    spec = synth_spec(11, 8, 4, seed=2, query_size=5, query_kind="dense", query_seed=2)
+    qp, prefix = tmp_path / "q.graph", tmp_path / "g"
+    generate_binary(spec, qp, prefix)
+    files = (str(qp), str(prefix), str(tmp_path / "unused.txt"))
+    r = subprocess.run([_exe("cgcsm_merged_matching"), *files, "0", "600"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    m = re.search(r"^Num Positive Matches: (\d+)$", r.stdout, re.M)
+    assert m, r.stdout[-2000:]
+    want, closed, nb = _expected(files, 600)
+    assert int(m.group(1)) == want == closed
