@@ -125,9 +125,16 @@ __global__ void KPmaCompact(const uint64_t* __restrict__ keys, const uint32_t* _
 
 __global__ void KPmaInsCount(const uint32_t* __restrict__ leaf_of, const unsigned long long* __restrict__ n_dev,
                              uint32_t* __restrict__ ins_cnt) {
+    // Per run of equal leaves, its end is added and its start subtracted
+    // (mod 2^32): two atomics per run instead of one per key on a hot leaf.
+    // Each maximal run adds its length, so any order of leaf_of is counted
+    // right; the routed keys are sorted, so a leaf is one run.
     const uint64_t n = *n_dev;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        atomicAdd(&ins_cnt[leaf_of[i]], 1u);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t l = leaf_of[i];
+        if (i == 0 || leaf_of[i - 1] != l) atomicAdd(&ins_cnt[l], (uint32_t)(0u - (uint32_t)i));
+        if (i + 1 == n || leaf_of[i + 1] != l) atomicAdd(&ins_cnt[l], (uint32_t)(i + 1));
+    }
 }
 
 __global__ void KPmaLevel0(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ ins_cnt, uint32_t nleaves,

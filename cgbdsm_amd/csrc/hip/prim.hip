@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kScanThreads) KScanSingle(const T* in, T* out,
     if (threadIdx.x == 0 && total) *total = (uint64_t)carry;
 }
 
-constexpr uint64_t kScanSingleMax = 16 * kScanTile;  // 32768 elements
+constexpr uint64_t kScanSingleMax = 4 * kScanTile;  // 8192 elements (the block walks its tiles in turn)
 
 template <class T>
 void ScanImpl(const T* in, T* out, uint64_t n, uint64_t* total_dev, void* scratch, hipStream_t s) {
