@@ -567,7 +567,11 @@ __device__ __forceinline__ uint32_t ChooseList(const Ctx& X, const Tables& T, ui
     return Uniform(key) & 15u;
 }
 
-constexpr uint32_t kStabCap = 768;  // u32 per wave for the cached stable intersection
+// u32 per wave for the cached stable intersection: what the LDS left by the
+// other per-wave tables allows at 3 blocks (12 waves) per CU, 53 KB a block.
+// Intersections that do not fit take the uncached path; 768 -> 1536 took
+// the dense 6-vertex bench stream from 30.4 to 26.4 ms per DFS launch.
+constexpr uint32_t kStabCap = 1536;
 
 __device__ __forceinline__ bool LdsContains(const uint32_t* a, uint32_t n, uint32_t c) {
     uint32_t lo = 0, hi = n;
