@@ -549,7 +549,8 @@ def main():
         log(f"batch {warm + i}: {c} matches in {lat_ms[-1]:.1f} ms, frontier {st['last_frontier_rows']} rows at "
             f"depth {st['last_frontier_depth']}, scanned {st['last_dfs_scanned']}, probes {st['last_dfs_probes']}, "
             f"donated {st['last_dfs_donated']}, busy {st['last_dfs_busy']:.3f}, steps {st['last_dfs_steps']}, "
-            f"lanes/step {st['last_dfs_step_lanes'] / max(1, st['last_dfs_steps']):.1f}")
+            f"lanes/step {st['last_dfs_step_lanes'] / max(1, st['last_dfs_steps']):.1f}, "
+            f"LDS cache {st['last_dfs_lds_cache']} u32/wave")
     q.sync()
     elapsed = time.perf_counter() - t0
     kt = q.kernel_times()

@@ -37,7 +37,7 @@ class Stats(C.Structure):
         "last_upd", "last_rows_touched", "last_idx_bytes", "last_frontier_rows",
         "last_frontier_depth", "last_matches", "compactions", "last_dfs_scanned", "last_dfs_probes",
         "last_dfs_donated", "gpma_rebuilds", "gpma_windows", "gpma_top_level", "last_dfs_steps",
-        "last_dfs_step_lanes", "last_dfs_batched")] + [
+        "last_dfs_step_lanes", "last_dfs_batched", "last_dfs_lds_cache")] + [
         (n, C.c_double) for n in ("ms_apply", "ms_update_index", "ms_masks", "ms_match", "ms_commit",
                                   "last_dfs_busy")]
 

@@ -446,9 +446,11 @@ void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
     CGD_HIP(hipMemcpyAsync(q->d_rowbase.p, q->h_rowbase.data(), q->h_rowbase.size() * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, q->s));
     const int k = tuples ? -1 : KBegin(q, CGBDSM_K_DFS);
-    const uint32_t bpc = q->dfs_bpc ? std::min(q->dfs_bpc, DfsBlocksPerCU()) : DfsBlocksPerCU();
+    uint32_t levels = 1;  // survivor stacks: qv - 1 levels cover every job
+    for (const DfsJob& j : q->h_jobs) levels = std::max<uint32_t>(levels, j.T.qv - 1);
+    q->st.last_dfs_lds_cache = DfsLdsShape(levels, tuples != nullptr).stab_cap;
     LaunchDfs(q->d_jobs.p, (uint32_t)q->h_jobs.size(), q->d_rowbase.p, n, q->d_cnt, MakeSteal(q, tuples == nullptr),
-              tuples, tup_cap, q->ctx->num_cus * bpc, q->dfs_grab, q->s);
+              tuples, tup_cap, q->ctx->num_cus, q->dfs_bpc, q->dfs_grab, levels, q->s);
     KEnd(q, k);
     CGD_HIP(hipGetLastError());
     if (q->deadline_ms > 0) {

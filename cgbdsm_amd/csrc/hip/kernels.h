@@ -72,9 +72,16 @@ struct DfsJob {
 };
 // Final DFS over the rows of njobs queries (row_base[j] = first global row of
 // job j, row_base[njobs] = all rows); `cnt` holds the shared work queue.
+// surv_levels >= qv - 1 of every job (survivor stacks in LDS); blocks_per_cu
+// 0 = as many as fit.
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
-               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, uint32_t grab,
-               hipStream_t s);
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t num_cus, uint32_t blocks_per_cu,
+               uint32_t grab, uint32_t surv_levels, hipStream_t s);
+// LDS plan of a final-DFS launch (see DfsLdsShapeT in match.hip).
+struct DfsShape {
+    uint32_t blocks_per_cu, dyn_bytes, stab_cap;
+};
+DfsShape DfsLdsShape(uint32_t surv_levels, bool tuples);
 // Resident KDfs blocks per CU at full occupancy.
 uint32_t DfsBlocksPerCU();
 

@@ -19,6 +19,8 @@
 //   directions) its U list, plus its U list when that query edge's index < ei;
 //   support mask bit of (ei, u_d); injective w.r.t. positions 0..d-1; present
 //   in every other backward neighbour's list (binary search in M then U).
+#include <mutex>
+
 #include "engine.h"
 #include "kernels.h"
 #include "wave.h"
@@ -567,11 +569,10 @@ __device__ __forceinline__ uint32_t ChooseList(const Ctx& X, const Tables& T, ui
     return Uniform(key) & 15u;
 }
 
-// u32 per wave for the cached stable intersection: what the LDS left by the
-// other per-wave tables allows at 3 blocks (12 waves) per CU, 53 KB a block.
-// Intersections that do not fit take the uncached path; 768 -> 1536 took
+// The cached stable intersection takes the LDS that the other per-wave
+// tables leave at full occupancy (sized per launch: DfsLdsShape);
+// intersections that do not fit take the uncached path.  768 -> 1536 u32 took
 // the dense 6-vertex bench stream from 30.4 to 26.4 ms per DFS launch.
-constexpr uint32_t kStabCap = 1536;
 
 __device__ __forceinline__ bool LdsContains(const uint32_t* a, uint32_t n, uint32_t c) {
     uint32_t lo = 0, hi = n;
@@ -591,6 +592,7 @@ __device__ __forceinline__ bool LdsContains(const uint32_t* a, uint32_t n, uint3
 // than the buffer (not cached).
 struct StabCache {
     uint32_t* buf;
+    uint32_t cap;  // u32 the buffer holds
     uint32_t len, la, state;
     uint32_t pos, vk, deps;  // the cached position, its deepest backward neighbour, positions S depends on
 };
@@ -635,7 +637,7 @@ __device__ __forceinline__ void StabBuild(const Ctx& X, const Tables& T, uint32_
         }
         const uint64_t m = __ballot(ok);
         const uint32_t cnt = (uint32_t)__popcll(m);
-        if (len + cnt > kStabCap) {
+        if (len + cnt > sc.cap) {
             big = true;
             break;
         }
@@ -1003,19 +1005,24 @@ template <bool kTuples>
 __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __restrict__ jobs, uint32_t njobs,
                                                               const unsigned long long* __restrict__ row_base,
                                                               uint32_t grab, Counters* __restrict__ cnt, StealQueue Q,
-                                                              uint32_t* __restrict__ tuples, uint64_t tup_cap) {
+                                                              uint32_t* __restrict__ tuples, uint64_t tup_cap,
+                                                              uint32_t surv_levels, uint32_t stab_cap) {
     // Per-edge counts gather in LDS per wave and reach the query's counters
     // when the wave changes query or ends: a device-scope atomic per item on
     // one address would serialize.
     __shared__ unsigned long long s_pe[kWpb][kMaxQE];
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
     __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
-    __shared__ uint32_t s_surv[kWpb][kMaxQV][kWave];
     __shared__ PlanLds s_plan[kWpb];
     __shared__ uint32_t s_csize[kWpb][kMaxQV];
-    __shared__ uint32_t s_stab[kWpb][kStabCap];
     __shared__ BxLds s_bx[kWpb];
+    // Dynamic LDS, per wave: survivor stacks of surv_levels levels (64 u32
+    // each; every job of the launch has qv - 1 <= surv_levels) and the
+    // stable-intersection cache of stab_cap u32.
+    extern __shared__ uint32_t s_dyn[];
     const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
+    uint32_t* surv = s_dyn + wv * (surv_levels * kWave + stab_cap);
+    uint32_t* stab = surv + surv_levels * kWave;
     uint32_t* matched = s_matched[wv];
     DfsLevel* lvl = s_lvl[wv];
     unsigned long long* pe = s_pe[wv];
@@ -1027,7 +1034,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     const uint32_t* rows = nullptr;
     Counters* jc = nullptr;
     Ctx X = MakeCtx(T, &s_plan[wv]);
-    StabCache sc{s_stab[wv], 0, 0, 0, 0xFFu, 0, 0};
+    StabCache sc{stab, stab_cap, 0, 0, 0, 0xFFu, 0, 0};
     bool sets = false, cart = false, bsfx = false;
     auto flush_pe = [&]() {
         if (!jc) return;
@@ -1052,7 +1059,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         last = qv - 1;
         tag_shift = T.P->tag_shift;
         v_mask = T.P->v_mask;
-        sc = StabCache{s_stab[wv], 0, 0, 0, T.P->stab_pos, T.P->stab_var_k, T.P->stab_deps};
+        sc = StabCache{stab, stab_cap, 0, 0, 0, T.P->stab_pos, T.P->stab_var_k, T.P->stab_deps};
         // Every list is a set when the update lists hold no duplicate value.
         const bool lists_are_sets = Uniform64(jc->u_dups) == 0;
         const uint32_t opts = jobs[j].opts;
@@ -1234,7 +1241,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                                 PutAgent(&it->depth, d + 1);
                                 PutAgent(&it->ei, ei | (kNoSel << 8) | (job << 16));
                                 for (uint32_t f = 0; f < d; f++) PutAgent(&it->v[f], matched[f]);
-                                PutAgent(&it->v[d], s_surv[wv][d][ccount - k + lane]);
+                                PutAgent(&it->v[d], surv[d * kWave + ccount - k + lane]);
                                 PutAgent(&it->lo, 0);
                                 PutAgent(&it->hi, kFull);
                                 PublishItem(Q, slot, t0 + lane);
@@ -1246,7 +1253,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                             WaveSync();
                         }
                     }
-                    const uint32_t x = Uniform(s_surv[wv][d][pos]);
+                    const uint32_t x = Uniform(surv[d * kWave + pos]);
                     if (lane == 0) {
                         lvl[d].pos = pos + 1;
                         matched[d] = x;
@@ -1360,7 +1367,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 WaveSync();
                 continue;
             }
-            if (ok) s_surv[wv][d][RankBelow(m)] = c;
+            if (ok) surv[d * kWave + RankBelow(m)] = c;
             if (lane == 0) {
                 lvl[d].off = off + kWave;
                 lvl[d].cnt = (uint32_t)__popcll(m);
@@ -1436,31 +1443,72 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
     KExtend<<<Grid1(n_in, kWpb, 65535), kThreads, 0, s>>>(T, in, n_in, depth, out, cap_rows, cnt);
 }
 
-uint32_t DfsBlocksPerCU() {
-    // Resident blocks of the count kernel per CU (registers bound it); the
-    // persistent grid is sized to exactly what can run at once.
-    static uint32_t cached = 0;
-    if (!cached) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KDfs<false>, kThreads, 0) != hipSuccess || nb <= 0)
-            nb = 4;
-        cached = (uint32_t)nb;
+// LDS per final-DFS block (static tables + survivor stacks + cache) that
+// still runs 3 blocks per CU.
+constexpr size_t kDfsBlockLds = 53248;
+
+// LDS plan of a final-DFS launch: the register budget fixes the resident
+// blocks per CU; the LDS those blocks leave after the static tables is split
+// among their waves into survivor stacks and the stable-intersection cache.
+template <bool kTuples>
+DfsShape DfsLdsShapeT(uint32_t surv_levels) {
+    static DfsShape cache[kMaxQV + 1] = {};
+    static std::mutex mu;  // queries launch from several host threads
+    std::lock_guard<std::mutex> lock(mu);
+    if (surv_levels > (uint32_t)kMaxQV) surv_levels = kMaxQV;
+    DfsShape& c = cache[surv_levels];
+    if (c.blocks_per_cu) return c;
+    int nb = 0;
+    const size_t surv_bytes = (size_t)kWpb * surv_levels * kWave * sizeof(uint32_t);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KDfs<kTuples>, kThreads, surv_bytes) != hipSuccess ||
+        nb <= 0)
+        throw HipError(hipErrorInvalidValue, "final-DFS occupancy query", __FILE__, __LINE__);
+    // The cache takes what is left of kDfsBlockLds per block.  The occupancy
+    // query accepted 54.2 KB blocks at 3 per CU, but those ran 25 % slower
+    // than 53.2 KB ones (measured), as if only 2 blocks fit: the budget is
+    // the largest block measured to run at full occupancy.
+    hipFuncAttributes attr{};
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(KDfs<kTuples>)) != hipSuccess)
+        throw HipError(hipErrorInvalidValue, "final-DFS attributes", __FILE__, __LINE__);
+    const size_t fixed = attr.sharedSizeBytes + surv_bytes;
+    uint32_t cap = kDfsBlockLds > fixed ? (uint32_t)((kDfsBlockLds - fixed) / (kWpb * sizeof(uint32_t))) : 0u;
+    cap = cap > 4096 ? 4096 : cap & ~63u;
+    for (;; cap = cap >= 64 ? cap - 64 : 0) {
+        const size_t dyn = surv_bytes + (size_t)kWpb * cap * sizeof(uint32_t);
+        int got = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&got, KDfs<kTuples>, kThreads, dyn) == hipSuccess &&
+            got >= nb) {
+            c.dyn_bytes = (uint32_t)dyn;
+            c.stab_cap = cap;
+            c.blocks_per_cu = (uint32_t)nb;
+            return c;
+        }
+        if (!cap) throw HipError(hipErrorInvalidValue, "final-DFS LDS plan", __FILE__, __LINE__);
     }
-    return cached;
 }
+
+DfsShape DfsLdsShape(uint32_t surv_levels, bool tuples) {
+    return tuples ? DfsLdsShapeT<true>(surv_levels) : DfsLdsShapeT<false>(surv_levels);
+}
+
+uint32_t DfsBlocksPerCU() { return DfsLdsShape(kMaxQV, false).blocks_per_cu; }
 
 void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>(cnt); }
 
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
-               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t grid_blocks, uint32_t grab,
-               hipStream_t s) {
+               const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t num_cus, uint32_t blocks_per_cu,
+               uint32_t grab, uint32_t surv_levels, hipStream_t s) {
     if (!n_rows || !njobs) return;
-    const unsigned g = Grid1(n_rows, kWpb, grid_blocks);
+    const DfsShape sh = DfsLdsShape(surv_levels, tuples != nullptr);
+    const uint32_t bpc = blocks_per_cu && blocks_per_cu < sh.blocks_per_cu ? blocks_per_cu : sh.blocks_per_cu;
+    const unsigned g = Grid1(n_rows, kWpb, num_cus * bpc);
     grab = grab < 1 ? 1 : grab;
     if (tuples)
-        KDfs<true><<<g, kThreads, 0, s>>>(jobs, njobs, row_base, grab, cnt, Q, tuples, tup_cap);
+        KDfs<true><<<g, kThreads, sh.dyn_bytes, s>>>(jobs, njobs, row_base, grab, cnt, Q, tuples, tup_cap,
+                                                      surv_levels, sh.stab_cap);
     else
-        KDfs<false><<<g, kThreads, 0, s>>>(jobs, njobs, row_base, grab, cnt, Q, nullptr, 0);
+        KDfs<false><<<g, kThreads, sh.dyn_bytes, s>>>(jobs, njobs, row_base, grab, cnt, Q, nullptr, 0, surv_levels,
+                                                       sh.stab_cap);
 }
 
 }  // namespace cgd

@@ -108,6 +108,8 @@ typedef struct {
                                      utilisation = step_lanes / (64 * steps)      */
     uint64_t last_dfs_batched;    /* steps whose product suffixes were counted
                                      for all their survivors at once              */
+    uint64_t last_dfs_lds_cache;  /* u32 per wave of the stable-intersection
+                                     cache in the last final-DFS launch           */
     double   ms_apply, ms_update_index, ms_masks, ms_match, ms_commit; /* last batch, device time */
     double   last_dfs_busy;       /* share of final-DFS wave time spent working   */
 } cgbdsm_stats;
