@@ -880,10 +880,10 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
         const uint32_t len = Ls.bl + Ls.al;
         const bool ones = (X.ones >> qj) & 1u;
         uint32_t raw;
+        bool use_sc = false;  // S = j's stable lists and mask, cached in LDS
         if (nbn == 1 && ones) {
             raw = len;
         } else {
-            bool use_sc = false;
             if (nbn > 1 && sc.pos == j && sc.vk == kp) {
                 if (sc.state == 0) StabBuild(X, T, j, matched, sc, scanned, probes);
                 use_sc = sc.state == 1;
@@ -968,12 +968,17 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
         // Positions before the suffix with j's label class that lie in C_j.
         uint32_t hits = 0;
         if (prod && raw) {
+            // The sibling-independent part first (from the cached S when
+            // valid), the sibling's own list last.
             auto member = [&](uint32_t y) -> bool {
-                if (!TileBit(X, T, qj, y)) return false;
-                if (!InList(Ls, y, probes)) return false;
-                for (uint32_t k = 0; k < nbn; k++)
-                    if (k != kp && !InList(BnList(X, T, j, k, matched), y, probes)) return false;
-                return true;
+                if (use_sc) {
+                    if (!StabContains(sc, y)) return false;
+                } else {
+                    if (!TileBit(X, T, qj, y)) return false;
+                    for (uint32_t k = 0; k < nbn; k++)
+                        if (k != kp && !InList(BnList(X, T, j, k, matched), y, probes)) return false;
+                }
+                return InList(Ls, y, probes);
             };
             for (uint32_t i = 0; i < p; i++)
                 if (Cls(X, i) == cls && member(matched[i])) hits++;
