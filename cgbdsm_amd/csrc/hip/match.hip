@@ -851,16 +851,18 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
     // Cheap positions first (sizes independent of the sibling, or a single
     // list with no mask to apply): a sibling they leave empty adds no
     // entries to the costly intersections that follow.
-    uint32_t cheap = 0;
+    // Three tiers: sibling-independent or a bare single list; a single list
+    // filtered by its mask; intersections of several lists.
+    uint32_t tier[3] = {0, 0, 0};
     for (uint32_t j = p + 1; j < qv; j++) {
         const uint32_t nbn = Nbn(X, j);
         bool varying = false;
         for (uint32_t k = 0; k < nbn; k++) varying |= (Pair(X, j, k) >> 8) == p;
-        if (!varying || (nbn == 1 && ((X.ones >> Vs(X, j)) & 1u))) cheap |= 1u << j;
+        const uint32_t t = !varying || (nbn == 1 && ((X.ones >> Vs(X, j)) & 1u)) ? 0u : nbn == 1 ? 1u : 2u;
+        tier[t] |= 1u << j;
     }
-    const uint32_t tail = ((1u << qv) - 1u) & ~((1u << (p + 1)) - 1u);
-    for (uint32_t pass = 0, todo = cheap; pass < 2; pass++, todo = tail & ~cheap)
-    for (uint32_t jm = todo; jm; jm &= jm - 1) {
+    for (uint32_t pass = 0; pass < 3; pass++)
+    for (uint32_t jm = tier[pass]; jm; jm &= jm - 1) {
         const uint32_t j = (uint32_t)__builtin_ctz(jm);
         if (!__ballot(prod != 0)) break;
         const uint32_t nbn = Nbn(X, j);
