@@ -1463,9 +1463,11 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
     KExtend<<<Grid1(n_in, kWpb, 65535), kThreads, 0, s>>>(T, in, n_in, depth, out, cap_rows, cnt);
 }
 
-// LDS per final-DFS block (static tables + survivor stacks + cache) that
-// still runs 3 blocks per CU.
-constexpr size_t kDfsBlockLds = 53248;
+// LDS per final-DFS block (static tables + survivor stacks + cache) at nb
+// blocks per CU: the CU's 160 KB less 4 KB, split evenly, in 1 KB steps
+// (53 248 B at 3 blocks per CU, the largest block measured to keep full
+// occupancy there).
+constexpr size_t DfsBlockLds(int nb) { return ((size_t)(160 * 1024 - 4096) / (size_t)nb) & ~(size_t)1023; }
 
 // LDS plan of a final-DFS launch: the register budget fixes the resident
 // blocks per CU; the LDS those blocks leave after the static tables is split
@@ -1483,7 +1485,7 @@ DfsShape DfsLdsShapeT(uint32_t surv_levels) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KDfs<kTuples>, kThreads, surv_bytes) != hipSuccess ||
         nb <= 0)
         throw HipError(hipErrorInvalidValue, "final-DFS occupancy query", __FILE__, __LINE__);
-    // The cache takes what is left of kDfsBlockLds per block.  The occupancy
+    // The cache takes what is left of DfsBlockLds per block.  The occupancy
     // query accepted 54.2 KB blocks at 3 per CU, but those ran 25 % slower
     // than 53.2 KB ones (measured), as if only 2 blocks fit: the budget is
     // the largest block measured to run at full occupancy.
@@ -1491,7 +1493,8 @@ DfsShape DfsLdsShapeT(uint32_t surv_levels) {
     if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(KDfs<kTuples>)) != hipSuccess)
         throw HipError(hipErrorInvalidValue, "final-DFS attributes", __FILE__, __LINE__);
     const size_t fixed = attr.sharedSizeBytes + surv_bytes;
-    uint32_t cap = kDfsBlockLds > fixed ? (uint32_t)((kDfsBlockLds - fixed) / (kWpb * sizeof(uint32_t))) : 0u;
+    const size_t budget = DfsBlockLds(nb);
+    uint32_t cap = budget > fixed ? (uint32_t)((budget - fixed) / (kWpb * sizeof(uint32_t))) : 0u;
     cap = cap > 4096 ? 4096 : cap & ~63u;
     for (;; cap = cap >= 64 ? cap - 64 : 0) {
         const size_t dyn = surv_bytes + (size_t)kWpb * cap * sizeof(uint32_t);
