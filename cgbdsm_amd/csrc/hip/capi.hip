@@ -63,9 +63,11 @@ struct cgbdsm_query {
     uint32_t donate_min = 4;
     bool dfs_shortest = true, dfs_suffix = true;
     // Final-DFS launch shape: frontier rows taken per cursor atomic, and
-    // resident blocks per CU (0: as many as fit).  4 rows measured best on
-    // dense 6-vertex streams (larger grabs leave a tail that stealing has to
-    // repair); cgbdsm_set_dfs_tuning changes both.
+    // resident blocks per CU (0: as many as fit).  The autotuner's choice for
+    // rows varies with the workload (1 on the scale-22 mix, 2 on the dense
+    // 6-vertex scale-20 stream, 4-8 on the scale-20 mix); the default 4 is
+    // the best untuned value on the mix (2 measured 8 % slower there, 8 %
+    // faster on the dense stream); cgbdsm_set_dfs_tuning changes both.
     uint32_t dfs_grab = 4, dfs_bpc = 0;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;

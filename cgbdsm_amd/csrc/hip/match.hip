@@ -27,7 +27,7 @@
 
 // Register budget of the final DFS (occupancy against spills), a build-time
 // tuning knob: -DCGD_DFS_WAVES_PER_EU=N asks for N resident waves per SIMD
-// (3: about 168 VGPRs with a handful of spills; 0: no cap).
+// (3: 168 VGPRs with some spills, measured best; 2 and 4 are slower; 0: no cap).
 #ifndef CGD_DFS_WAVES_PER_EU
 #define CGD_DFS_WAVES_PER_EU 3
 #endif
