@@ -779,11 +779,19 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
     const uint32_t qv = X.qv;
     uint32_t done = 0;
     unsigned long long result = 1;
-    for (uint32_t j = j0; j < qv; j++) {
-        if ((done >> j) & 1u) continue;
+    // Classes whose sizes are all cached first: an empty one ends the count
+    // before any list is read.
+    // (A class is found at its first member; `seen` keeps a class skipped in
+    // one pass from being found again at its later members.)
+    const uint32_t cv0 = cvalid;
+    for (uint32_t pass = 0; pass < 2; pass++)
+    for (uint32_t j = j0, seen = done; j < qv; j++) {
+        if ((seen >> j) & 1u) continue;
         uint32_t members = 0;
         for (uint32_t i = j; i < qv; i++)
             if (Cls(X, i) == Cls(X, j)) members |= 1u << i;
+        seen |= members;
+        if ((pass == 0) != ((members & ~cv0) == 0)) continue;
         done |= members;
         const uint32_t p0 = j;
         const uint32_t r1 = members & ~(1u << p0);
