@@ -1235,7 +1235,12 @@ const Frontier& BuildFrontier(cgbdsm_query* q) {
         cur = 1;
     }
     // Step 2: BFS levels while the frontier is small.
-    while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv) {
+    // With a batched product suffix the DFS must still scan the position
+    // before the suffix (cp_start - 1) itself: BFS levels stop short of it.
+    const bool batched_suffix =
+        q->hplan.sfx_batch && q->dfs_suffix && q->h_cnt->u_dups == 0 && q->hplan.cp_start < q->qv;
+    while (n > 0 && n < q->bfs_row_limit && depth + 1 < q->qv &&
+           !(batched_suffix && depth + 2 > q->hplan.cp_start)) {
         const uint64_t want_rows =
             fixed_cap ? q->frontier_cap_rows : std::min<uint64_t>(std::max<uint64_t>(n * 64, 1u << 20), 1ull << 28);
         const uint64_t want_words = want_rows * (depth + 1);
