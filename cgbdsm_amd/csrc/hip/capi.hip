@@ -1209,28 +1209,38 @@ const Frontier& BuildFrontier(cgbdsm_query* q) {
     const bool fixed_cap = q->frontier_cap_rows != 0;
     uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
     Ensure(q, q->front[0], cap_rows * 2);
+    // Product suffix from position 2 (counted for all siblings at once):
+    // depth-1 rows, so that the DFS scans position 1 itself and the sizes
+    // that depend on v0 alone are computed once per v0, not once per (v0, v1).
+    bool seed1 = false;
+    if (q->hplan.sfx_batch && q->dfs_suffix && q->hplan.cp_start == 2 && q->qv > 2) {
+        ReadCounters(q);
+        seed1 = q->h_cnt->u_dups == 0;
+    }
+    const uint32_t words = seed1 ? 1u : 2u;
     // Step 1: seeds (retry once with a buffer of the exact size on overflow).
     uint64_t n = 0;
     for (int attempt = 0;; attempt++) {
         const int ks = KBegin(q, CGBDSM_K_SEED);
-        LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
+        if (seed1) LaunchSeed1(T, L, q->front[0].p, q->front[0].cap, q->d_cnt, q->s);
+        else LaunchSeed(T, L, q->front[0].p, q->front[0].cap / 2, q->d_cnt, q->s);
         KEnd(q, ks);
         ReadCounters(q);
         n = q->h_cnt->rows_out;
         if (!(q->h_cnt->flags & kFlagFrontier)) break;
         if (attempt) Throw(CGBDSM_ERR_OVERFLOW, "seed frontier overflow");
-        Ensure(q, q->front[0], n * 2 + 64);
+        Ensure(q, q->front[0], n * words + 64);
         SetCounter(q, &Counters::rows_out, 0);
         SetCounter(q, &Counters::flags, 0);
     }
-    uint32_t depth = 2, cur = 0;
+    uint32_t depth = seed1 ? 1u : 2u, cur = 0;
     // Seeds come out grouped by owner; group them by edge tag.
     if (n > 1 && q->qe > 1) {
-        Ensure(q, q->front[1], n * 2);
+        Ensure(q, q->front[1], n * words);
         Ensure(q, q->tag_hist, 2 * kMaxQE);
         FillU64((uint64_t*)q->tag_hist.p, 0, 2 * kMaxQE, q->s);
         const int kg = KBegin(q, CGBDSM_K_SEED);
-        LaunchGroupByTag(q->front[0].p, n, q->hplan.tag_shift, q->tag_hist.p, q->front[1].p, q->s);
+        LaunchGroupByTag(q->front[0].p, n, words, q->hplan.tag_shift, q->tag_hist.p, q->front[1].p, q->s);
         KEnd(q, kg);
         cur = 1;
     }

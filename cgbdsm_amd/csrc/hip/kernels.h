@@ -50,10 +50,12 @@ void LaunchMaskDepth(const Tables& T, uint32_t d, const EiList& L, Counters* cnt
 
 // match.hip
 void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);
-// Depth-2 rows (2 words, word 0 = tag << tag_shift | v0) of `in` copied to
-// `out` grouped by tag; hist = 2 * kMaxQE zeroed counters (scratch).
-void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t tag_shift, unsigned long long* hist, uint32_t* out,
-                      hipStream_t s);
+// Rows of `words` u32 (1 or 2; word 0 = tag << tag_shift | v0) of `in`
+// copied to `out` grouped by tag; hist = 2 * kMaxQE zeroed counters.
+void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t words, uint32_t tag_shift, unsigned long long* hist,
+                      uint32_t* out, hipStream_t s);
+// Depth-1 rows [(ei << tag_shift) | v0] (one word each).
+void LaunchSeed1(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s);
 void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out,
                   uint64_t cap_rows, Counters* cnt, hipStream_t s);
 // One query's share of a final-DFS launch: its index tables, its frontier

@@ -363,17 +363,74 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
     }
 }
 
+// Depth-1 rows [(ei << tag_shift) | v0] for every active edge tag whose
+// support mask holds v0 and whose first-pair list of v0 is not empty: the
+// frontier when the product suffix starts at position 2, so that the final
+// DFS scans position 1 itself and counts the suffixes of v0's candidates
+// together.  Same visibility as KSeed (own edge: U; later tags: M and U;
+// earlier tags: M).
+__global__ void __launch_bounds__(kThreads) KSeed1(Tables T, EiList EL, uint32_t* __restrict__ rows,
+                                                   uint64_t cap_rows, Counters* __restrict__ cnt) {
+    __shared__ uint32_t s_tag[kMaxQE];
+    const cgbdsm_plan& p = T.P->p;
+    const uint32_t q0 = p.go_vs[0], q1 = p.go_vs[1];
+    const uint32_t idx = p.eidx[q0 * kMaxQV + q1];
+    const uint32_t own = p.dir_to_edge[idx];
+    const uint32_t n = EL.n;
+    const uint32_t lane = LaneId();
+    uint64_t ones0 = 0, use_m = 0, use_u = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t ei = EL.ei[k];
+        if (T.P->all_ones[ei][q0]) ones0 |= 1ull << k;
+        if (ei != own) use_m |= 1ull << k;
+        if (ei >= own) use_u |= 1ull << k;
+    }
+    const uint32_t tag_shift = T.P->tag_shift;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s_tag[k] = (uint32_t)EL.ei[k] << tag_shift;
+    __syncthreads();
+    const uint64_t nw = (uint64_t)gridDim.x * kWpb;
+    for (uint64_t w = (uint64_t)blockIdx.x * kWpb + threadIdx.x / kWave; w < T.W; w += nw) {
+        const uint32_t v = (uint32_t)(w * kWave + lane);
+        uint64_t m0 = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint64_t word =
+                ((ones0 >> k) & 1ull) ? ~0ull : T.masks[((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w];
+            m0 |= ((word >> lane) & 1ull) << k;
+        }
+        if (v >= T.dv) m0 = 0;
+        if (!__ballot(m0 != 0)) continue;
+        const uint64_t r = (uint64_t)v * T.ndir + idx;
+        uint64_t emit = 0;
+        if ((m0 & use_m) && T.m_rec[r].len) emit |= m0 & use_m;
+        if ((m0 & use_u) && T.u_rec[r].len) emit |= m0 & use_u;
+        const uint32_t nr = (uint32_t)__popcll(emit);
+        const uint32_t incl = WaveInclusiveScan(nr);
+        const uint32_t tot = Uniform(__shfl(incl, kWave - 1, kWave));
+        if (!tot) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)tot);
+        base = Uniform64(__shfl(base, 0, kWave));
+        if (base + tot > cap_rows) {
+            if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
+            continue;
+        }
+        uint64_t pos = base + incl - nr;
+        for (; emit; emit &= emit - 1, pos++) rows[pos] = s_tag[__builtin_ctzll(emit)] | v;
+    }
+}
+
 // Counting sort of depth-2 rows by edge tag (KSeed emits the tags of one
 // owner together; the final DFS runs faster over tag-grouped rows, whose
 // waves then share one tag's support masks in L2).  Order inside a tag is
 // arbitrary.  hist[0..kMaxQE) = rows per tag, hist[kMaxQE..) = cursors.
+template <uint32_t W>
 __global__ void __launch_bounds__(kThreads) KTagHist(const uint32_t* __restrict__ in, uint64_t n, uint32_t tag_shift,
                                                      uint64_t per_block, unsigned long long* __restrict__ hist) {
     __shared__ uint32_t h[kMaxQE];
     for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) h[t] = 0;
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * per_block, hi = lo + per_block < n ? lo + per_block : n;
-    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[2 * r] >> tag_shift], 1u);
+    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[W * r] >> tag_shift], 1u);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x)
         if (h[t]) atomicAdd(&hist[t], (unsigned long long)h[t]);
@@ -386,6 +443,7 @@ __global__ void __launch_bounds__(kWave) KTagScan(unsigned long long* __restrict
     if (lane < (uint32_t)kMaxQE) hist[kMaxQE + lane] = incl - c;
 }
 
+template <uint32_t W>
 __global__ void __launch_bounds__(kThreads) KTagScatter(const uint32_t* __restrict__ in, uint64_t n, uint32_t tag_shift,
                                                         uint64_t per_block, unsigned long long* __restrict__ hist,
                                                         uint32_t* __restrict__ out) {
@@ -394,7 +452,7 @@ __global__ void __launch_bounds__(kThreads) KTagScatter(const uint32_t* __restri
     for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) h[t] = 0;
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * per_block, hi = lo + per_block < n ? lo + per_block : n;
-    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[2 * r] >> tag_shift], 1u);
+    for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) atomicAdd(&h[in[W * r] >> tag_shift], 1u);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < (uint32_t)kMaxQE; t += blockDim.x) {
         base[t] = h[t] ? atomicAdd(&hist[kMaxQE + t], (unsigned long long)h[t]) : 0ull;
@@ -402,11 +460,11 @@ __global__ void __launch_bounds__(kThreads) KTagScatter(const uint32_t* __restri
     }
     __syncthreads();
     for (uint64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-        const uint32_t w0 = in[2 * r], w1 = in[2 * r + 1];
+        const uint32_t w0 = in[W * r];
         const uint32_t k = w0 >> tag_shift;
         const uint64_t pos = base[k] + atomicAdd(&h[k], 1u);
-        out[2 * pos] = w0;
-        out[2 * pos + 1] = w1;
+        out[W * pos] = w0;
+        if (W > 1) out[W * pos + 1] = in[W * r + 1];
     }
 }
 
@@ -1455,14 +1513,25 @@ void LaunchSeed(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_r
     KSeed<<<Grid1(T.W, kWpb, 8192), kThreads, 0, s>>>(T, L, rows, cap_rows, cnt);
 }
 
-void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t tag_shift, unsigned long long* hist, uint32_t* out,
-                      hipStream_t s) {
+void LaunchGroupByTag(const uint32_t* in, uint64_t n, uint32_t words, uint32_t tag_shift, unsigned long long* hist,
+                      uint32_t* out, hipStream_t s) {
     if (!n) return;
     const uint64_t per_block = 4096;
     const unsigned g = (unsigned)((n + per_block - 1) / per_block);
-    KTagHist<<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist);
-    KTagScan<<<1, kWave, 0, s>>>(hist);
-    KTagScatter<<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist, out);
+    if (words == 1) {
+        KTagHist<1><<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist);
+        KTagScan<<<1, kWave, 0, s>>>(hist);
+        KTagScatter<1><<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist, out);
+    } else {
+        KTagHist<2><<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist);
+        KTagScan<<<1, kWave, 0, s>>>(hist);
+        KTagScatter<2><<<g, kThreads, 0, s>>>(in, n, tag_shift, per_block, hist, out);
+    }
+}
+
+void LaunchSeed1(const Tables& T, const EiList& L, uint32_t* rows, uint64_t cap_rows, Counters* cnt, hipStream_t s) {
+    if (!L.n || !T.W) return;
+    KSeed1<<<Grid1(T.W, kWpb, 8192), kThreads, 0, s>>>(T, L, rows, cap_rows, cnt);
 }
 
 void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t depth, uint32_t* out, uint64_t cap_rows,

@@ -151,6 +151,10 @@ SUFFIX_STREAMS = [
     ("sfx-s12-l4-dense6", 12, 8, 4, 6, "dense", 22, 400, None),
     ("sfx-s12-l4-sparse6", 12, 8, 4, 6, "sparse", 21, 400, None),
     ("sfx-s12-l4-tree6", 12, 8, 4, 6, "tree", 20, 400, None),
+    # product suffix from position 2: depth-1 frontier rows
+    ("sfx-s11-l6-tree4-a", 11, 8, 6, 4, "tree", 31, 300, None),
+    ("sfx-s11-l6-tree4-b", 11, 8, 6, 4, "tree", 33, 300, None),
+    ("sfx-s11-l6-tree5", 11, 8, 6, 5, "tree", 35, 300, None),
 ]
 
 
@@ -168,7 +172,7 @@ def test_batched_suffix_parity(eng, case):
     assert res["batches"] > 0
     assert res["mismatches"] == 0
     if name in ("sfx-s12-l2-dense5", "sfx-s13-l2-tree5", "sfx-s12-l4-dense6", "sfx-s12-l4-sparse6",
-                "sfx-s12-l4-tree6"):
+                "sfx-s12-l4-tree6", "sfx-s11-l6-tree4-a", "sfx-s11-l6-tree4-b", "sfx-s11-l6-tree5"):
         assert res["dfs_batched"] > 0   # every class of the suffix has one member
 
 
