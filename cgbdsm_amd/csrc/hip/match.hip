@@ -7,12 +7,18 @@
 //            wavefront per 64 owners reads the shared first-pair lists once,
 //            spreads their entries over lanes; scan + one atomic per 64
 //            entries reserves rows.
+//   KSeed1   depth-1 rows [(ei << tag_shift) | v0] instead, when the product
+//            suffix starts at position 2.
+//   KTag*    counting sort of the seed rows by edge tag.
 //   KExtend  one BFS level, one wavefront per row, 64 candidates per step
 //            (extendBFSAllBit with write_res, kernels/enumeration.cu:811-1155).
-//   KDfs     the final count: persistent wavefronts pull rows from a global
-//            atomic queue; each runs an iterative DFS whose per-level survivor
-//            stacks live in LDS (64 u32 per level).  At the last level the
-//            survivors are only counted (popcount of the ballot).
+//   KDfs     the final count: persistent wavefronts pull rows (of one or
+//            several queries) from a global atomic queue; each runs an
+//            iterative DFS whose per-level survivor stacks live in LDS (64 u32
+//            per level).  At the last level the survivors are only counted
+//            (popcount of the ballot); the product suffix of the order is
+//            counted in closed form, for a whole step of siblings at once
+//            when its label classes have one member each (BatchSuffix).
 //
 // Candidate rule at depth d (global order position d, query vertex u_d):
 //   list of the first backward neighbour, in its M list or (for ei's own two
