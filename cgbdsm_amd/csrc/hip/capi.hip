@@ -82,6 +82,9 @@ struct cgbdsm_query {
     // first count until the next build_local_index).
     Frontier fr;
     bool fr_ready = false;
+    // Recorded on this query's stream when its frontier is complete: a
+    // batched launch on another query's stream waits for it.
+    hipEvent_t fr_ev = nullptr;
     uint64_t last_rows = 0;
     cgbdsm_stats st{};
     hipEvent_t ev[8] = {};
@@ -714,6 +717,7 @@ cgbdsm_status cgbdsm_query_create(cgbdsm_ctx* ctx, const cgbdsm_plan* plan, cgbd
             q->umask = (1u << (32 - plan->ebits)) - 1u;
             CGD_HIP(hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
             for (auto& e : q->ev) CGD_HIP(hipEventCreate(&e));
+            CGD_HIP(hipEventCreateWithFlags(&q->fr_ev, hipEventDisableTiming));
             q->d_plan = (DevPlan*)DevAlloc(ctx, sizeof(DevPlan));
             q->d_cnt = (Counters*)DevAlloc(ctx, sizeof(Counters));
             CGD_HIP(hipHostMalloc((void**)&q->h_cnt, sizeof(Counters), hipHostMallocDefault));
@@ -777,6 +781,7 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
         if (q->steal_seq) DevFree(q->ctx, q->steal_seq, q->steal_cap * sizeof(unsigned long long));
         for (auto& e : q->ev)
             if (e) (void)hipEventDestroy(e);
+        if (q->fr_ev) (void)hipEventDestroy(q->fr_ev);
         for (auto& k : q->kpend) {
             (void)hipEventDestroy(k.a);
             (void)hipEventDestroy(k.b);
@@ -1199,7 +1204,9 @@ cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
 
 // Seeds and BFS levels of one query's matching (phase 4): the rows left for
 // the final DFS, their depth and the buffer holding them, kept in q->fr.
-// Reads the counters back, so q's stream is idle on return.
+// The last kernel (the tag grouping) may still run on q's stream on return;
+// q->fr_ev marks the frontier's completion there, for a batched DFS launch
+// on another query's stream to wait on.
 const Frontier& BuildFrontier(cgbdsm_query* q) {
     if (q->fr_ready) return q->fr;
     const EiList L = ActiveEdges(q);
@@ -1268,6 +1275,7 @@ const Frontier& BuildFrontier(cgbdsm_query* q) {
         cur ^= 1;
         depth++;
     }
+    CGD_HIP(hipEventRecord(q->fr_ev, q->s));
     q->fr = Frontier{n, depth, cur};
     q->fr_ready = true;
     return q->fr;
@@ -1358,10 +1366,10 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
         std::vector<Frontier> fr(nq);
         for (int i = 0; i < nq; i++) fr[i] = BuildFrontier(qs[i]);
         // One final-DFS launch for all of them, on the first query's stream
-        // and work queue.  Every frontier is complete (BuildFrontier read its
-        // counters back), so the other streams hold no pending work the
-        // launch depends on.
+        // and work queue, after every other query's frontier is complete on
+        // its own stream (the tag grouping may still be running there).
         cgbdsm_query* lead = qs[0];
+        for (int i = 1; i < nq; i++) CGD_HIP(hipStreamWaitEvent(lead->s, qs[i]->fr_ev, 0));
         lead->h_jobs.clear();
         lead->h_rowbase.assign(1, 0ull);
         for (int i = 0; i < nq; i++) {
@@ -1369,8 +1377,8 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
             lead->h_jobs.push_back(MakeJob(qs[i], qs[i]->front[fr[i].cur].p, fr[i].depth));
             lead->h_rowbase.push_back(lead->h_rowbase.back() + fr[i].n);
         }
-        // Per-query counts are zeroed on the launch's stream; every other
-        // query's stream is idle (BuildFrontier ended by reading its counters).
+        // Per-query counts are zeroed on the launch's stream (the grouping
+        // kernels still in flight on the other streams do not touch them).
         for (int i = 0; i < nq; i++) ResetMatchCounters(qs[i], lead->s);
         ResetDfsCounters(lead);
         RunDfsJobs(lead, nullptr, 0);

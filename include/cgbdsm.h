@@ -212,7 +212,8 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
  * leaving the frontier for the final DFS of the next cgbdsm_matching or
  * cgbdsm_matching_batch call.  Lets a serving loop build the frontiers of
  * several queries concurrently (one host thread per query) before one
- * batched DFS launch.  Returns with the query's stream idle. */
+ * batched DFS launch.  Its last kernels may still run on the query's
+ * stream on return; cgbdsm_matching_batch orders its launch after them. */
 cgbdsm_status cgbdsm_build_frontier(cgbdsm_query* q);
 
 /* cgbdsm_matching for nq registered queries of one context at once: each

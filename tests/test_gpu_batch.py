@@ -180,3 +180,63 @@ def test_dfs_tuning_keeps_counts(eng):
         sess[0][1].set_dfs_tuning(2048, 0)
     for _, q, _, _ in sess:
         q.close()
+
+
+def test_two_batched_launches_at_once(eng):
+    """Two groups of queries, each counted by its own batched launch from its
+    own host thread, so that one group's index maintenance and frontier
+    builds overlap the other group's launch: every count equals the count of
+    the same query run alone.  (A batched launch runs on its first query's
+    stream and reads the other queries' frontiers, so a frontier build must
+    return with its own stream idle.)"""
+    import threading
+
+    print("case two batched launches at once", flush=True)
+    groups = [MIX[:3], MIX[3:]]
+    sess = [_open(eng, g) for g in groups]
+    nb = min(h.num_batches for g in sess for h, _, _, _ in g)
+    assert nb >= 3
+    # Stage 1 on this thread, before the device work starts
+    kbs = [[[h.next_batch() for _ in range(nb)] for h, _, _, _ in g] for g in sess]
+    got = [[[0] * len(g) for _ in range(nb)] for g in groups]
+    errors = []
+
+    def run(gi):
+        try:
+            qs = [q for _, q, _, _ in sess[gi]]
+            for b in range(nb):
+                live = []
+                for k, q in enumerate(qs):
+                    q.stage(kbs[gi][k][b])
+                    q.transfer_update_all()
+                    if q.transfer_update_index():
+                        q.build_local_index()
+                        q.build_frontier()
+                        live.append(k)
+                if live:
+                    for k, (c, _) in zip(live, eng.matching_batch([qs[k] for k in live])):
+                        got[gi][b][k] = c
+                for k in live:
+                    qs[k].batch_update_global_index()
+                for q in qs:
+                    q.sync()
+        except BaseException as e:  # raised on the test thread below
+            errors.append(e)
+
+    th = [threading.Thread(target=run, args=(gi,)) for gi in range(len(groups))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for g in sess:
+        for _, q, _, _ in g:
+            q.close()
+    assert not errors, errors[0]
+    for gi, g in enumerate(groups):
+        for k, (host, q, _, _) in enumerate(_open(eng, g)):
+            for b in range(nb):
+                q.stage(kbs[gi][k][b])
+                c, _, _ = q.run_batch()
+                assert c == got[gi][b][k], (gi, k, b, c, got[gi][b][k])
+            q.close()
+    print(f"groups {groups}: {nb} batches, counts equal alone", flush=True)
