@@ -155,6 +155,10 @@ SUFFIX_STREAMS = [
     ("sfx-s11-l6-tree4-a", 11, 8, 6, 4, "tree", 31, 300, None),
     ("sfx-s11-l6-tree4-b", 11, 8, 6, 4, "tree", 33, 300, None),
     ("sfx-s11-l6-tree5", 11, 8, 6, 5, "tree", 35, 300, None),
+    # few labels: suffix classes of two members
+    ("sfx-s12-l2-tree6", 12, 8, 2, 6, "tree", 7, 400, None),
+    ("sfx-s12-l3-sparse6", 12, 8, 3, 6, "sparse", 8, 400, None),
+    ("sfx-s12-l3-dense5", 12, 8, 3, 5, "dense", 9, 400, None),
 ]
 
 
