@@ -1382,9 +1382,13 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
         for (int i = 0; i < nq; i++) ResetMatchCounters(qs[i], lead->s);
         ResetDfsCounters(lead);
         RunDfsJobs(lead, nullptr, 0);
+        // The other queries' counters come back on the launch's stream, with
+        // the lead's (CheckDfs): one wait instead of one per query.
+        for (int i = 0; i < nq; i++)
+            if (qs[i] != lead)
+                CGD_HIP(hipMemcpyAsync(qs[i]->h_cnt, qs[i]->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, lead->s));
         CheckDfs(lead, lead->h_rowbase.back());
         for (int i = 0; i < nq; i++) {
-            if (qs[i] != lead) ReadCounters(qs[i]);
             FinishMatching(qs[i], fr[i], lead, num_matches ? &num_matches[i] : nullptr,
                            per_edge ? per_edge[i] : nullptr);
         }
