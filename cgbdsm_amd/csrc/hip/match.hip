@@ -569,7 +569,7 @@ __device__ __forceinline__ List BnList(const Ctx& X, const Tables& T, uint32_t d
 }
 
 // Occurrences test by binary search in both parts of a list (probes counted).
-__device__ __forceinline__ bool InList(const List& L, uint32_t c, unsigned long long& probes) {
+__device__ __forceinline__ bool InList(const List& L, uint32_t c, uint32_t& probes) {
     uint32_t lo = 0, hi = L.bl;
     while (lo < hi) {
         const uint32_t m = (lo + hi) >> 1;
@@ -600,7 +600,7 @@ __device__ __forceinline__ bool InList(const List& L, uint32_t c, unsigned long 
 // Candidate c for position d: support-mask bit, distinct from positions
 // 0..ninj-1, present in the list of every backward neighbour except `skip`.
 __device__ __forceinline__ bool AcceptSel(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched, uint32_t c,
-                                          uint32_t skip, uint32_t ninj, unsigned long long& probes) {
+                                          uint32_t skip, uint32_t ninj, uint32_t& probes) {
     if (!TileBit(X, T, Vs(X, d), c)) return false;
     for (uint32_t i = 0; i < ninj; i++)
         if (matched[i] == c) return false;
@@ -670,7 +670,7 @@ __device__ __forceinline__ bool StabContains(const StabCache& sc, uint32_t c) {
 // the shortest of those lists and checks the others; state 2 when the
 // result does not fit the buffer.
 __device__ __forceinline__ void StabBuild(const Ctx& X, const Tables& T, uint32_t jf, const uint32_t* matched,
-                                          StabCache& sc, unsigned long long& scanned, unsigned long long& probes) {
+                                          StabCache& sc, uint32_t& scanned, uint32_t& probes) {
     const uint32_t lane = LaneId();
     const uint32_t vk = sc.vk;
     const uint32_t nbn = Nbn(X, jf);
@@ -721,8 +721,8 @@ __device__ __forceinline__ void StabBuild(const Ctx& X, const Tables& T, uint32_
 // positions 0..j0-1 in; `hits` counts those of them that lie in C_B (only
 // positions of the same label class can).  All lists must be sets.
 __device__ __forceinline__ void SetSize(const Ctx& X, const Tables& T, uint32_t B, uint32_t j0, const uint32_t* matched, bool need_raw,
-                        uint32_t& raw, uint32_t& hits, StabCache& sc, unsigned long long& scanned,
-                        unsigned long long& probes) {
+                        uint32_t& raw, uint32_t& hits, StabCache& sc, uint32_t& scanned,
+                        uint32_t& probes) {
     const uint32_t lane = LaneId();
     const uint32_t jf = (uint32_t)__builtin_ctz(B);
     const uint32_t cls = Cls(X, jf);
@@ -839,7 +839,7 @@ __device__ __forceinline__ void SetSize(const Ctx& X, const Tables& T, uint32_t 
 // until a position they depend on changes.
 __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_t j0, const uint32_t* matched,
                                           uint32_t* csize, uint32_t& cvalid, StabCache& sc,
-                                          unsigned long long& scanned, unsigned long long& probes) {
+                                          uint32_t& scanned, uint32_t& probes) {
     const uint32_t qv = X.qv;
     uint32_t done = 0;
     unsigned long long result = 1;
@@ -915,7 +915,7 @@ struct BxLds {
 // sibling in LDS.  Equal to SuffixCount per sibling (identical counts).
 __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_t p, const uint32_t* matched,
                                           uint64_t m, uint32_t x, uint32_t* csize, uint32_t& cvalid, StabCache& sc,
-                                          BxLds& B, unsigned long long& scanned, unsigned long long& probes) {
+                                          BxLds& B, uint32_t& scanned, uint32_t& probes) {
     const uint32_t lane = LaneId();
     const uint32_t qv = X.qv;
     const uint32_t pcls = Cls(X, p);
@@ -1161,7 +1161,19 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         bsfx = cart && T.P->sfx_batch;
     };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
-    unsigned long long my_scanned = 0, my_probes = 0, donated = 0, steps = 0, step_lanes = 0, batched = 0;
+    // Candidates scanned and binary-search probes: 32-bit per lane (two
+    // fewer registers each), folded into the wave's 64-bit totals at every
+    // wall-clock poll (a lane cannot reach 2^32 within one poll period) and
+    // at exit.
+    uint32_t my_scanned = 0, my_probes = 0;
+    unsigned long long w_scanned = 0, w_probes = 0;
+    auto fold = [&]() {
+        w_scanned += Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_scanned), kWave - 1, kWave));
+        w_probes += Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_probes), kWave - 1, kWave));
+        my_scanned = 0;
+        my_probes = 0;
+    };
+    unsigned long long donated = 0, steps = 0, step_lanes = 0, batched = 0;
     uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
     bool primary = true, idle = false, stop = false, have_ticket = false, from_row = false;
     uint32_t iter = 0, backoff = 1;
@@ -1178,6 +1190,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         const long long now = (long long)Uniform64((unsigned long long)wall_clock64());
         if (now - t_poll < Q.poll_ticks) return false;
         t_poll = now;
+        if (__ballot(my_scanned >= (1u << 30) || my_probes >= (1u << 30))) fold();
         const bool host = now - t_host >= Q.host_ticks;
         if (host) t_host = now;
         unsigned long long idl = 0, fl = 0;
@@ -1483,11 +1496,10 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             atomicAdd(&cnt->idle_ticks, (unsigned long long)idle_sum);
         }
     }
-    const unsigned long long s_tot = __shfl(WaveInclusiveScan(my_scanned), 63, kWave);
-    const unsigned long long p_tot = __shfl(WaveInclusiveScan(my_probes), 63, kWave);
+    fold();
     if (lane == 0) {
-        atomicAdd(&cnt->scanned, s_tot);
-        atomicAdd(&cnt->probes, p_tot);
+        atomicAdd(&cnt->scanned, w_scanned);
+        atomicAdd(&cnt->probes, w_probes);
         atomicAdd(&cnt->steps, steps);
         atomicAdd(&cnt->step_lanes, step_lanes);
         if (batched) atomicAdd(&cnt->batched, batched);
