@@ -308,51 +308,49 @@ __global__ void KPmaFinalize(const uint32_t* __restrict__ win_of, const PmaWin* 
     }
 }
 
-// Deletes: one wavefront per group of 64 sorted delete keys; each leaf whose
-// first delete key falls in the group is compacted by that wavefront.
+// Deletes: one wavefront per leaf that holds delete keys (a wave per sorted
+// delete key, which goes on only when its key is the first of its leaf), so
+// that scattered deletes compact their leaves in parallel rather than up to
+// 64 leaves one after another per wave.
 __global__ void __launch_bounds__(256) KPmaDelete(const uint64_t* __restrict__ dkeys, const uint32_t* __restrict__ dleaf,
                                                   uint64_t n, uint64_t* __restrict__ slot, uint32_t* __restrict__ cnt) {
     const uint32_t lane = LaneId();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    for (uint64_t g = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; g * kWave < n; g += nw) {
-        const uint64_t i = g * kWave + lane;
-        const bool head = i < n && (i == 0 || dleaf[i] != dleaf[i - 1]);
-        for (uint64_t hm = __ballot(head); hm; hm &= hm - 1) {
-            const uint64_t h = g * kWave + (uint64_t)__builtin_ctzll(hm);
-            const uint32_t leaf = Uniform(dleaf[h]);
-            uint64_t e = h;  // end of this leaf's delete keys
-            {
-                uint64_t lo = h, hi = n;
-                while (lo < hi) {
-                    const uint64_t m = (lo + hi) >> 1;
-                    if (dleaf[m] <= leaf) lo = m + 1;
-                    else hi = m;
-                }
-                e = lo;
+    for (uint64_t h = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; h < n; h += nw) {
+        const uint32_t leaf = Uniform(dleaf[h]);
+        if (h > 0 && Uniform(dleaf[h - 1]) == leaf) continue;
+        uint64_t e = h;  // end of this leaf's delete keys
+        {
+            uint64_t lo = h, hi = n;
+            while (lo < hi) {
+                const uint64_t m = (lo + hi) >> 1;
+                if (dleaf[m] <= leaf) lo = m + 1;
+                else hi = m;
             }
-            uint64_t* L = slot + (uint64_t)leaf * kPmaLeaf;
-            const uint32_t c = Uniform(cnt[leaf]);
-            uint32_t w = 0;
-            for (uint32_t k = 0; k < c; k += kWave) {
-                const uint32_t t = k + lane;
-                const bool valid = t < c;
-                const uint64_t key = valid ? L[t] : kEmpty;
-                bool del = false;
-                if (valid) {
-                    const uint64_t p = h + LowerBoundU64(dkeys + h, e - h, key);
-                    del = p < e && dkeys[p] == key;
-                }
-                const bool keep = valid && !del;
-                const uint64_t m = __ballot(keep);
-                WaveSync();
-                if (keep) L[w + RankBelow(m)] = key;
-                WaveSync();
-                w += (uint32_t)__popcll(m);
-            }
-            for (uint32_t t = w + lane; t < c; t += kWave) L[t] = kEmpty;
-            if (lane == 0) cnt[leaf] = w;
-            WaveSync();
+            e = lo;
         }
+        uint64_t* L = slot + (uint64_t)leaf * kPmaLeaf;
+        const uint32_t c = Uniform(cnt[leaf]);
+        uint32_t w = 0;
+        for (uint32_t k = 0; k < c; k += kWave) {
+            const uint32_t t = k + lane;
+            const bool valid = t < c;
+            const uint64_t key = valid ? L[t] : kEmpty;
+            bool del = false;
+            if (valid) {
+                const uint64_t p = h + LowerBoundU64(dkeys + h, e - h, key);
+                del = p < e && dkeys[p] == key;
+            }
+            const bool keep = valid && !del;
+            const uint64_t m = __ballot(keep);
+            WaveSync();
+            if (keep) L[w + RankBelow(m)] = key;
+            WaveSync();
+            w += (uint32_t)__popcll(m);
+        }
+        for (uint32_t t = w + lane; t < c; t += kWave) L[t] = kEmpty;
+        if (lane == 0) cnt[leaf] = w;
+        WaveSync();
     }
 }
 
@@ -445,7 +443,7 @@ void LaunchPmaRebalance(const uint8_t* chosen, uint32_t nleaves, uint32_t height
 }
 void LaunchPmaDelete(const uint64_t* dkeys, const uint32_t* dleaf, uint64_t n, uint64_t* slot, uint32_t* cnt,
                      hipStream_t s) {
-    if (n) KPmaDelete<<<GridFor64((n + kWave - 1) / kWave, 4, 16384), 256, 0, s>>>(dkeys, dleaf, n, slot, cnt);
+    if (n) KPmaDelete<<<GridFor64(n, 4, 16384), 256, 0, s>>>(dkeys, dleaf, n, slot, cnt);
 }
 void LaunchPmaFlat(const uint64_t* slot, const uint32_t* cnt, const uint32_t* gpos, uint32_t nleaves, uint64_t* flat,
                    hipStream_t s) {
