@@ -201,6 +201,77 @@ __global__ void __launch_bounds__(kRsThreads) KRsScatter(const uint64_t* in, uin
     }
 }
 
+// Whole sort of at most one tile (kRsTile keys) in one block: the keys stay
+// in LDS across the digit passes; each pass takes the digit histogram, its
+// exclusive scan, then the same stable round-by-round scatter as KRsScatter
+// (the block's keys held in registers while LDS is rewritten).  One launch
+// instead of three per digit for the small key sets of most batches.
+__global__ void __launch_bounds__(kRsThreads) KRsSortTile(uint64_t* keys, uint32_t n, uint32_t bits) {
+    __shared__ uint64_t buf[kRsTile];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kRsThreads / kWave][256];
+    const int tid = threadIdx.x, w = tid / kWave;
+    const uint32_t lane = tid & (kWave - 1);
+    for (uint32_t i = tid; i < n; i += kRsThreads) buf[i] = keys[i];
+    __syncthreads();
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        run[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kRsThreads) atomicAdd(&run[(uint32_t)(buf[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        // exclusive scan of the 256 digit counts (Hillis-Steele in LDS)
+        uint32_t v = run[tid];
+        const uint32_t own = v;
+        for (int o = 1; o < 256; o <<= 1) {
+            __syncthreads();
+            const uint32_t add = tid >= o ? run[tid - o] : 0u;
+            __syncthreads();
+            v += add;
+            run[tid] = v;
+        }
+        __syncthreads();
+        run[tid] = v - own;
+        uint64_t k[kRsItems];
+#pragma unroll
+        for (int r = 0; r < kRsItems; r++) {
+            const uint32_t idx = (uint32_t)r * kRsThreads + tid;
+            k[r] = idx < n ? buf[idx] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRsItems; r++) {
+#pragma unroll
+            for (int ww = 0; ww < kRsThreads / kWave; ww++) wcnt[ww][tid] = 0;
+            __syncthreads();
+            const uint32_t idx = (uint32_t)r * kRsThreads + tid;
+            const bool valid = idx < n;
+            const uint64_t key = k[r];
+            const uint32_t d = (uint32_t)(key >> shift) & 255u;
+            uint64_t m = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                uint64_t bb = __ballot((d >> b) & 1u);
+                m &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t rank = __popcll(m & LaneMaskLt(lane));
+            if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+            __syncthreads();
+            if (valid) {
+                uint32_t pre = run[d];
+                for (int ww = 0; ww < w; ww++) pre += wcnt[ww][d];
+                buf[pre + rank] = key;
+            }
+            __syncthreads();
+            uint32_t add = 0;
+#pragma unroll
+            for (int ww = 0; ww < kRsThreads / kWave; ww++) add += wcnt[ww][tid];
+            run[tid] += add;
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = tid; i < n; i += kRsThreads) keys[i] = buf[i];
+}
+
 }  // namespace
 
 size_t ScanScratchBytes(uint64_t n) { return ((n + kScanTile - 1) / kScanTile + 1) * sizeof(uint64_t); }
@@ -232,6 +303,11 @@ size_t RadixHistWords(uint64_t n) {
 
 void RadixSortU64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t bits, uint32_t* hist, hipStream_t s) {
     if (n < 2) return;
+    if (n <= (uint64_t)kRsTile) {
+        KRsSortTile<<<1, kRsThreads, 0, s>>>(keys, (uint32_t)n, bits);
+        CGD_HIP(hipGetLastError());
+        return;
+    }
     const uint32_t nb = (uint32_t)((n + kRsTile - 1) / kRsTile);
     const uint64_t hn = (uint64_t)nb * 256;
     // The scan scratch lives right after the histogram.
