@@ -223,6 +223,19 @@ void SetCounter(cgbdsm_query* q, unsigned long long Counters::*f, uint64_t v) {
     FillU64((uint64_t*)Cnt(q, f), v, 1, q->s);
 }
 
+// Several counters of q set by one launch on stream s.
+void SetCounters(cgbdsm_query* q, std::initializer_list<std::pair<unsigned long long Counters::*, uint64_t>> fv,
+                 hipStream_t s) {
+    static const Counters probe{};
+    WordSet ws{};
+    for (const auto& [f, v] : fv) {
+        if (ws.n == 16) Throw(CGBDSM_ERR_INVALID, "too many counters in one set");
+        ws.off[ws.n] = (uint32_t)(((const char*)&(probe.*f) - (const char*)&probe) / sizeof(uint64_t));
+        ws.val[ws.n++] = v;
+    }
+    SetWords((uint64_t*)q->d_cnt, ws, s);
+}
+
 void GrowDV(cgbdsm_query* q, uint32_t dv) {
     if (dv > q->dv_cap) {
         uint32_t nc = std::max<uint32_t>({dv, q->dv_cap + q->dv_cap / 2, 1024});
@@ -436,11 +449,17 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
 void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
     const uint64_t n = q->h_rowbase.back();
     const uint32_t depth = q->h_jobs.empty() ? 0 : q->h_jobs[0].D;
-    SetCounter(q, &Counters::flags, 0);
     LaunchQueueSync(q->d_cnt, q->s);
-    FillU64((uint64_t*)&q->d_cnt->q_rows.v, 0, 1, q->s);
-    FillU64((uint64_t*)&q->d_cnt->q_pending.v, n, 1, q->s);
-    FillU64((uint64_t*)&q->d_cnt->q_idle.v, 0, 1, q->s);
+    {
+        // flags, row cursor, pending work and idle count in one launch
+        WordSet ws{};
+        const size_t w[4] = {offsetof(Counters, flags), offsetof(Counters, q_rows), offsetof(Counters, q_pending),
+                             offsetof(Counters, q_idle)};
+        for (int i = 0; i < 4; i++) ws.off[i] = (uint32_t)(w[i] / sizeof(uint64_t));
+        ws.val[2] = n;
+        ws.n = 4;
+        SetWords((uint64_t*)q->d_cnt, ws, q->s);
+    }
     *(volatile uint32_t*)q->h_abort = 0;
     std::atomic_thread_fence(std::memory_order_seq_cst);
     if (!n) return;
@@ -1210,8 +1229,7 @@ cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
 const Frontier& BuildFrontier(cgbdsm_query* q) {
     if (q->fr_ready) return q->fr;
     const EiList L = ActiveEdges(q);
-    for (auto f : {&Counters::rows_out, &Counters::flags, &Counters::tuples})
-        SetCounter(q, f, 0);
+    SetCounters(q, {{&Counters::rows_out, 0}, {&Counters::flags, 0}, {&Counters::tuples, 0}}, q->s);
     const Tables T = MakeTables(q);
     const bool fixed_cap = q->frontier_cap_rows != 0;
     uint64_t cap_rows = fixed_cap ? q->frontier_cap_rows : (1u << 20);  // seeds grow the buffer on overflow
@@ -1289,9 +1307,9 @@ void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
 }
 
 void ResetDfsCounters(cgbdsm_query* q) {
-    for (auto f : {&Counters::scanned, &Counters::probes, &Counters::donated, &Counters::wave_ticks,
-                   &Counters::idle_ticks, &Counters::steps, &Counters::step_lanes, &Counters::batched})
-        SetCounter(q, f, 0);
+    SetCounters(q, {{&Counters::scanned, 0}, {&Counters::probes, 0}, {&Counters::donated, 0},
+                    {&Counters::wave_ticks, 0}, {&Counters::idle_ticks, 0}, {&Counters::steps, 0},
+                    {&Counters::step_lanes, 0}, {&Counters::batched, 0}}, q->s);
 }
 
 // Statistics of the final-DFS launch whose queue lived in `leader`.

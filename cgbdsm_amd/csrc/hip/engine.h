@@ -174,6 +174,13 @@ size_t ScanScratchBytes(uint64_t n);
 void RadixSortU64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t bits, uint32_t* hist, hipStream_t s);
 size_t RadixHistWords(uint64_t n);
 void FillU64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+// Up to 16 words of one block set in one launch (word offsets from base).
+struct WordSet {
+    uint32_t n;
+    uint32_t off[16];
+    uint64_t val[16];
+};
+void SetWords(uint64_t* base, const WordSet& ws, hipStream_t s);
 void FillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 
 }  // namespace cgd

@@ -134,6 +134,9 @@ __global__ void KFill64(uint64_t* p, uint64_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+__global__ void KSetWords(uint64_t* base, WordSet ws) {
+    if (threadIdx.x < ws.n) base[ws.off[threadIdx.x]] = ws.val[threadIdx.x];
+}
 __global__ void KFill32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -287,6 +290,9 @@ void FillU64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
     if (!n) return;
     unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
     KFill64<<<g, 256, 0, s>>>(p, v, n);
+}
+void SetWords(uint64_t* base, const WordSet& ws, hipStream_t s) {
+    if (ws.n) KSetWords<<<1, 64, 0, s>>>(base, ws);
 }
 void FillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     if (!n) return;
