@@ -187,8 +187,8 @@ def test_two_batched_launches_at_once(eng):
     own host thread, so that one group's index maintenance and frontier
     builds overlap the other group's launch: every count equals the count of
     the same query run alone.  (A batched launch runs on its first query's
-    stream and reads the other queries' frontiers, so a frontier build must
-    return with its own stream idle.)"""
+    stream and reads the other queries' frontiers, so it must wait for each
+    frontier's completion event on that query's own stream.)"""
     import threading
 
     print("case two batched launches at once", flush=True)
