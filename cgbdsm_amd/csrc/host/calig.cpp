@@ -410,16 +410,20 @@ void CaLiG::DelAndCheck(std::vector<Task>& global, std::vector<Task>& next) {
     });
     std::vector<DelTask> check;
     Gather(lcheck, check);
+    // The reference re-checks only the group of the removed candidate's label
+    // (tryNei on rep_nei[lb], index/calig.cpp:452-470) but dedups the checks
+    // of a pair with one flag per (vertex, query vertex): when candidates of
+    // two label groups of one pair go in the same round, whichever check
+    // runs first sets the flag and the other group is never re-checked, so
+    // the pair can stay lit without an injective support (a TBB-order
+    // dependent result).  Here the one check of a pair covers all of its
+    // groups (checkNei), which makes the lit bits the greatest fixpoint.
     pool_.For(check.size(), 256, [&](size_t b, size_t e, int tid) {
-        std::vector<int> used;
         for (size_t i = b; i < e; i++) {
-            int uj = check[i].ui, vj = check[i].vi, lb = check[i].lb;
+            int uj = check[i].ui, vj = check[i].vi;
             if (TestAndSetChecked(vj, uj)) continue;
-            const std::vector<int>* grp = nullptr;
-            for (auto& g : rep_nei_[uj])
-                if (g.first == lb) grp = &g.second;
-            used.clear();
-            if (!TryNei(0, vj, uj, used, *grp)) {
+            if (!(__atomic_load_n(&li_[vj], __ATOMIC_RELAXED) & (1u << uj))) continue;  // emptied in phase 2
+            if (!CheckNei(vj, uj)) {
                 LiAnd(vj, uj);
                 lnext[tid].push_back({uj, vj});
             }
