@@ -34,6 +34,10 @@ typedef struct {
     uint64_t n_batch;
     uint32_t tag;                    /* anchored mode: tag edge, else UINT32_MAX */
     uint32_t img[CGBDSM_MAX_QV];
+    /* recording (or_brute_tuples): rows of qv ids in query-vertex order */
+    uint32_t* rec;
+    uint8_t* rec_tag;
+    uint64_t rec_cap, rec_n;
 } bt_t;
 
 static int has_edge(const bt_t* b, uint32_t x, uint32_t y) {
@@ -105,7 +109,16 @@ static void setup_order(bt_t* b, uint32_t first, uint32_t second) {
 }
 
 static uint64_t extend(bt_t* b, uint32_t k) {
-    if (k == b->qv) return 1;
+    if (k == b->qv) {
+        if (b->rec) {
+            if (b->rec_n < b->rec_cap) {
+                memcpy(b->rec + b->rec_n * b->qv, b->img, sizeof(uint32_t) * b->qv);
+                b->rec_tag[b->rec_n] = (uint8_t)b->tag;
+            }
+            b->rec_n++;
+        }
+        return 1;
+    }
     uint32_t u = b->order[k];
     uint64_t cnt = 0;
     if (b->nnb[k] == 0) {
@@ -228,6 +241,33 @@ uint64_t or_anchored_count(uint32_t n, const uint64_t* offs, const uint32_t* adj
         tot += P.per_edge[e];
     }
     return tot;
+}
+
+uint64_t or_brute_tuples(uint32_t n, const uint64_t* offs, const uint32_t* adj, const int32_t* labels, uint32_t qv,
+                         const int32_t* qlabels, uint32_t qe, const uint32_t* qedges, const uint64_t* batch_keys,
+                         uint64_t n_batch, uint32_t* out, uint8_t* tags, uint64_t cap) {
+    bt_t b;
+    memset(&b, 0, sizeof(b));
+    b.n = n;
+    b.offs = offs;
+    b.adj = adj;
+    b.labels = labels;
+    b.qv = qv;
+    b.qe = qe;
+    b.ql = qlabels;
+    b.qedges = qedges;
+    b.batch = batch_keys;
+    b.n_batch = n_batch;
+    b.rec = out;
+    b.rec_tag = tags;
+    b.rec_cap = out && tags ? cap : 0;
+    uint64_t total = 0;
+    for (uint32_t e = 0; e < qe; e++) {
+        b.tag = e;
+        setup_order(&b, qedges[2 * e], qedges[2 * e + 1]);
+        for (uint64_t bi = 0; bi < n_batch; bi++) total += anchored_one(&b, e, bi);
+    }
+    return total;
 }
 
 /* ---------------------------------------------------------------- LI */

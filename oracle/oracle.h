@@ -81,6 +81,15 @@ uint64_t or_anchored_count(uint32_t n, const uint64_t* offs, const uint32_t* adj
                            uint32_t qv, const int32_t* qlabels, uint32_t qe, const uint32_t* qedges,
                            const uint64_t* batch_keys, uint64_t n_batch, uint64_t* per_edge, int threads);
 
+/* The new embeddings of one batch themselves, found as or_anchored_count
+ * finds them (single-threaded): up to cap rows of qv original vertex ids in
+ * query-vertex order go to out[], the tag (largest query edge, qe_list
+ * order, mapped to a batch edge) of each to tags[].  Returns the number of
+ * new embeddings (also when it exceeds cap). */
+uint64_t or_brute_tuples(uint32_t n, const uint64_t* offs, const uint32_t* adj, const int32_t* labels, uint32_t qv,
+                         const int32_t* qlabels, uint32_t qe, const uint32_t* qedges, const uint64_t* batch_keys,
+                         uint64_t n_batch, uint32_t* out, uint8_t* tags, uint64_t cap);
+
 /* Greatest fixpoint of the lit condition on G: li_out[v] bit u set iff v
  * has u's label and, for every query neighbour uj of u, v has a neighbour
  * lit for uj, with distinct supports for query neighbours sharing a label. */

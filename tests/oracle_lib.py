@@ -47,6 +47,9 @@ def lib():
         L.or_anchored_count.restype = U64
         L.or_li_fixpoint.argtypes = [U32, C.c_void_p, C.c_void_p, C.c_void_p, U32, C.c_void_p, U32, C.c_void_p,
                                      C.c_void_p]
+        L.or_brute_tuples.argtypes = [U32, C.c_void_p, C.c_void_p, C.c_void_p, U32, C.c_void_p, U32, C.c_void_p,
+                                      C.c_void_p, U64, C.c_void_p, C.c_void_p, U64]
+        L.or_brute_tuples.restype = U64
         _lib = L
     return _lib
 
@@ -140,6 +143,29 @@ def anchored_count(offs, adj, labels, qlabels, qedges, batch_pairs, threads=4):
                               qlabels.ctypes.data, qedges.size // 2, qedges.ctypes.data, _p(keys), keys.size,
                               per.ctypes.data, threads)
     return tot, per
+
+
+def _batch_keys(batch_pairs):
+    bp = np.asarray(batch_pairs, dtype=np.uint64).reshape(-1, 2)
+    return np.unique((np.minimum(bp[:, 0], bp[:, 1]) << np.uint64(32)) | np.maximum(bp[:, 0], bp[:, 1]))
+
+
+def brute_tuples(offs, adj, labels, qlabels, qedges, batch_pairs, cap=1 << 20):
+    """New embeddings of one batch (graph = G_b) as an (n, qv) array of
+    original ids in query-vertex order, and the tag of each."""
+    L = lib()
+    qlabels = np.ascontiguousarray(qlabels, dtype=np.int32)
+    qedges = np.ascontiguousarray(qedges, dtype=np.uint32)
+    keys = _batch_keys(batch_pairs)
+    qv = qlabels.size
+    out = np.zeros(max(cap * qv, 1), dtype=np.uint32)
+    tags = np.zeros(max(cap, 1), dtype=np.uint8)
+    n = L.or_brute_tuples(len(labels), offs.ctypes.data, adj.ctypes.data, labels.ctypes.data, qv, qlabels.ctypes.data,
+                          qedges.size // 2, qedges.ctypes.data, _p(keys), keys.size, out.ctypes.data,
+                          tags.ctypes.data, cap)
+    if n > cap:
+        raise ValueError(f"{n} new embeddings exceed the tuple capacity {cap}")
+    return out[: n * qv].reshape(n, qv), tags[:n]
 
 
 def li_fixpoint(offs, adj, labels, qlabels, qedges):

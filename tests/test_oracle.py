@@ -170,11 +170,40 @@ def test_index_algebra_matches_numpy_sets(ebits):
 
 # ---------------------------------------------------------------- full CPU path
 
-def _cpu_stream(spec, batch_edges, check_li=False, threads=4):
+def split_mismatch(per, ref):
+    """Query-edge tags whose counts differ between two per-edge splits."""
+    per, ref = np.asarray(per, dtype=np.uint64), np.asarray(ref, dtype=np.uint64)
+    return [e for e in range(max(per.size, ref.size)) if e >= per.size or e >= ref.size or per[e] != ref[e]]
+
+
+def match_rows(plan, tup, tei, old_ids):
+    """or_match tuples (compressed ids in global-order positions) as rows of
+    original ids in query-vertex order, with the tag as the last column."""
+    rows = np.zeros((len(tup), plan.qv + 1), dtype=np.int64)
+    for p in range(plan.qv):
+        rows[:, plan.go_vs[p]] = old_ids[tup[:, p]] if len(tup) else 0
+    rows[:, plan.qv] = tei
+    return rows
+
+
+def rows_of(tup, tags):
+    return np.concatenate([np.asarray(tup, dtype=np.int64).reshape(len(tags), -1),
+                           np.asarray(tags, dtype=np.int64)[:, None]], axis=1)
+
+
+def rows_equal(a, b):
+    if a.shape != b.shape:
+        return False
+    return np.array_equal(a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])])
+
+
+def _cpu_stream(spec, batch_edges, check_li=False, threads=4, tuples_cap=0):
     host = HostSession(spec=spec, batch_edges=batch_edges, threads=threads)
     keys, dv, card, avg = host.initial_keys()
     plan = host.make_plan(card, avg)
     ql, qe = host.query_info()
+    stream = host.update_stream()
+    pos = 0
     idx = O.Index(plan.ndir, 5)
     idx.set_dv(dv)
     idx.init(keys)
@@ -183,32 +212,114 @@ def _cpu_stream(spec, batch_edges, check_li=False, threads=4):
     if check_li:
         li = host.lit(len(lab))
         assert np.array_equal(li, O.li_fixpoint(offs, adj, lab, ql, qe)), "initial lit bits differ from the fixpoint"
+    splits = []
     for _ in range(host.num_batches):
         kb = host.next_batch()
+        pairs = stream[pos:pos + kb.pairs]
+        pos += kb.pairs
         idx.set_dv(kb.dv)
         idx.insert(kb.ins)
         idx.remove(kb.dele)
         idx.rebuild_update(kb.upd)
         masks = idx.masks(plan)
-        cnt, per = idx.match(plan, kb.edge_ok, masks=masks)
+        if tuples_cap:
+            cnt, per, tup, tei = idx.match(plan, kb.edge_ok, masks=masks, tuples_cap=tuples_cap)
+        else:
+            cnt, per = idx.match(plan, kb.edge_ok, masks=masks)
         cnt0, per0 = idx.match(plan, kb.edge_ok, masks=None)
         idx.insert(kb.upd)
         offs, adj, lab = host.graph()
         cur = O.brute_count(offs, adj, lab, ql, qe)
         assert cnt == cnt0 and np.array_equal(per, per0), "support masks must be a pure prune"
         assert cnt == cur - prev, "count differs from emb(G_b) - emb(G_b-1)"
+        atot, aper = O.anchored_count(offs, adj, lab, ql, qe, pairs)
+        assert atot == cnt and not split_mismatch(per, aper), \
+            f"per-edge split {list(per)} differs from the anchored enumeration {list(aper)}"
+        if tuples_cap:
+            assert cnt <= tuples_cap
+            btup, btag = O.brute_tuples(offs, adj, lab, ql, qe, pairs, cap=tuples_cap)
+            assert rows_equal(match_rows(plan, tup, tei, host.old_ids()), rows_of(btup, btag)), \
+                "oracle tuples differ from the brute-force new embeddings"
         if check_li:
             li = host.lit(len(lab))
             fp = O.li_fixpoint(offs, adj, lab, ql, qe)
             assert np.array_equal(li, fp), "incremental lit bits differ from the from-scratch fixpoint"
+        splits.append(per)
         prev = cur
-    return host
+    return host, splits
 
 
 @pytest.mark.parametrize("kind,seed", [("tree", 1), ("dense", 2), ("sparse", 3), ("dense", 4)])
 def test_cpu_path_matches_closed_form(kind, seed):
     spec = synth_spec(10, 8, 4, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
     _cpu_stream(spec, 300)
+
+
+@pytest.mark.parametrize("kind,seed,scale,labels", [("tree", 7, 8, 4), ("dense", 8, 8, 3), ("sparse", 9, 8, 3),
+                                                    ("dense", 10, 9, 6)])
+def test_oracle_tuples_equal_brute_force(kind, seed, scale, labels):
+    """The embeddings or_match enumerates (mapped to original ids and query
+    vertices) and their edge tags equal the brute-force new embeddings of
+    every batch; the per-edge split equals the anchored enumeration."""
+    spec = synth_spec(scale, 6, labels, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
+    _cpu_stream(spec, 60, tuples_cap=200000)
+
+
+def _one_batch(seed=8):
+    spec = synth_spec(8, 6, 3, seed=seed, query_size=5, query_kind="dense", query_seed=seed)
+    host = HostSession(spec=spec, batch_edges=60, threads=2)
+    keys, dv, card, avg = host.initial_keys()
+    plan = host.make_plan(card, avg)
+    ql, qe = host.query_info()
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    pairs = host.update_stream()
+    for _ in range(host.num_batches):
+        kb = host.next_batch()
+        b_pairs, pairs = pairs[:kb.pairs], pairs[kb.pairs:]
+        idx.set_dv(kb.dv)
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        idx.rebuild_update(kb.upd)
+        cnt, per, tup, tei = idx.match(plan, kb.edge_ok, masks=idx.masks(plan), tuples_cap=200000)
+        idx.insert(kb.upd)
+        if np.count_nonzero(per) >= 2 and cnt > 10:
+            offs, adj, lab = host.graph()
+            return plan, per, (offs, adj, lab, ql, qe, b_pairs), match_rows(plan, tup, tei, host.old_ids())
+    raise AssertionError("no batch with two non-zero edge tags")
+
+
+def test_split_check_catches_planted_permutation():
+    """A per-edge split with two tags' counts swapped must fail the anchored
+    comparison that passes on the true split."""
+    plan, per, g, _ = _one_batch()
+    _, aper = O.anchored_count(*g)
+    assert not split_mismatch(per, aper)
+    nz = [e for e in range(len(per)) if per[e]]
+    bad = per.copy()
+    other = next(e for e in range(len(per)) if per[e] != per[nz[0]])
+    bad[nz[0]], bad[other] = per[other], per[nz[0]]
+    assert split_mismatch(bad, aper)
+
+
+def test_tuple_check_catches_planted_column_swap():
+    """Oracle tuples with two query vertices' columns swapped (or one tag
+    changed) must fail the brute-force comparison that passes on the true
+    tuples."""
+    plan, per, g, rows = _one_batch()
+    btup, btag = O.brute_tuples(*g)
+    ref = rows_of(btup, btag)
+    assert rows_equal(rows, ref)
+    a, b = 0, 1
+    while np.array_equal(rows[:, a], rows[:, b]):
+        b += 1
+    swapped = rows.copy()
+    swapped[:, [a, b]] = rows[:, [b, a]]
+    assert not rows_equal(swapped, ref)
+    retag = rows.copy()
+    retag[0, plan.qv] = (retag[0, plan.qv] + 1) % plan.qe
+    assert not rows_equal(retag, ref)
 
 
 @pytest.mark.parametrize("scale,deg,labels,seed,batch,threads", [
@@ -272,5 +383,4 @@ def test_masks_match_definition():
                     ok = ok and any(bit(ei, u1, x) for x in cand)
                 assert ok == bit(ei, u0, v), (ei, d, v)
                 checked += 1
-            break
     assert checked > 0
