@@ -151,18 +151,34 @@ cgbdsm_status Guard(F&& f) {
 
 [[noreturn]] void Throw(cgbdsm_status c, const std::string& m) { throw ApiError{c, m}; }
 
+// A new batch may be staged once the previous one is committed
+// (cgbdsm_batch_update_global_index) or was skipped for an empty update
+// index (main_calig.cpp:301-302 skips the rest of such a batch): staging over
+// a batch still between apply and commit would drop that batch's commit.
+void CheckCanStage(const cgbdsm_query* q) {
+    if (q->phase == 0 || (q->phase == 3 && !q->u_nonempty)) return;
+    static const char* names[] = {"idle", "staged", "applied", "update index built", "masks built", "matched"};
+    Throw(CGBDSM_ERR_INVALID, std::string("the previous batch is ") + names[q->phase] +
+                                  " but not committed: call cgbdsm_batch_update_global_index first");
+}
+
 void* DevAlloc(cgbdsm_ctx* ctx, size_t bytes) {
     if (!bytes) return nullptr;
-    if (ctx->budget && ctx->used.load() + bytes > ctx->budget)
-        Throw(CGBDSM_ERR_OOM, "device memory budget exceeded (" + std::to_string(ctx->used.load() + bytes) + " > " +
+    // Reserve the bytes against the budget before allocating, so that
+    // queries allocating from several threads cannot pass the check together.
+    const uint64_t before = ctx->used.fetch_add(bytes);
+    if (ctx->budget && before + bytes > ctx->budget) {
+        ctx->used.fetch_sub(bytes);
+        Throw(CGBDSM_ERR_OOM, "device memory budget exceeded (" + std::to_string(before + bytes) + " > " +
                                   std::to_string(ctx->budget) + " bytes)");
+    }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        ctx->used.fetch_sub(bytes);
         Throw(CGBDSM_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
-    ctx->used.fetch_add(bytes);
     return p;
 }
 
@@ -971,6 +987,7 @@ cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q, const uint64_t* ins, uint64_t 
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
         if (!q->have_order) Throw(CGBDSM_ERR_INVALID, "cgbdsm_query_set_plan must run before the first batch");
+        CheckCanStage(q);
         if (!edge_ok) Throw(CGBDSM_ERR_INVALID, "null edge_ok");
         if (dv_count < q->dv) Throw(CGBDSM_ERR_INVALID, "compressed vertex count cannot shrink");
         if (dv_count > (1u << (32 - q->ebits)) || dv_count > q->hplan.v_mask)
@@ -1045,6 +1062,7 @@ cgbdsm_status cgbdsm_batch_stage_next(cgbdsm_query* q, int* staged) {
         if (!q->have_order) Throw(CGBDSM_ERR_INVALID, "cgbdsm_query_set_plan must run before the first batch");
         if (staged) *staged = 0;
         if (q->fifo.empty()) return;
+        CheckCanStage(q);
         cgbdsm_query::Resident& r = q->fifo.front();
         GrowDV(q, r.dv);
         const uint64_t nmax = std::max<uint64_t>({r.n_ins, r.n_del, r.n_upd, 1});
@@ -1316,6 +1334,8 @@ void ResetDfsCounters(cgbdsm_query* q) {
 void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
     ReadCounters(leader);
     const Counters* c = leader->h_cnt;
+    if (c->flags & kFlagBadRow)
+        Throw(CGBDSM_ERR_DEVICE, "final DFS read a frontier row with an out-of-range tag or vertex id");
     if (c->flags & (kFlagSteal | kFlagAborted))
         Throw(CGBDSM_ERR_OVERFLOW, "final DFS did not complete (flags " + std::to_string(c->flags) + ")");
     if (rows) KAdd(leader, CGBDSM_K_DFS, 4ull * (c->scanned + c->probes), rows);

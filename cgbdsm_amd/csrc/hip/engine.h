@@ -74,7 +74,8 @@ enum : unsigned long long {
     kFlagFrontier = 2,
     kFlagTuples = 4,
     kFlagSteal = 8,
-    kFlagAborted = 16
+    kFlagAborted = 16,
+    kFlagBadRow = 32   // a final-DFS frontier row held an out-of-range tag or vertex id
 };
 
 // One stolen piece of work: positions 0..depth-1 of the global order are

@@ -1237,7 +1237,18 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 const uint64_t lr = r - row_base[jlo];
                 const uint32_t w0 = Uniform(rows[lr * D]);
                 ei = w0 >> tag_shift;
-                if (lane < D) matched[lane] = lane == 0 ? (w0 & v_mask) : rows[lr * D + lane];
+                const uint32_t mv = lane == 0 ? (w0 & v_mask) : (lane < D ? rows[lr * D + lane] : 0u);
+                // A row whose tag or vertex ids are out of range (frontier
+                // written out of order with this launch) is not followed into
+                // the index: it is dropped and flagged, and the call fails.
+                if (ei >= T.qe || __ballot(lane < D && mv >= T.dv)) {
+                    if (lane == 0) {
+                        atomicOr(&cnt->flags, kFlagBadRow);
+                        if (r_next >= r_end) atomicSub(&cnt->q_pending.v, (unsigned long long)(r_end - r_first));
+                    }
+                    continue;
+                }
+                if (lane < D) matched[lane] = mv;
                 D0 = D;
                 got = true;
             }

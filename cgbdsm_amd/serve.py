@@ -68,6 +68,8 @@ class _Query:
     joined_at: int = None            # first batch counted in the live set
     ready: bool = False              # caught up, waiting to join
     failed: object = None
+    stop: bool = False               # the loop ended: the catch-up thread returns
+    thread: threading.Thread = None
     cond: threading.Condition = field(default_factory=threading.Condition)
 
 
@@ -112,9 +114,15 @@ class Server:
         dq.sync()
 
     def _catch_up(self, q):
+        """Load q and apply the batches it missed until it reaches the live
+        set's position; returns when it joined, or when the loop sets q.stop
+        (the loop then joins this thread before it closes q's handle)."""
         try:
             self._open(q)
             while True:
+                with q.cond:
+                    if q.stop:
+                        return
                 with self.lock:
                     target = self.b_next
                 if q.applied < target:
@@ -123,9 +131,10 @@ class Server:
                     continue
                 with q.cond:
                     q.ready = True
-                    # wait until joined, or until the live set moved on
-                    q.cond.wait_for(lambda: q.joined_at is not None or self.b_next > q.applied, timeout=1.0)
-                    if q.joined_at is not None:
+                    # wait until joined, stopped, or the live set moved on
+                    q.cond.wait_for(lambda: q.stop or q.joined_at is not None or self.b_next > q.applied,
+                                    timeout=1.0)
+                    if q.joined_at is not None or q.stop:
                         return
                     q.ready = False
         except Exception as e:  # admission failure (OOM) or device error
@@ -136,6 +145,16 @@ class Server:
             with q.cond:
                 q.ready = True
                 q.cond.notify_all()
+
+    def _stop_catching(self, catching):
+        """Stop and join the catch-up threads of queries that never joined."""
+        for q in catching:
+            with q.cond:
+                q.stop = True
+                q.cond.notify_all()
+        for q in catching:
+            if q.thread is not None:
+                q.thread.join()
 
     # ------------------------------------------------------------ one batch
     def _prepare(self, q, b):
@@ -226,7 +245,8 @@ class Server:
                 q = _Query(r)
                 catching.append(q)
                 admitted[r.name] = {"arrived_s": r.arrive_s, "admitted_s": now(), "at_batch": b}
-                threading.Thread(target=self._catch_up, args=(q,), daemon=True).start()
+                q.thread = threading.Thread(target=self._catch_up, args=(q,), daemon=True)
+                q.thread.start()
             # joins at this batch boundary; failed loads go back to the queue
             for q in list(catching):
                 with q.cond:
@@ -234,6 +254,7 @@ class Server:
                         continue
                     if q.failed is not None:
                         catching.remove(q)
+                        q.thread.join()
                         if isinstance(q.failed, CgbdsmError) and q.failed.code == OOM:
                             rejected_oom += 1
                             blocked = True
@@ -269,9 +290,14 @@ class Server:
                 q.dq = None
                 blocked = False
         t_end = now()
+        # Queries still catching up stop applying batches before their
+        # handles close: a thread still queuing work on a closed handle
+        # would touch freed device buffers.
+        self._stop_catching(catching)
         for q in live + catching:
             if q.dq is not None:
                 q.dq.close()
+                q.dq = None
         edges = sum(p for _, p in arrivals)
         return {
             "counts": counts,

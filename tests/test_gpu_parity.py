@@ -208,11 +208,93 @@ def test_rejects_out_of_range_keys(eng):
 
 
 def test_call_order_is_checked(eng):
+    """Out-of-order calls fail with CGBDSM_ERR_INVALID: nothing staged, and
+    a new batch staged while the previous one sits between stage and commit
+    (it would lose that batch's commit).  After the commit, staging works
+    and the counts still equal the oracle's."""
     spec = synth_spec(8, 4, 2, seed=10, query_size=4, query_kind="tree", query_seed=10)
-    host = HostSession(spec=spec, batch_edges=None, threads=2)
+    host = HostSession(spec=spec, batch_edges=25, threads=2)
     keys, dv, card, avg = host.initial_keys()
     q = eng.query(host.query_tables)
     q.index_init(keys, dv)
     with pytest.raises(CgbdsmError):
         q.transfer_update_all()  # nothing staged
+    plan = host.make_plan(card, avg)
+    q.set_plan(plan)
+    idx = O.Index(plan.ndir, host.ebits)
+    idx.set_dv(dv)
+    idx.init(keys)
+    assert host.num_batches >= 2
+    kb = host.next_batch()
+    nxt = host.next_batch()
+    q.stage(kb)
+    with pytest.raises(CgbdsmError):
+        q.stage(nxt)  # staged, not applied
+    q.transfer_update_all()
+    with pytest.raises(CgbdsmError):
+        q.stage(nxt)  # applied, not committed
+    for b in (kb, nxt):
+        if b is nxt:
+            q.stage(nxt)
+            q.transfer_update_all()
+        idx.set_dv(b.dv)
+        idx.insert(b.ins)
+        idx.remove(b.dele)
+        idx.rebuild_update(b.upd)
+        if q.transfer_update_index():
+            q.build_local_index()
+            cnt, per = q.matching()
+            with pytest.raises(CgbdsmError):
+                q.stage(nxt)  # matched, not committed
+            ocnt, oper = idx.match(plan, b.edge_ok, masks=idx.masks(plan))
+            assert cnt == ocnt and np.array_equal(per, oper)
+            q.batch_update_global_index()
+        idx.insert(b.upd)
+    assert np.array_equal(q.export(0), idx.export(0))
+    q.close()
+
+
+def test_deadline_abort_then_recount(eng):
+    """A final DFS past its deadline: the host raises the mapped abort flag,
+    every wave leaves its wait and the call fails with CGBDSM_ERR_OVERFLOW.
+    The same query, deadline off, then counts the same batch again (the work
+    queue resynchronised past the tickets abandoned mid-launch) and the
+    total and per-edge split equal the oracle's."""
+    print("case deadline-abort", flush=True)
+    spec = synth_spec(12, 10, 3, seed=6, query_size=7, query_kind="dense", query_seed=6)
+    host = HostSession(spec=spec, batch_edges=800, threads=4)
+    keys, dv, card, avg = host.initial_keys()
+    q = eng.query(host.query_tables)
+    q.index_init(keys, dv)
+    plan = host.make_plan(card, avg)
+    q.set_plan(plan)
+    # every frontier row goes to the DFS, which enumerates (no closed-form
+    # suffix), so the launch lasts long enough for its waves to poll the flag
+    q.set_match_params(bfs_row_limit=0)
+    q.set_dfs_options(shortest_list=False, product_suffix=False)
+    idx = O.Index(plan.ndir, host.ebits)
+    idx.set_dv(dv)
+    idx.init(keys)
+    kb = host.next_batch()
+    q.stage(kb)
+    q.transfer_update_all()
+    idx.set_dv(kb.dv)
+    idx.insert(kb.ins)
+    idx.remove(kb.dele)
+    idx.rebuild_update(kb.upd)
+    assert q.transfer_update_index()
+    q.build_local_index()
+    q.set_deadline(0.001)
+    with pytest.raises(CgbdsmError) as ex:
+        q.matching()
+    assert ex.value.code == 5, ex.value  # CGBDSM_ERR_OVERFLOW
+    print(f"aborted: {ex.value}", flush=True)
+    q.set_deadline(0)
+    cnt, per = q.matching()
+    ocnt, oper = idx.match(plan, kb.edge_ok, masks=idx.masks(plan), threads=4)
+    assert cnt == ocnt and np.array_equal(per, oper), (cnt, ocnt, list(per), list(oper))
+    assert cnt > 0
+    # and once more, on a queue that has now completed normally
+    cnt2, per2 = q.matching()
+    assert cnt2 == ocnt and np.array_equal(per2, oper)
     q.close()

@@ -1,0 +1,97 @@
+"""Serving-loop control flow on the CPU, with a fake engine that records
+every call on a query handle (no GPU): a query still catching up when the
+stream ends must not be used after its handle is closed."""
+import threading  # noqa: F401
+import time
+
+from cgbdsm_amd.serve import Registration, Server
+
+
+class FakeQuery:
+    def __init__(self, eng, name):
+        self.eng, self.name, self.closed = eng, name, False
+        self.phase = 0
+
+    def _use(self, what):
+        if self.closed:
+            self.eng.after_close.append((self.name, what))
+        self.eng.calls.append((self.name, what))
+
+    def set_layout(self, layout):
+        self._use("set_layout")
+
+    def index_init(self, keys, dv):
+        self._use("index_init")
+
+    def set_plan(self, plan):
+        self._use("set_plan")
+
+    def set_deadline(self, ms):
+        self._use("set_deadline")
+
+    def set_dfs_tuning(self, *a):
+        self._use("set_dfs_tuning")
+
+    def stage(self, kb):
+        self._use("stage")
+        time.sleep(self.eng.apply_s if self.name in self.eng.slow else 0)
+
+    def transfer_update_all(self):
+        self._use("transfer_update_all")
+
+    def transfer_update_index(self):
+        self._use("transfer_update_index")
+        return True
+
+    def build_local_index(self):
+        self._use("build_local_index")
+
+    def build_frontier(self):
+        self._use("build_frontier")
+
+    def batch_update_global_index(self):
+        self._use("batch_update_global_index")
+
+    def sync(self):
+        self._use("sync")
+
+    def close(self):
+        self._use("close")
+        self.closed = True
+
+
+class FakeEngine:
+    def __init__(self, slow=(), apply_s=0.05):
+        self.calls, self.after_close, self.slow, self.apply_s = [], [], set(slow), apply_s
+        self.lock = threading.Lock()
+
+    def query(self, tables):
+        return FakeQuery(self, tables)
+
+    def memory(self):
+        return 0, 0
+
+    def matching_batch(self, qs):
+        return [(1, None) for _ in qs]
+
+
+def _reg(name, arrive_s=0.0, nbatches=4):
+    return Registration(name=name, tables=name, plan=None, keys=[], dv=0, batches=[object()] * nbatches,
+                        arrive_s=arrive_s, footprint=1)
+
+
+def test_catching_query_is_stopped_before_its_handle_closes():
+    eng = FakeEngine(slow={"late"}, apply_s=0.05)
+    srv = Server(eng, autotune=False, tuning=(4, 0))
+    n = 4
+    # "late" arrives after the first batch and needs 50 ms per missed batch;
+    # the stream ends while it is still catching up.
+    arrivals = [(0.002 * b, 10) for b in range(n)]
+    regs = [_reg("live", 0.0, n), _reg("late", 0.001, n)]
+    res = srv.run(regs, arrivals)
+    srv.close()
+    time.sleep(0.3)  # a catch-up thread left running would now call into the closed handle
+    assert eng.after_close == [], f"calls on closed handles: {eng.after_close}"
+    closes = [c for c in eng.calls if c == ("late", "close")]
+    assert len(closes) == 1
+    assert len(res["counts"]["live"]) == n
