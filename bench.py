@@ -419,25 +419,218 @@ def index_stress(args, threads, log):
             "lists_equal": bool(np.array_equal(finals["gamma"], finals["gpma"]))}
 
 
+def numa_nodes():
+    try:
+        return max(1, len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and
+                           d[4:].isdigit()]))
+    except OSError:
+        return 1
+
+
+def reference_threads():
+    """The reference CPU path's default thread count: hardware threads / NUMA
+    nodes (index/calig.cpp:1092-1095)."""
+    total = os.cpu_count() or 1
+    return max(1, total // numa_nodes()), total, numa_nodes()
+
+
+def config4(args, log):
+    """Batched multi-query headline: one update stream, many registered
+    queries of uneven size, Stage 1 and Stage 2 on the device, plain (one
+    query after another) vs served (concurrent Stage 1 and index work, one
+    batched final-DFS launch).  A query whose exact count cannot be taken
+    within --probe-ms on the first batch (or does not fit 64 bits) is
+    excluded by name before timing."""
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import Workload, synth_spec
+    from cgbdsm_amd.multi import QuerySet, sample_mix
+    from cgbdsm_amd.tune import autotune
+
+    t_setup = time.perf_counter()
+    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                      initial_frac=args.initial_frac, stream_edges=args.stream_edges)
+    w = Workload(spec)
+    specs = sample_mix(w, args.queries, args.mix_seed)
+    log(f"workload: {w.n} vertices, {w.edges.shape[0]} initial edges, {w.stream.size // 2} stream edges, "
+        f"{len(specs)} queries, generated in {time.perf_counter() - t_setup:.1f}s")
+    eng = Engine(0)
+    g = eng.graph(w.labels, w.edges)
+    g.set_stream(w.stream)
+    t_att = time.perf_counter()
+    plain = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.probe_ms)
+    served = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.deadline_ms)
+    attach_s = time.perf_counter() - t_att
+    log(f"{2 * len(specs)} device queries attached (CaLiG static filter + initial index on the GPU) in "
+        f"{attach_s:.1f}s")
+    bi = 2 * args.batch_edges
+    nb = (w.stream.size + bi - 1) // bi
+    warm = max(1, min(args.warmup, nb - 1))
+    steps = max(1, min(args.steps, nb - warm))
+    # Batch 0: which counts can be taken (plain, each query with the probe deadline).
+    g.apply_range(0, bi)
+    errors = {}
+    t0 = time.perf_counter()
+    c_plain = plain.run_plain(errors)
+    probe_s = time.perf_counter() - t0
+    excluded = {k: v.split(":")[0].strip() + ": " + v.split(";")[0].split(":", 1)[-1].strip()[:80]
+                for k, v in errors.items()}
+    plain.drop(set(errors))
+    served.drop(set(errors))
+    for name in errors:
+        log(f"excluded {name}: {errors[name][:160]}")
+    c_plain = [c for c in c_plain if c is not None]
+    tuned = {}
+
+    def tune(qs):
+        if tuned or args.no_autotune:
+            return
+        best, table, _ = autotune(eng, qs, grabs=TUNE_GRABS, blocks_per_cu=TUNE_BPC, log=log)
+        tuned.update(rows_per_grab=best[0], blocks_per_cu=best[1],
+                     dfs_ms={f"{a}/{b}": round(ms, 3) for (a, b), ms in table.items()})
+
+    c_served, _ = served.run_served(tune=tune)
+    counts_equal = [c_plain == c_served]
+    for m in plain.members:
+        m.q.set_deadline(args.deadline_ms)
+    for b in range(1, warm):
+        g.apply_range(b * bi, bi)
+        counts_equal.append(plain.run_plain() == served.run_served()[0])
+    log(f"{len(plain.members)} queries kept, {len(errors)} excluded; batch 0 plain {probe_s:.2f}s")
+    lat_plain, lat_served, lat_apply, dfs_ms, dfs_bytes, s1_ms, matches, edges = [], [], [], [], [], [], 0, 0
+    for b in range(warm, warm + steps):
+        ta = time.perf_counter()
+        g.apply_range(b * bi, bi)
+        t_apply = time.perf_counter() - ta
+        edges += min(bi, w.stream.size - b * bi) // 2
+        tp = time.perf_counter()
+        cp = plain.run_plain()
+        t_plain = time.perf_counter() - tp
+        s1_ms.append(sum(m.q.calig_stats()["ms"] for m in plain.members))
+        ts = time.perf_counter()
+        cs, dfs = served.run_served(profile_lead=True)
+        t_served = time.perf_counter() - ts
+        counts_equal.append(cp == cs)
+        matches += sum(cs)
+        lat_apply.append(1e3 * t_apply)
+        lat_plain.append(1e3 * (t_apply + t_plain))
+        lat_served.append(1e3 * (t_apply + t_served))
+        if dfs and dfs["launches"]:
+            dfs_ms.append(dfs["ms"] / dfs["launches"])
+            dfs_bytes.append(dfs["bytes"] / dfs["launches"])
+        log(f"batch {b}: apply {1e3 * t_apply:.1f} ms, plain {lat_plain[-1]:.1f} ms, served {lat_served[-1]:.1f} ms"
+            f", {sum(cs)} matches, counts equal {cp == cs}")
+    value = edges / (1e-3 * sum(lat_served))
+    plain_rate = edges / (1e-3 * sum(lat_plain))
+    launch_ms = statistics.mean(dfs_ms) if dfs_ms else 0.0
+    bpl = statistics.mean(dfs_bytes) if dfs_bytes else 0.0
+    achieved = bpl / (launch_ms * 1e-3) / 1e9 if launch_ms else 0.0
+    workload = (f"config 4: rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-zipf{args.zipf:g}, "
+                f"90/10 split seed 1, {args.batch_edges}-edge batches, {args.queries} random-walk queries of 4-12 "
+                f"vertices (1/3 each tree/sparse/dense, mix seed {args.mix_seed}), device Stage 1 + Stage 2")
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "update edges/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": round(statistics.mean(lat_served), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": workload, "batch_edges": args.batch_edges, "queries": args.queries,
+                   "queries_kept": len(plain.members), "layout": args.layout, "parallelism": "single GPU"},
+        "p50_batch_ms": round(percentile(lat_served, 50), 4), "p99_batch_ms": round(percentile(lat_served, 99), 4),
+        "plain": {"update_edges_per_s": round(plain_rate, 3), "p50_batch_ms": round(percentile(lat_plain, 50), 4),
+                  "p99_batch_ms": round(percentile(lat_plain, 99), 4)},
+        "served_vs_plain": round(value / plain_rate, 4),
+        "graph_apply_ms": round(statistics.mean(lat_apply), 4),
+        "stage1_device_ms_per_batch_plain": round(statistics.mean(s1_ms), 3),
+        "matches": int(matches), "matches_per_s": round(matches / (1e-3 * sum(lat_served)), 3),
+        "counts_equal_plain_served": all(counts_equal),
+        "excluded": excluded, "queries_kept": plain.names(), "attach_s": round(attach_s, 2),
+        "dfs_tuning": tuned or None,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("dfs_batched", workload),
+                     "kernel": "KDfs (batched launch)", "launch_ms": round(launch_ms, 4),
+                     "algorithmic_bytes_per_launch": int(bpl)},
+    }
+    kept = [m.spec for m in plain.members]
+    plain.close()
+    served.close()
+    g.close()
+    eng.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_config4(args, kept, w, log)
+    w.close()
+    return out
+
+
+def cpu_baseline_config4(args, kept, w, log):
+    """The CPU path (host CaLiG Stage 1 + the oracle's Stage 2, the reference's
+    thread count) on a bounded sample: the first batch of the stream for the
+    first kept query of each kind; the job's rate is the batch's edges over
+    the sample's mean per-query time times the number of kept queries."""
+    from cgbdsm_amd.host import HostSession, synth_spec
+
+    threads, total, nodes = reference_threads()
+    sample, per_q, s1_tot, s2_tot = [], [], 0.0, 0.0
+    seen = set()
+    for sp in kept:
+        kind = sp.name.split("-")[1].rstrip("0123456789")
+        if kind in seen:
+            continue
+        seen.add(kind)
+        size = int(sp.name.split("-")[1][len(kind):])
+        qseed = int(sp.name.split("-s")[-1])
+        spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                          initial_frac=args.initial_frac, stream_edges=args.stream_edges, query_size=size,
+                          query_kind=kind, query_seed=qseed)
+        t0 = time.perf_counter()
+        host = HostSession(spec=spec, batch_edges=args.batch_edges, threads=threads)
+        keys, dv, card, avg = host.initial_keys()
+        plan = host.make_plan(card, avg)
+        setup = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        kb = host.next_batch()
+        s1 = time.perf_counter() - t1
+        edges, s2, counts = cpu_baseline(keys, dv, plan, [kb], 0.0, threads)
+        sample.append(f"{sp.name} (setup {setup:.1f}s untimed)")
+        per_q.append(s1 + s2)
+        s1_tot += s1
+        s2_tot += s2
+        log(f"cpu baseline {sp.name}: [S1] {1e3 * s1:.0f} ms, CPU [S2] {1e3 * s2:.0f} ms, count {counts[0]}")
+        host.close()
+        if s1_tot + s2_tot > args.cpu_seconds:
+            break
+    mean = statistics.mean(per_q)
+    return {"value": round(args.batch_edges / (mean * len(kept)), 3), "unit": "update edges/s",
+            "cores": threads, "nproc": total, "numa_nodes": nodes, "kind": "port",
+            "sample": f"batch 0 ({args.batch_edges} edges) for {len(per_q)} of the {len(kept)} kept queries: "
+                      + ", ".join(sample) + f"; job rate = batch edges / (mean per-query time x {len(kept)})",
+            "s1_ms_per_query": round(1e3 * s1_tot / len(per_q), 3),
+            "s2_ms_per_query": round(1e3 * s2_tot / len(per_q), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--avg-degree", type=float, default=16)
+    ap.add_argument("--single", action="store_true",
+                    help="plain single-query Stage-2 line (host Stage 1; R-MAT s20 dense query) instead of the "
+                         "batched multi-query headline")
+    ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--avg-degree", type=float, default=None)
     ap.add_argument("--labels", type=int, default=16)
     ap.add_argument("--zipf", type=float, default=1.2)
-    ap.add_argument("--batch-edges", type=int, default=100_000)
+    ap.add_argument("--batch-edges", type=int, default=None)
     ap.add_argument("--initial-frac", type=float, default=0.9)
     ap.add_argument("--stream-edges", type=int, default=0, help="edges in the update stream (0: all the rest)")
+    ap.add_argument("--queries", type=int, default=64, help="registered queries of the headline mix")
+    ap.add_argument("--mix-seed", type=int, default=1, help="seed of the headline query mix")
+    ap.add_argument("--probe-ms", type=float, default=1000.0,
+                    help="headline: a query whose exact count of the first batch takes longer is excluded")
     ap.add_argument("--query-size", type=int, default=6)
     ap.add_argument("--query-kind", default="dense", choices=["dense", "tree", "sparse"])
     ap.add_argument("--query-seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--multi-only", action="store_true",
-                    help="run only the multi-query comparison (plain / served_threads / served)")
+                    help="run only the 8-query comparison with host Stage 1 (plain / served_threads / served)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default final-DFS launch shape instead of tuning it on the first warmup batch")
     ap.add_argument("--deadline-ms", type=float, default=60_000)
@@ -455,7 +648,7 @@ def main():
     ap.add_argument("--index-stress", action="store_true",
                     help="index maintenance only, GAMMA vs GPMA (prints its own JSON line)")
     ap.add_argument("--multi-queries", type=int, default=8,
-                    help="queries in the multi-query (plain vs served) comparison; 0 skips it")
+                    help="queries in the host-Stage-1 multi-query comparison of --single; 0 skips it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -469,6 +662,13 @@ def main():
     from cgbdsm_amd.host import HostSession, synth_spec
 
     threads = min(16, os.cpu_count() or 1)
+    headline = not (args.single or args.index_stress or args.serve or args.multi_only)
+    if args.scale is None:
+        args.scale = 21 if headline else 20
+    if args.avg_degree is None:
+        args.avg_degree = 24 if headline else 16
+    if args.batch_edges is None:
+        args.batch_edges = 250_000 if headline else 100_000
     if args.index_stress:
         print(json.dumps(index_stress(args, threads, log)), flush=True)
         return
@@ -479,6 +679,9 @@ def main():
         out = multi_query(args, threads, log)
         out["metric"] = "update edges/s, served multi-query vs plain (exact counts)"
         print(json.dumps(out), flush=True)
+        return
+    if headline:
+        print(json.dumps(config4(args, log)), flush=True)
         return
     t_setup = time.perf_counter()
     spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,

@@ -117,6 +117,18 @@ def host_lib():
         L.cgh_export_li.argtypes = [P, C.c_void_p, U32]
         L.cgh_export_cand.argtypes = [P, U32, U32, U32, C.c_void_p, U32, C.POINTER(C.c_int64)]
         L.cgh_query_info.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(U32), C.POINTER(U32)]
+        L.cgh_workload_open.argtypes = [C.POINTER(SynthSpec), C.POINTER(P)]
+        L.cgh_workload_close.argtypes = [P]
+        L.cgh_workload_close.restype = None
+        L.cgh_workload_graph.argtypes = [P, C.POINTER(U32), C.POINTER(C.POINTER(C.c_int32)),
+                                         C.POINTER(C.POINTER(U32)), C.POINTER(U64)]
+        L.cgh_workload_stream.argtypes = [P, C.POINTER(C.POINTER(C.c_int32)), C.POINTER(U64)]
+        L.cgh_workload_query.argtypes = [P, U32, U32, U64, C.c_void_p, C.c_void_p, C.POINTER(U32), C.POINTER(U32)]
+        L.cgh_planner_open.argtypes = [C.c_void_p, U32, C.c_void_p, U32, U32, C.POINTER(P)]
+        L.cgh_planner_close.argtypes = [P]
+        L.cgh_planner_close.restype = None
+        L.cgh_planner_tables.argtypes = [P, C.POINTER(Plan)]
+        L.cgh_planner_make_plan.argtypes = [P, C.POINTER(U32), C.POINTER(C.c_float), C.POINTER(Plan)]
         _host = L
     return _host
 
@@ -160,6 +172,17 @@ def dev_lib():
         L.cgbdsm_kernel_times.argtypes = [P, C.c_void_p, I]
         L.cgbdsm_batch_enqueue.argtypes = [P, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U64, C.c_void_p, U32]
         L.cgbdsm_batch_stage_next.argtypes = [P, C.POINTER(I)]
+        L.cgbdsm_graph_create.argtypes = [P, U32, C.c_void_p, C.c_void_p, U64, C.POINTER(P)]
+        L.cgbdsm_graph_destroy.argtypes = [P]
+        L.cgbdsm_graph_set_stream.argtypes = [P, C.c_void_p, U64]
+        L.cgbdsm_graph_apply_range.argtypes = [P, U64, U64]
+        L.cgbdsm_graph_info.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
+        L.cgbdsm_calig_attach.argtypes = [P, P, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cgbdsm_calig_update.argtypes = [P]
+        L.cgbdsm_calig_export_lit.argtypes = [P, C.c_void_p, U32]
+        L.cgbdsm_calig_old_ids.argtypes = [P, C.c_void_p, U32, C.POINTER(U32)]
+        L.cgbdsm_calig_stats.argtypes = [P, C.POINTER(C.c_double), C.c_void_p]
+        L.cgbdsm_staged_keys.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
         _dev = L
     return _dev
 
@@ -173,4 +196,7 @@ DEV_SYMBOLS = [
     "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
     "cgbdsm_profile", "cgbdsm_query_set_layout", "cgbdsm_set_dfs_tuning", "cgbdsm_build_frontier",
     "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
+    "cgbdsm_graph_create", "cgbdsm_graph_destroy", "cgbdsm_graph_set_stream", "cgbdsm_graph_apply_range",
+    "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_export_lit",
+    "cgbdsm_calig_old_ids", "cgbdsm_calig_stats", "cgbdsm_staged_keys",
 ]

@@ -11,142 +11,18 @@
 #include <deque>
 #include <new>
 
-#include "engine.h"
-#include "kernels.h"
+#include "api.h"
 
 using namespace cgd;
+using namespace cgapi;
 
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t cap = 0;  // elements
-};
-
-struct cgbdsm_ctx {
-    int device = 0;
-    uint64_t budget = 0;
-    std::atomic<uint64_t> used{0};  // queries of one context may run on different threads
-    int num_cus = 256;
-    int wall_khz = 100000;  // device wall clock (wall_clock64) rate
-};
-
-// Rows left for the final DFS after seeds and BFS levels: count, depth and
-// which of the two frontier buffers holds them.
-struct Frontier {
-    uint64_t n = 0;
-    uint32_t depth = 0, cur = 0;
-};
-
-struct cgbdsm_query {
-    cgbdsm_ctx* ctx = nullptr;
-    hipStream_t s = nullptr;
-    cgbdsm_plan plan{};
-    DevPlan hplan{};
-    DevPlan* d_plan = nullptr;
-    bool have_order = false;
-    uint32_t qv = 0, qe = 0, ndir = 0, ebits = 5, ushift = 59, umask = 0;
-    uint32_t dv = 0, dv_cap = 0, vbits = 1, rbits = 1, W = 0;
-    DBuf<MRec> m_rec;
-    DBuf<URec> u_rec;
-    DBuf<uint32_t> pool;
-    Counters* d_cnt = nullptr;
-    Counters* h_cnt = nullptr;
-    DBuf<uint64_t> k_ins, k_del, k_upd, k_usort, k_alt, seg_start, caps;
-    DBuf<uint32_t> flags, seg_row, sc_val, sc_pos, rs_hist, u_vals, u_rows;
-    DBuf<uint8_t> scan_scratch;
-    DBuf<uint64_t> masks;
-    DBuf<uint32_t> front[2];
-    DBuf<unsigned long long> tag_hist;  // 2 * kMaxQE counters of the seed grouping
-    uint64_t n_ins = 0, n_del = 0, n_upd = 0, u_rows_upper = 0;
-    uint8_t edge_ok[kMaxQE] = {};
-    uint64_t bfs_row_limit = 65536, frontier_cap_rows = 0;
-    uint32_t donate_min = 4;
-    bool dfs_shortest = true, dfs_suffix = true;
-    // Final-DFS launch shape: frontier rows taken per cursor atomic, and
-    // resident blocks per CU (0: as many as fit).  The autotuner's choice for
-    // rows varies with the workload (1 on the scale-22 mix, 2 on the dense
-    // 6-vertex scale-20 stream, 4-8 on the scale-20 mix); the default 4 is
-    // the best untuned value on the mix (2 measured 8 % slower there, 8 %
-    // faster on the dense stream); cgbdsm_set_dfs_tuning changes both.
-    uint32_t dfs_grab = 4, dfs_bpc = 0;
-    double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
-    StealItem* steal_items = nullptr;
-    unsigned long long* steal_seq = nullptr;
-    uint64_t steal_cap = 1ull << 20;
-    uint32_t* h_abort = nullptr;  // coherent pinned host memory, mapped
-    uint32_t* d_abort = nullptr;
-    int phase = 0;  // 0 idle, 1 staged, 2 applied, 3 U built, 4 masks, 5 matched
-    bool u_nonempty = false;
-    uint32_t last_depth = 0, last_buf = 0;
-    // Frontier of the current batch (valid from cgbdsm_build_frontier or the
-    // first count until the next build_local_index).
-    Frontier fr;
-    bool fr_ready = false;
-    // Recorded on this query's stream when its frontier is complete: a
-    // batched launch on another query's stream waits for it.
-    hipEvent_t fr_ev = nullptr;
-    uint64_t last_rows = 0;
-    cgbdsm_stats st{};
-    hipEvent_t ev[8] = {};
-    // kernel-class timing (events on this query's stream)
-    bool profile = false;
-    struct KPending {
-        int cls;
-        hipEvent_t a, b;
-    };
-    std::vector<KPending> kpend;
-    std::vector<hipEvent_t> ev_pool;
-    cgbdsm_ktime kt[CGBDSM_K_COUNT] = {};
-    uint64_t batch_idx_bytes = 0;
-    struct Resident {
-        DBuf<uint64_t> ins, del, upd;
-        uint64_t n_ins = 0, n_del = 0, n_upd = 0;
-        uint8_t edge_ok[kMaxQE] = {};
-        uint32_t dv = 0;
-    };
-    std::deque<Resident> fifo;
-    // final-DFS job table (one entry per query of a launch)
-    DBuf<DfsJob> d_jobs;
-    DBuf<unsigned long long> d_rowbase;
-    std::vector<DfsJob> h_jobs;
-    std::vector<unsigned long long> h_rowbase;
-    // GPMA layout state (packed-memory array of all main-index keys)
-    int layout = CGBDSM_LAYOUT_GAMMA;
-    bool inited = false;
-    struct Pma {
-        DBuf<uint64_t> slot, shadow, lo, S, keys, keys2;
-        DBuf<uint32_t> cnt, ins_cnt, win_of, gpos, wins, leaf, leaf2, keep, pos, chosen;
-        DBuf<PmaWin> meta;
-        uint32_t nleaves = 0, height = 0;
-        uint64_t n = 0;      // keys stored
-        bool stale = false;  // flat lists lag the array (after a commit)
-    } pma;
-};
-
-namespace {
+namespace cgapi {
 
 thread_local std::string g_err;
 
 cgbdsm_status Fail(cgbdsm_status c, const std::string& m) {
     g_err = m;
     return c;
-}
-
-template <class F>
-cgbdsm_status Guard(F&& f) {
-    try {
-        f();
-        g_err.clear();
-        return CGBDSM_OK;
-    } catch (const ApiError& e) {
-        return Fail(e.code, e.what);
-    } catch (const HipError& e) {
-        return Fail(e.e == hipErrorOutOfMemory ? CGBDSM_ERR_OOM : CGBDSM_ERR_DEVICE, e.what);
-    } catch (const std::bad_alloc&) {
-        return Fail(CGBDSM_ERR_OOM, "host allocation failed");
-    } catch (const std::exception& e) {
-        return Fail(CGBDSM_ERR_DEVICE, e.what());
-    }
 }
 
 [[noreturn]] void Throw(cgbdsm_status c, const std::string& m) { throw ApiError{c, m}; }
@@ -188,28 +64,6 @@ void DevFree(cgbdsm_ctx* ctx, void* p, size_t bytes) {
     uint64_t cur = ctx->used.load();
     while (!ctx->used.compare_exchange_weak(cur, cur - std::min<uint64_t>(cur, bytes))) {
     }
-}
-
-template <class T>
-void Ensure(cgbdsm_query* q, DBuf<T>& b, size_t n, bool keep = false, size_t keep_n = 0) {
-    if (n <= b.cap) return;
-    size_t nc = std::max<size_t>({n, b.cap + b.cap / 2, 1024});
-    T* np = (T*)DevAlloc(q->ctx, nc * sizeof(T));
-    if (keep && b.p && keep_n)
-        CGD_HIP(hipMemcpyAsync(np, b.p, std::min(keep_n, b.cap) * sizeof(T), hipMemcpyDeviceToDevice, q->s));
-    if (b.p) {
-        CGD_HIP(hipStreamSynchronize(q->s));
-        DevFree(q->ctx, b.p, b.cap * sizeof(T));
-    }
-    b.p = np;
-    b.cap = nc;
-}
-
-template <class T>
-void Release(cgbdsm_query* q, DBuf<T>& b) {
-    DevFree(q->ctx, b.p, b.cap * sizeof(T));
-    b.p = nullptr;
-    b.cap = 0;
 }
 
 Tables MakeTables(cgbdsm_query* q) {
@@ -695,7 +549,75 @@ void PmaDelete(cgbdsm_query* q, const uint64_t* keys, uint64_t n) {
     KAdd(q, CGBDSM_K_GPMA_UPDATE, 8ull * n, n);
 }
 
-}  // namespace
+void IndexInitDevice(cgbdsm_query* q, uint64_t n, uint32_t dv_count, uint32_t* card, float* avg) {
+    GrowDV(q, std::max<uint32_t>(dv_count, 1));
+    q->dv = dv_count;
+    q->W = (q->dv + 63) / 64;
+    FillU64((uint64_t*)q->m_rec.p, 0, q->m_rec.cap * 2, q->s);
+    FillU64((uint64_t*)q->u_rec.p, 0, q->u_rec.cap, q->s);
+    FillU64((uint64_t*)q->d_cnt, 0, sizeof(Counters) / 8, q->s);
+    q->u_rows_upper = 0;
+    EnsureKeyScratch(q, n);
+    DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, n);
+    LaunchUnique(q->k_alt.p, q->k_ins.p, n, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique), q->s);
+    LaunchSegments(q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p, q->seg_row.p, q->seg_start.p,
+                   Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
+    LaunchInitCaps(q->seg_start.p, Cnt(q, &Counters::n_segs), n, q->caps.p, q->s);
+    // exclusive scan of capacities -> offsets; total into pool_top
+    // (flags hold the head positions needed by the layout kernel)
+    ScanU64(q->caps.p, q->caps.p, std::max<uint64_t>(n, 1), (uint64_t*)Cnt(q, &Counters::pool_top),
+            q->scan_scratch.p, q->s);
+    ReadCounters(q);
+    const uint64_t total = q->h_cnt->pool_top;
+    const uint64_t want = std::max<uint64_t>(total + total / 2 + (16ull << 20), 1ull << 24);
+    if (q->pool.cap < want) {
+        Release(q, q->pool);
+        q->pool.p = (uint32_t*)DevAlloc(q->ctx, want * sizeof(uint32_t));
+        q->pool.cap = want;
+    }
+    LaunchInitLayout(MakeTables(q), q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p,
+                     q->seg_row.p, q->seg_start.p, Cnt(q, &Counters::n_segs), q->caps.p, q->d_cnt, q->s);
+    ReadCounters(q);
+    for (uint32_t e = 0; e < q->ndir; e++) {
+        const uint64_t c = q->h_cnt->card[e], nd = q->h_cnt->nodes[e];
+        if (card) card[e] = (uint32_t)c;
+        if (avg) avg[e] = nd ? (float)(uint32_t)c / (float)(uint32_t)nd : 0.0f;
+    }
+    q->st.main_entries = q->h_cnt->n_unique;
+    if (q->layout == CGBDSM_LAYOUT_GPMA) {
+        // The lists built above give the cardinalities; the matching
+        // lists come from the array from here on.
+        PmaBulk(q, q->k_ins.p, q->h_cnt->n_unique);
+        PmaExtract(q);
+    }
+    q->inited = true;
+    q->phase = 0;
+}
+
+void StageDeviceKeys(cgbdsm_query* q, uint64_t n_ins, uint64_t n_del, uint64_t n_upd, const uint8_t* edge_ok,
+                     uint32_t dv) {
+    if (dv > (1u << (32 - q->ebits)) || dv > q->hplan.v_mask)
+        Throw(CGBDSM_ERR_LIMIT, "compressed vertex count exceeds the key / row-tag width");
+    GrowDV(q, dv);
+    const uint64_t nmax = std::max<uint64_t>({n_ins, n_del, n_upd, 1});
+    Ensure(q, q->k_ins, std::max<uint64_t>(n_ins, 1), true, n_ins);
+    Ensure(q, q->k_del, std::max<uint64_t>(n_del, 1), true, n_del);
+    Ensure(q, q->k_upd, std::max<uint64_t>(n_upd, 1), true, n_upd);
+    Ensure(q, q->k_usort, std::max<uint64_t>(n_upd, 1));
+    Ensure(q, q->u_vals, std::max<uint64_t>(n_upd, 1));
+    Ensure(q, q->u_rows, std::max<uint64_t>(n_upd, 1), true, q->u_rows_upper);
+    EnsureKeyScratch(q, nmax);
+    q->n_ins = n_ins;
+    q->n_del = n_del;
+    q->n_upd = n_upd;
+    memcpy(q->edge_ok, edge_ok, q->qe);
+    q->st.last_ins = n_ins;
+    q->st.last_del = n_del;
+    q->st.last_upd = n_upd;
+    q->phase = 1;
+}
+
+}  // namespace cgapi
 
 extern "C" {
 
@@ -785,6 +707,7 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
     return Guard([&] {
         if (!q) return;
         if (q->s) (void)hipStreamSynchronize(q->s);
+        S1Detach(q);
         Release(q, q->m_rec);
         Release(q, q->u_rec);
         Release(q, q->pool);
@@ -931,53 +854,13 @@ cgbdsm_status cgbdsm_index_init(cgbdsm_query* q, const uint64_t* keys, uint64_t 
                                 uint32_t* cardinalities, float* avg_degrees) {
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (q->s1) Throw(CGBDSM_ERR_INVALID, "the device Stage 1 built this query's index (cgbdsm_calig_attach)");
         if (dv_count > (1u << (32 - q->ebits)) || dv_count > q->hplan.v_mask)
             Throw(CGBDSM_ERR_LIMIT, "compressed vertex count exceeds the key / row-tag width");
         CheckKeys(q, keys, n, dv_count, "index_init");
-        GrowDV(q, std::max<uint32_t>(dv_count, 1));
-        q->dv = dv_count;
-        q->W = (q->dv + 63) / 64;
-        FillU64((uint64_t*)q->m_rec.p, 0, q->m_rec.cap * 2, q->s);
-        FillU64((uint64_t*)q->u_rec.p, 0, q->u_rec.cap, q->s);
-        FillU64((uint64_t*)q->d_cnt, 0, sizeof(Counters) / 8, q->s);
-        q->u_rows_upper = 0;
         Ensure(q, q->k_ins, std::max<uint64_t>(n, 1));
-        EnsureKeyScratch(q, n);
         if (n) CGD_HIP(hipMemcpyAsync(q->k_ins.p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, q->s));
-        DecodeSort(q, q->k_ins.p, q->k_alt.p, q->k_ins.p, n);
-        LaunchUnique(q->k_alt.p, q->k_ins.p, n, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique), q->s);
-        LaunchSegments(q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p, q->seg_row.p, q->seg_start.p,
-                       Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
-        LaunchInitCaps(q->seg_start.p, Cnt(q, &Counters::n_segs), n, q->caps.p, q->s);
-        // exclusive scan of capacities -> offsets; total into pool_top
-        // (flags hold the head positions needed by the layout kernel)
-        ScanU64(q->caps.p, q->caps.p, std::max<uint64_t>(n, 1), (uint64_t*)Cnt(q, &Counters::pool_top),
-                q->scan_scratch.p, q->s);
-        ReadCounters(q);
-        const uint64_t total = q->h_cnt->pool_top;
-        const uint64_t want = std::max<uint64_t>(total + total / 2 + (16ull << 20), 1ull << 24);
-        if (q->pool.cap < want) {
-            Release(q, q->pool);
-            q->pool.p = (uint32_t*)DevAlloc(q->ctx, want * sizeof(uint32_t));
-            q->pool.cap = want;
-        }
-        LaunchInitLayout(MakeTables(q), q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p,
-                         q->seg_row.p, q->seg_start.p, Cnt(q, &Counters::n_segs), q->caps.p, q->d_cnt, q->s);
-        ReadCounters(q);
-        for (uint32_t e = 0; e < q->ndir; e++) {
-            const uint64_t c = q->h_cnt->card[e], nd = q->h_cnt->nodes[e];
-            if (cardinalities) cardinalities[e] = (uint32_t)c;
-            if (avg_degrees) avg_degrees[e] = nd ? (float)(uint32_t)c / (float)(uint32_t)nd : 0.0f;
-        }
-        q->st.main_entries = q->h_cnt->n_unique;
-        if (q->layout == CGBDSM_LAYOUT_GPMA) {
-            // The lists built above give the cardinalities; the matching
-            // lists come from the array from here on.
-            PmaBulk(q, q->k_ins.p, q->h_cnt->n_unique);
-            PmaExtract(q);
-        }
-        q->inited = true;
-        q->phase = 0;
+        IndexInitDevice(q, n, dv_count, cardinalities, avg_degrees);
     });
 }
 
@@ -987,6 +870,7 @@ cgbdsm_status cgbdsm_batch_stage(cgbdsm_query* q, const uint64_t* ins, uint64_t 
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
         if (!q->have_order) Throw(CGBDSM_ERR_INVALID, "cgbdsm_query_set_plan must run before the first batch");
+        if (q->s1) Throw(CGBDSM_ERR_INVALID, "this query's batches come from cgbdsm_calig_update");
         CheckCanStage(q);
         if (!edge_ok) Throw(CGBDSM_ERR_INVALID, "null edge_ok");
         if (dv_count < q->dv) Throw(CGBDSM_ERR_INVALID, "compressed vertex count cannot shrink");
@@ -1024,6 +908,7 @@ cgbdsm_status cgbdsm_batch_enqueue(cgbdsm_query* q, const uint64_t* ins, uint64_
                                    uint32_t dv_count) {
     return Guard([&] {
         if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        if (q->s1) Throw(CGBDSM_ERR_INVALID, "this query's batches come from cgbdsm_calig_update");
         if (!edge_ok) Throw(CGBDSM_ERR_INVALID, "null edge_ok");
         const uint32_t prev = q->fifo.empty() ? q->dv : q->fifo.back().dv;
         if (dv_count < prev) Throw(CGBDSM_ERR_INVALID, "compressed vertex count cannot shrink");
@@ -1320,7 +1205,8 @@ const Frontier& BuildFrontier(cgbdsm_query* q) {
 // Zeroes the per-edge counts (and the frontier/tuple flags) of `q` on the
 // stream `s` that will run the final DFS.
 void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
-    FillU64((uint64_t*)&q->d_cnt->per_edge[0], 0, kMaxQE, s);
+    static_assert(offsetof(Counters, per_edge) == offsetof(Counters, count_ovf) + 8, "count_ovf precedes per_edge");
+    FillU64((uint64_t*)&q->d_cnt->count_ovf, 0, 1 + kMaxQE, s);
     FillU64((uint64_t*)&q->d_cnt->tuples, 0, 1, s);
 }
 
@@ -1343,6 +1229,16 @@ void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
 
 // Per-query results of a completed final DFS; `launch` holds the counters of
 // the launch's work queue (the query itself for a single-query launch).
+// A query whose batch count did not fit 64 bits (the count is then wrong:
+// the call fails instead of returning it).
+void CheckWide(cgbdsm_query* const* qs, int nq) {
+    std::string names;
+    for (int i = 0; i < nq; i++)
+        if (qs[i]->count_wide) names += (names.empty() ? "" : ", ") + std::to_string(i);
+    if (!names.empty())
+        Throw(CGBDSM_ERR_OVERFLOW, "the batch's match count does not fit 64 bits (query " + names + " of the call)");
+}
+
 void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* launch, unsigned long long* num_matches,
                     uint64_t* per_edge) {
     const Counters* c = launch->h_cnt;
@@ -1354,10 +1250,14 @@ void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* laun
     q->st.last_dfs_batched = c->batched;
     q->st.last_dfs_busy = c->wave_ticks ? 1.0 - (double)c->idle_ticks / (double)c->wave_ticks : 0.0;
     unsigned long long total = 0;
+    bool wide = q->h_cnt->count_ovf != 0;
     for (uint32_t e = 0; e < q->qe; e++) {
-        total += q->h_cnt->per_edge[e];
-        if (per_edge) per_edge[e] = q->h_cnt->per_edge[e];
+        const unsigned long long pe = q->h_cnt->per_edge[e];
+        wide |= total + pe < total;
+        total += pe;
+        if (per_edge) per_edge[e] = pe;
     }
+    q->count_wide = wide;
     if (num_matches) *num_matches += total;
     q->last_depth = f.depth;
     q->last_rows = f.n;
@@ -1385,6 +1285,7 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
         RunDfs(q, q->front[f.cur].p, f.n, f.depth, nullptr, 0);
         CheckDfs(q, f.n);
         FinishMatching(q, f, q, num_matches, per_edge);
+        CheckWide(&q, 1);
     });
 }
 
@@ -1430,6 +1331,7 @@ cgbdsm_status cgbdsm_matching_batch(cgbdsm_query* const* qs, int nq, unsigned lo
             FinishMatching(qs[i], fr[i], lead, num_matches ? &num_matches[i] : nullptr,
                            per_edge ? per_edge[i] : nullptr);
         }
+        CheckWide(qs, nq);
     });
 }
 
@@ -1575,6 +1477,20 @@ cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uin
         std::sort(h.begin(), h.end());
         if (n_out) *n_out = total;
         if (out) memcpy(out, h.data(), std::min(cap, total) * 8);
+    });
+}
+
+cgbdsm_status cgbdsm_staged_keys(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out) {
+    return Guard([&] {
+        if (!q || which < 0 || which > 2) Throw(CGBDSM_ERR_INVALID, "bad query or key selector");
+        if (q->phase != 1) Throw(CGBDSM_ERR_INVALID, "no batch is staged");
+        const DBuf<uint64_t>& b = which == 0 ? q->k_ins : which == 1 ? q->k_del : q->k_upd;
+        const uint64_t n = which == 0 ? q->n_ins : which == 1 ? q->n_del : q->n_upd;
+        if (n_out) *n_out = n;
+        if (out && cap && n) {
+            CGD_HIP(hipMemcpyAsync(out, b.p, std::min(cap, n) * 8, hipMemcpyDeviceToHost, q->s));
+            CGD_HIP(hipStreamSynchronize(q->s));
+        }
     });
 }
 

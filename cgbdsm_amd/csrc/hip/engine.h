@@ -50,7 +50,7 @@ struct alignas(128) Counters {
     unsigned long long u_rows;        // touched rows of the current U
     unsigned long long step_lanes;    // candidates covered by those steps (active lanes)
     unsigned long long batched;       // steps whose product suffixes were counted together
-    unsigned long long unused_q[1];
+    unsigned long long count_ovf;     // final DFS: a count did not fit 64 bits (precedes per_edge)
     unsigned long long per_edge[kMaxQE];
     unsigned long long card[kMaxDir], nodes[kMaxDir];
     unsigned long long tuples;

@@ -529,6 +529,19 @@ struct DfsLevel {
     uint32_t sel;                 // backward neighbour whose list is scanned (kNoSel: not chosen yet)
 };
 
+// Counts saturate at kCountSat: a count that does not fit 64 bits stays
+// there through sums and products (a product with an empty factor is 0),
+// and the host turns it into CGBDSM_ERR_OVERFLOW instead of a wrapped value.
+constexpr unsigned long long kCountSat = ~0ull;
+__device__ __forceinline__ unsigned long long SatAdd(unsigned long long a, unsigned long long b) {
+    const unsigned long long r = a + b;
+    return r < a ? kCountSat : r;
+}
+__device__ __forceinline__ unsigned long long SatMul(unsigned long long a, unsigned long long b) {
+    unsigned long long r;
+    return __builtin_mul_overflow(a, b, &r) ? kCountSat : r;
+}
+
 constexpr uint32_t kFull = 0xFFFFFFFFu;
 constexpr uint32_t kNoSel = 0xFFu;
 constexpr uint32_t kNoSkip = 0xFFu;
@@ -890,7 +903,7 @@ __device__ unsigned long long SuffixCount(const Ctx& X, const Tables& T, uint32_
             nc = (__int128)v[1] * v[2] * v[4] - (__int128)v[3] * v[4] - (__int128)v[5] * v[2] - (__int128)v[6] * v[1] +
                  2 * (__int128)v[7];
         if (nc <= 0) return 0;
-        result *= (unsigned long long)nc;
+        result = SatMul(result, (nc >> 64) ? kCountSat : (unsigned long long)nc);
     }
     return result;
 }
@@ -957,7 +970,7 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
                 for (uint32_t k = 0; k < nbn && in; k++) in = InList(BnList(X, T, j, k, matched), x, probes);
                 val -= in ? 1 : 0;
             }
-            prod = val > 0 ? prod * (unsigned long long)val : 0ull;
+            prod = val > 0 ? SatMul(prod, (unsigned long long)val) : 0ull;
             continue;
         }
         // j's list from p, per sibling
@@ -1072,10 +1085,10 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
             if (same && member(x)) hits++;
         }
         const long long val = (long long)raw - (long long)hits;
-        prod = val > 0 ? prod * (unsigned long long)val : 0ull;
+        prod = val > 0 ? SatMul(prod, (unsigned long long)val) : 0ull;
     }
     unsigned long long sum = prod;
-    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+    for (int o = 32; o > 0; o >>= 1) sum = SatAdd(sum, __shfl_xor(sum, o, kWave));
     return Uniform64(sum);
 }
 
@@ -1132,7 +1145,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         if (!jc) return;
         for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave)
             if (pe[t]) {
-                atomicAdd(&jc->per_edge[t], pe[t]);
+                const unsigned long long old = atomicAdd(&jc->per_edge[t], pe[t]);
+                if (pe[t] == kCountSat || old + pe[t] < old) atomicOr(&jc->count_ovf, 1ull);
                 pe[t] = 0;
             }
         WaveSync();
@@ -1318,7 +1332,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             }
             // The product suffix is counted, not enumerated.
             if (cart && d >= cps) {
-                count += SuffixCount(X, T, d, matched, s_csize[wv], cvalid, sc, my_scanned, my_probes);
+                count = SatAdd(count, SuffixCount(X, T, d, matched, s_csize[wv], cvalid, sc, my_scanned, my_probes));
                 if (d == D0) break;
                 d--;
                 continue;
@@ -1458,8 +1472,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             // every survivor of this step at once instead of descending.
             if (blev) {
                 if (m) {
-                    count += BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv], my_scanned,
-                                         my_probes);
+                    count = SatAdd(count, BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv],
+                                                      my_scanned, my_probes));
                     batched++;
                 }
                 if (lane == 0) lvl[d].off = off + kWave;
@@ -1467,7 +1481,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 continue;
             }
             if (d == last) {
-                count += (unsigned long long)__popcll(m);
+                count = SatAdd(count, (unsigned long long)__popcll(m));
                 if (kTuples && ok) {
                     unsigned long long slot = atomicAdd(&cnt->tuples, 1ull);
                     if (slot < tup_cap) {
@@ -1493,7 +1507,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             WaveSync();
         }
         if (lane == 0) {
-            if (count) pe[ei] += count;
+            if (count) pe[ei] = SatAdd(pe[ei], count);
             // Rows retire from `pending` a whole grab at a time, stolen items one by one.
             if (!from_row) atomicSub(&cnt->q_pending.v, 1ull);
             else if (r_next >= r_end) atomicSub(&cnt->q_pending.v, (unsigned long long)(r_end - r_first));

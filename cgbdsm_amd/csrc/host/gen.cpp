@@ -116,24 +116,35 @@ SynthGraph GenerateRmat(const RmatSpec& sp, ThreadPool* pool) {
     return g;
 }
 
-SampledQuery SampleQuery(const SynthGraph& g, uint32_t size, QueryKind kind, uint64_t seed) {
-    if (size < 2 || size > kMaxQV) throw Error("query size out of range");
+QuerySampler::QuerySampler(const SynthGraph& g) : g_(g) {
     // Adjacency of the final graph.
-    std::vector<uint32_t> deg(g.n + 1, 0);
+    deg_.assign(g.n + 1, 0);
     for (auto* set : {&g.initial, &g.stream})
         for (auto& [a, b] : *set) {
-            deg[a]++;
-            deg[b]++;
+            deg_[a]++;
+            deg_[b]++;
         }
-    std::vector<uint64_t> off(g.n + 1, 0);
-    for (uint32_t v = 0; v < g.n; v++) off[v + 1] = off[v] + deg[v];
-    std::vector<uint32_t> adj(off[g.n]);
-    std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+    off_.assign(g.n + 1, 0);
+    for (uint32_t v = 0; v < g.n; v++) off_[v + 1] = off_[v] + deg_[v];
+    adj_.resize(off_[g.n]);
+    std::vector<uint64_t> pos(off_.begin(), off_.end() - 1);
     for (auto* set : {&g.initial, &g.stream})
         for (auto& [a, b] : *set) {
-            adj[pos[a]++] = b;
-            adj[pos[b]++] = a;
+            adj_[pos[a]++] = b;
+            adj_[pos[b]++] = a;
         }
+}
+
+SampledQuery SampleQuery(const SynthGraph& g, uint32_t size, QueryKind kind, uint64_t seed) {
+    return QuerySampler(g).Sample(size, kind, seed);
+}
+
+SampledQuery QuerySampler::Sample(uint32_t size, QueryKind kind, uint64_t seed) const {
+    if (size < 2 || size > kMaxQV) throw Error("query size out of range");
+    const SynthGraph& g = g_;
+    const auto& deg = deg_;
+    const auto& off = off_;
+    const auto& adj = adj_;
     auto adjacent = [&](uint32_t a, uint32_t b) {
         for (uint64_t i = off[a]; i < off[a + 1]; i++)
             if (adj[i] == b) return true;

@@ -53,6 +53,19 @@ struct SampledQuery {
 // stream edges; retries with derived seeds until the kind's rule holds.
 SampledQuery SampleQuery(const SynthGraph& g, uint32_t size, QueryKind kind, uint64_t seed);
 
+// The same sampler with the graph's adjacency built once (many queries).
+class QuerySampler {
+public:
+    explicit QuerySampler(const SynthGraph& g);
+    SampledQuery Sample(uint32_t size, QueryKind kind, uint64_t seed) const;
+
+private:
+    const SynthGraph& g_;
+    std::vector<uint32_t> deg_;
+    std::vector<uint64_t> off_;
+    std::vector<uint32_t> adj_;
+};
+
 void WriteDataGraph(const std::string& path, const SynthGraph& g);
 void WriteStream(const std::string& path, const SynthGraph& g);
 // The whole graph (initial and stream edges) as the binary CSR prefix read

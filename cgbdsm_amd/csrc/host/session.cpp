@@ -353,3 +353,115 @@ int cgh_query_info(cgh_session* s, int32_t* labels, uint32_t* edges_uv, uint32_t
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------- workloads and planners
+
+struct cgh_workload {
+    cg::SynthGraph g;
+    std::unique_ptr<cg::QuerySampler> sampler;
+    std::vector<uint32_t> edges;  // initial edges as u32 pairs
+    std::vector<int32_t> ints;    // the stream as ints, two per pair
+};
+
+using EdgeList = std::vector<std::pair<uint32_t, uint32_t>>;
+
+struct cgh_planner {
+    cg::Query q;
+    std::unique_ptr<cg::Plan> plan;
+    uint32_t ebits = 5;
+};
+
+extern "C" {
+
+int cgh_workload_open(const cgh_synth_spec* spec, cgh_workload** out) {
+    CGH_TRY({
+        auto* w = new cgh_workload;
+        try {
+            cg::ThreadPool pool(cg::DefaultThreads());
+            w->g = cg::GenerateRmat(ToRmat(spec), &pool);
+            w->edges.reserve(2 * w->g.initial.size());
+            for (const auto& e : w->g.initial) {
+                w->edges.push_back(e.first);
+                w->edges.push_back(e.second);
+            }
+            w->ints.reserve(2 * w->g.stream.size());
+            for (const auto& e : w->g.stream) {
+                w->ints.push_back((int32_t)e.first);
+                w->ints.push_back((int32_t)e.second);
+            }
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        *out = w;
+    })
+}
+
+void cgh_workload_close(cgh_workload* w) { delete w; }
+
+int cgh_workload_graph(cgh_workload* w, uint32_t* n, const int32_t** labels, const uint32_t** edges, uint64_t* m) {
+    CGH_TRY({
+        *n = w->g.n;
+        *labels = w->g.labels.data();
+        *edges = w->edges.data();
+        *m = w->edges.size() / 2;
+    })
+}
+
+int cgh_workload_stream(cgh_workload* w, const int32_t** ints, uint64_t* n_ints) {
+    CGH_TRY({
+        *ints = w->ints.data();
+        *n_ints = w->ints.size();
+    })
+}
+
+int cgh_workload_query(cgh_workload* w, uint32_t size, uint32_t kind, uint64_t seed, int32_t* labels,
+                       uint32_t* edges_uv, uint32_t* qv, uint32_t* qe) {
+    CGH_TRY({
+        if (!w->sampler) w->sampler = std::make_unique<cg::QuerySampler>(w->g);
+        cg::SampledQuery sq = w->sampler->Sample(size, (cg::QueryKind)kind, seed);
+        cg::Query q = cg::QueryFromEdges(sq.labels, sq.edges);
+        *qv = q.vcount;
+        *qe = q.ecount;
+        for (uint32_t u = 0; u < q.vcount; u++) labels[u] = q.raw_labels[u];
+        for (uint32_t e = 0; e < q.ecount; e++) {
+            edges_uv[2 * e] = q.qe_list[e].first;
+            edges_uv[2 * e + 1] = q.qe_list[e].second;
+        }
+    })
+}
+
+int cgh_planner_open(const int32_t* labels, uint32_t qv, const uint32_t* edges_uv, uint32_t qe, uint32_t ebits,
+                     cgh_planner** out) {
+    CGH_TRY({
+        if (ebits != 5 && ebits != 7) throw cg::Error("ebits must be 5 or 7");
+        if (2 * qe > (1u << ebits)) throw cg::Error("query has too many edges for the key format");
+        std::vector<int> lab(labels, labels + qv);
+        EdgeList ed;
+        for (uint32_t e = 0; e < qe; e++) ed.push_back(std::make_pair(edges_uv[2 * e], edges_uv[2 * e + 1]));
+        auto* p = new cgh_planner;
+        try {
+            p->q = cg::QueryFromEdges(lab, ed);
+            p->plan = std::make_unique<cg::Plan>(p->q);
+            p->plan->GenerateIndexingOrders();
+            p->ebits = ebits;
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    })
+}
+
+void cgh_planner_close(cgh_planner* p) { delete p; }
+
+int cgh_planner_tables(cgh_planner* p, cgbdsm_plan* out) { CGH_TRY(cg::FillQueryTables(p->q, p->plan.get(), p->ebits, out)) }
+
+int cgh_planner_make_plan(cgh_planner* p, const uint32_t* card, const float* avg, cgbdsm_plan* out) {
+    CGH_TRY({
+        p->plan->GenerateMatchingOrders(card, avg);
+        p->plan->Fill(out, p->ebits);
+    })
+}
+
+}  // extern "C"

@@ -56,6 +56,10 @@ class Engine:
         _check(self.L.cgbdsm_ctx_memory(self.h, C.byref(used), C.byref(budget)))
         return int(used.value), int(budget.value)
 
+    def graph(self, labels, edges):
+        """Device data graph shared by this engine's queries (device Stage 1)."""
+        return DeviceGraph(self, labels, edges)
+
     def matching_batch(self, queries):
         """Matching of several queries of this engine (each after
         build_local_index) with one final-DFS launch; returns a list of
@@ -71,6 +75,43 @@ class Engine:
     def close(self):
         if self.h:
             _check(self.L.cgbdsm_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """CaLiG's data graph in HBM (cgbdsm_graph_*): labels, adjacency, the
+    update stream and the latest batch's new edges."""
+
+    def __init__(self, engine, labels, edges):
+        self.eng, self.L = engine, engine.L
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        edges = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        self.n = int(labels.size)
+        self.h = C.c_void_p()
+        _check(self.L.cgbdsm_graph_create(engine.h, self.n, labels.ctypes.data, _ptr(edges), edges.shape[0],
+                                          C.byref(self.h)))
+
+    def set_stream(self, ints):
+        ints = np.ascontiguousarray(np.asarray(ints).reshape(-1), dtype=np.int32)
+        _check(self.L.cgbdsm_graph_set_stream(self.h, _ptr(ints), ints.size))
+
+    def apply_range(self, start, count):
+        _check(self.L.cgbdsm_graph_apply_range(self.h, int(start), int(count)))
+
+    def info(self):
+        b, m, nn = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self.L.cgbdsm_graph_info(self.h, C.byref(b), C.byref(m), C.byref(nn)))
+        return {"batches": b.value, "directed_edges": m.value, "last_new": nn.value}
+
+    def close(self):
+        if self.h:
+            _check(self.L.cgbdsm_graph_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -109,6 +150,49 @@ class DeviceQuery:
         avg = np.zeros(N.MAX_DIR, dtype=np.float32)
         _check(self.L.cgbdsm_index_init(self.h, _ptr(keys), keys.size, dv, card.ctypes.data, avg.ctypes.data))
         return card[: self.ndir], avg[: self.ndir]
+
+    def calig_attach(self, graph, qlabels):
+        """Device Stage 1: CaLiG index of this query over `graph`, then the
+        initial main index; returns (cardinalities, avg_degrees)."""
+        ql = np.zeros(N.MAX_QV, dtype=np.int32)
+        ql[: len(qlabels)] = qlabels
+        card = np.zeros(N.MAX_DIR, dtype=np.uint32)
+        avg = np.zeros(N.MAX_DIR, dtype=np.float32)
+        _check(self.L.cgbdsm_calig_attach(self.h, graph.h, ql.ctypes.data, card.ctypes.data, avg.ctypes.data))
+        self.graph = graph
+        return card[: self.ndir], avg[: self.ndir]
+
+    def calig_update(self):
+        """Device Stage 1 of the graph's latest batch; stages it."""
+        _check(self.L.cgbdsm_calig_update(self.h))
+
+    def calig_lit(self):
+        li = np.zeros(self.graph.n, dtype=np.uint32)
+        _check(self.L.cgbdsm_calig_export_lit(self.h, li.ctypes.data, self.graph.n))
+        return li
+
+    def calig_old_ids(self):
+        n = C.c_uint32()
+        _check(self.L.cgbdsm_calig_old_ids(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        _check(self.L.cgbdsm_calig_old_ids(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out[: n.value]
+
+    def calig_stats(self):
+        ms = C.c_double()
+        o = np.zeros(6, dtype=np.uint64)
+        _check(self.L.cgbdsm_calig_stats(self.h, C.byref(ms), o.ctypes.data))
+        return {"ms": ms.value, "rounds_act": int(o[0]), "rounds_del": int(o[1]), "activated": int(o[2]),
+                "killed": int(o[3]), "new_pairs": int(o[4]), "dv": int(o[5])}
+
+    def staged_keys(self, which):
+        """Keys of the staged batch: 'ins', 'del' or 'upd' (unsorted)."""
+        w = {"ins": 0, "del": 1, "upd": 2}[which]
+        n = C.c_uint64()
+        _check(self.L.cgbdsm_staged_keys(self.h, w, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(self.L.cgbdsm_staged_keys(self.h, w, out.ctypes.data, n.value, C.byref(n)))
+        return out[: n.value]
 
     def stage(self, kb):
         ins = np.ascontiguousarray(kb.ins, dtype=np.uint64)

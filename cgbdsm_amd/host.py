@@ -182,3 +182,79 @@ class HostSession:
         _check(self._L.cgh_query_info(self._h, lab.ctypes.data, ed.ctypes.data, C.byref(qv), C.byref(qe)),
                self._L)
         return lab[: qv.value].copy(), ed[: 2 * qe.value].copy()
+
+
+class Workload:
+    """A synthetic graph and its update stream without any host CaLiG (for
+    the device Stage 1): the same graph a HostSession of the spec builds."""
+
+    def __init__(self, spec):
+        L = self._L = N.host_lib()
+        self._h = C.c_void_p()
+        _check(L.cgh_workload_open(C.byref(spec), C.byref(self._h)), L)
+        n, m = C.c_uint32(), C.c_uint64()
+        lab = C.POINTER(C.c_int32)()
+        ed = C.POINTER(C.c_uint32)()
+        _check(L.cgh_workload_graph(self._h, C.byref(n), C.byref(lab), C.byref(ed), C.byref(m)), L)
+        self.n = n.value
+        self.labels = _arr(lab, n.value, np.int32)
+        self.edges = _arr(ed, 2 * m.value, np.uint32).reshape(-1, 2)
+        ints = C.POINTER(C.c_int32)()
+        ni = C.c_uint64()
+        _check(L.cgh_workload_stream(self._h, C.byref(ints), C.byref(ni)), L)
+        self.stream = _arr(ints, ni.value, np.int32)
+
+    def query(self, size, kind, seed):
+        """(labels, edges) of a seeded random-walk query of `size` vertices."""
+        lab = np.zeros(N.MAX_QV, dtype=np.int32)
+        ed = np.zeros(2 * N.MAX_QE, dtype=np.uint32)
+        qv, qe = C.c_uint32(), C.c_uint32()
+        _check(self._L.cgh_workload_query(self._h, size, QUERY_KINDS[kind], seed, lab.ctypes.data, ed.ctypes.data,
+                                          C.byref(qv), C.byref(qe)), self._L)
+        return lab[: qv.value].copy(), ed[: 2 * qe.value].reshape(-1, 2).copy()
+
+    def close(self):
+        if self._h:
+            self._L.cgh_workload_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Planner:
+    """The planner of one query given by labels and edges (no CaLiG)."""
+
+    def __init__(self, labels, edges, ebits=5):
+        L = self._L = N.host_lib()
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        edges = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1))
+        self._h = C.c_void_p()
+        _check(L.cgh_planner_open(labels.ctypes.data, labels.size, edges.ctypes.data, edges.size // 2, ebits,
+                                  C.byref(self._h)), L)
+        self.labels = labels
+        self.ebits = ebits
+        self.query_tables = N.Plan()
+        _check(L.cgh_planner_tables(self._h, C.byref(self.query_tables)), L)
+        self.qv, self.qe = self.query_tables.qv, self.query_tables.qe
+
+    def make_plan(self, card, avg):
+        p = N.Plan()
+        c = (C.c_uint32 * N.MAX_DIR)(*[int(x) for x in card])
+        a = (C.c_float * N.MAX_DIR)(*[float(x) for x in avg])
+        _check(self._L.cgh_planner_make_plan(self._h, c, a, C.byref(p)), self._L)
+        return p
+
+    def close(self):
+        if self._h:
+            self._L.cgh_planner_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

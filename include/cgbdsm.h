@@ -323,6 +323,68 @@ cgbdsm_status cgbdsm_run_batch(cgbdsm_query* q, unsigned long long* num_matches,
 /* Synchronise the query's stream. */
 cgbdsm_status cgbdsm_sync(cgbdsm_query* q);
 
+/* ------------------------------------------------------------------------
+ * Device Stage 1: the CaLiG producer on the GPU.
+ *
+ * One data graph per context is shared by every query attached to it (the
+ * reference builds one CaLiG, with its own copy of G, per process and query,
+ * main_calig.cpp:29-35).  Per batch: cgbdsm_graph_apply_range once, then
+ * cgbdsm_calig_update for every attached query, which stages that query's
+ * batch exactly as cgbdsm_batch_stage would stage the host producer's keys;
+ * the Stage-2 calls follow unchanged.  Lit bits equal the greatest fixpoint
+ * of the CaLiG check on the current graph; keys use the same boundary format
+ * and compressed ids (ascending original id among newly lit vertices).
+ * Insertion streams only: pairs with a negative id are skipped, as the host
+ * producer skips them.
+ * ------------------------------------------------------------------------ */
+typedef struct cgbdsm_graph cgbdsm_graph;
+
+/* CaLiG::inputG's G (labels, adjacency; index/calig.cpp:166-216) on the
+ * device: n vertices with raw labels[n], m undirected edges as u32 pairs
+ * edges[2*m] (self-loops, out-of-range ids and duplicates dropped).
+ * n < 2^28. */
+cgbdsm_status cgbdsm_graph_create(cgbdsm_ctx* ctx, uint32_t n, const int32_t* labels, const uint32_t* edges,
+                                  uint64_t m, cgbdsm_graph** out);
+/* Fails while queries are attached (destroy them first). */
+cgbdsm_status cgbdsm_graph_destroy(cgbdsm_graph* g);
+/* The update stream as CaLiG::inputUpdate holds it (index/calig.cpp:719-733):
+ * n_ints ints, two per pair, copied into HBM once. */
+cgbdsm_status cgbdsm_graph_set_stream(cgbdsm_graph* g, const int32_t* ints, uint64_t n_ints);
+/* G.nei inserts of CaLiG::updateIndex for stream ints [start, start+count)
+ * (index/calig.cpp:950-1003; pairs are ints start+2i, start+2i+1 as
+ * main_calig.cpp:220-222 slices them, so an odd count shifts the pairing of
+ * later batches as in the reference).  Every attached query must have
+ * consumed the previous batch. */
+cgbdsm_status cgbdsm_graph_apply_range(cgbdsm_graph* g, uint64_t start, uint64_t count);
+/* Batches applied, directed adjacency entries, new directed edges of the
+ * latest batch. */
+cgbdsm_status cgbdsm_graph_info(cgbdsm_graph* g, uint64_t* batches, uint64_t* directed_edges, uint64_t* last_new);
+
+/* CaLiG::constructCand + staticFilter (index/calig.cpp:325-356, 705-717)
+ * for query q over g, then the initial main index: ConvertGlobalIndex
+ * (index/calig_helper_gamma.cu:238-360), as cgbdsm_index_init with the
+ * host keys.  qlabels[qv] are the query's raw labels; cardinalities /
+ * avg_degrees as cgbdsm_index_init.  Call after cgbdsm_query_create (and
+ * cgbdsm_query_set_layout), before cgbdsm_query_set_plan. */
+cgbdsm_status cgbdsm_calig_attach(cgbdsm_query* q, cgbdsm_graph* g, const int32_t* qlabels, uint32_t* cardinalities,
+                                  float* avg_degrees);
+/* Stage 1 of the graph's latest batch for q: CaLiG::updateIndex,
+ * ConstructUpdate, mergeDeltas (index/calig.cpp:950-1088) and
+ * PrepareUpdateAll / PrepareUpdateIndex (index/calig_helper_gamma.cu:363-445),
+ * ending with the batch staged (as cgbdsm_batch_stage). */
+cgbdsm_status cgbdsm_calig_update(cgbdsm_query* q);
+/* Lit bitmask per original vertex (n = the graph's vertex count). */
+cgbdsm_status cgbdsm_calig_export_lit(cgbdsm_query* q, uint32_t* li, uint32_t n);
+/* Compressed id -> original vertex id; *n_out = compressed vertices. */
+cgbdsm_status cgbdsm_calig_old_ids(cgbdsm_query* q, uint32_t* ids, uint32_t cap, uint32_t* n_out);
+/* Last cgbdsm_calig_attach / _update: wall ms, and out6 = activation rounds,
+ * deletion rounds, pairs lit optimistically, pairs removed, new lit pairs,
+ * compressed vertices. */
+cgbdsm_status cgbdsm_calig_stats(cgbdsm_query* q, double* ms, uint64_t* out6);
+/* Keys of the staged batch in boundary format: which = 0 ins, 1 del,
+ * 2 upd (as staged, unsorted); *n_out = count. */
+cgbdsm_status cgbdsm_staged_keys(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

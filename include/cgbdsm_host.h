@@ -109,6 +109,36 @@ int cgh_export_cand(cgh_session* s, uint32_t v, uint32_t ui, uint32_t uj, uint32
 /* CaLiG's query: raw labels[qv], edges as (u,uu) pairs in qe_list order. */
 int cgh_query_info(cgh_session* s, int32_t* labels, uint32_t* edges_uv, uint32_t* qv, uint32_t* qe);
 
+/* ------------------------------------------------------------------------
+ * Workloads and planners for the device Stage 1 (cgbdsm_calig_*), which
+ * needs no host CaLiG: the synthetic graph and stream of a spec (the same
+ * graph cgh_session_open_synth builds), random-walk queries sampled on it,
+ * and the planner of a query given by labels and edges.
+ * ------------------------------------------------------------------------ */
+typedef struct cgh_workload cgh_workload;
+typedef struct cgh_planner cgh_planner;
+
+int cgh_workload_open(const cgh_synth_spec* spec, cgh_workload** out);
+void cgh_workload_close(cgh_workload* w);
+/* n vertices, raw labels[n], initial edges as m u32 pairs; pointers stay
+ * valid until the workload closes. */
+int cgh_workload_graph(cgh_workload* w, uint32_t* n, const int32_t** labels, const uint32_t** edges, uint64_t* m);
+/* The update stream as ints, two per pair, in stream order. */
+int cgh_workload_stream(cgh_workload* w, const int32_t** ints, uint64_t* n_ints);
+/* A query of `size` vertices and kind CGH_QUERY_* sampled by a seeded random
+ * walk on the whole graph (initial and stream edges): raw labels[qv] and
+ * edges as (u, uu) pairs in qe_list order. */
+int cgh_workload_query(cgh_workload* w, uint32_t size, uint32_t kind, uint64_t seed, int32_t* labels,
+                       uint32_t* edges_uv, uint32_t* qv, uint32_t* qe);
+
+/* Planner (Plan, graph/plan.h:38-71) of a query given by labels and edges. */
+int cgh_planner_open(const int32_t* labels, uint32_t qv, const uint32_t* edges_uv, uint32_t qe, uint32_t ebits,
+                     cgh_planner** out);
+void cgh_planner_close(cgh_planner* p);
+/* As cgh_query_tables / cgh_make_plan. */
+int cgh_planner_tables(cgh_planner* p, cgbdsm_plan* out);
+int cgh_planner_make_plan(cgh_planner* p, const uint32_t* card, const float* avg, cgbdsm_plan* out);
+
 #ifdef __cplusplus
 }
 #endif

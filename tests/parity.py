@@ -118,3 +118,105 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
     res["stats"] = q.stats()
     q.close()
     return res
+
+
+def run_stream_s1(host, eng, batch_ints, *, layout=None, check_masks=True, check_li=False, max_batches=None,
+                  log=None):
+    """The same stream with Stage 1 on the device (cgbdsm_calig_*): per batch
+    the device lit bits, compressed ids, staged update keys and edge_ok
+    equal the host producer's, and the device index, masks and counts equal
+    the oracle fed with the host producer's keys.  check_li also compares
+    the lit bits with the oracle's from-scratch fixpoint on the current
+    graph."""
+    keys, dv, card, avg = host.initial_keys()
+    offs, adj, lab = host.graph()
+    ql, qe = host.query_info()
+    n = lab.size
+    edges = np.stack([np.repeat(np.arange(n, dtype=np.uint32), np.diff(offs).astype(np.int64)),
+                      adj[: int(offs[-1])]], axis=1)
+    edges = edges[edges[:, 0] < edges[:, 1]]
+    g = eng.graph(lab, edges)
+    stream = host.update_stream().astype(np.int32).reshape(-1)
+    g.set_stream(stream)
+    q = eng.query(host.query_tables)
+    if layout:
+        q.set_layout(layout)
+    dcard, davg = q.calig_attach(g, ql)
+    res = {"batches": 0, "total": 0, "mismatches": 0, "counts": [], "s1": []}
+
+    def bad(msg):
+        res["mismatches"] += 1
+        if log:
+            log(msg)
+
+    if not np.array_equal(dcard, card) or not np.array_equal(davg, avg):
+        bad(f"initial card/avg differ: {list(dcard)} vs {list(card)}")
+    if not np.array_equal(q.calig_lit(), host.lit(n)):
+        bad("initial lit bits differ from the host producer's")
+    if not np.array_equal(q.calig_old_ids(), host.old_ids()):
+        bad("initial compressed ids differ")
+    plan = host.make_plan(card, avg)
+    q.set_plan(plan)
+    idx = O.Index(plan.ndir, host.ebits)
+    idx.set_dv(dv)
+    idx.init(keys)
+    if not np.array_equal(q.export(0), idx.export(0)):
+        bad("initial M differs")
+    nb = host.num_batches if max_batches is None else min(max_batches, host.num_batches)
+    for b in range(nb):
+        kb = host.next_batch()
+        g.apply_range(b * batch_ints, batch_ints)
+        q.calig_update()
+        res["s1"].append(q.calig_stats())
+        li = q.calig_lit()
+        if not np.array_equal(li, host.lit(n)):
+            bad(f"batch {b}: lit bits differ from the host producer's")
+        if check_li:
+            o2, a2, l2 = host.graph()
+            if not np.array_equal(li, O.li_fixpoint(o2, a2, l2, ql, qe)):
+                bad(f"batch {b}: lit bits differ from the from-scratch fixpoint")
+        if not np.array_equal(q.calig_old_ids(), host.old_ids()):
+            bad(f"batch {b}: compressed ids differ")
+        if not np.array_equal(np.sort(q.staged_keys("upd")), np.sort(kb.upd)):
+            bad(f"batch {b}: update keys differ")
+        if not np.array_equal(np.unique(q.staged_keys("del")), np.unique(kb.upd)):
+            bad(f"batch {b}: delete keys are not the batch's update entries")
+        st = q.stats()
+        idx.set_dv(kb.dv)
+        q.transfer_update_all()
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        if q.stats()["dv_count"] != kb.dv:
+            bad(f"batch {b}: dv {q.stats()['dv_count']} vs {kb.dv}")
+        if not np.array_equal(q.export(0), idx.export(0)):
+            bad(f"batch {b}: M after apply differs")
+        nonempty = q.transfer_update_index()
+        idx.rebuild_update(kb.upd)
+        res["batches"] += 1
+        if not nonempty:
+            res["counts"].append(0)
+            q.batch_update_global_index()
+            idx.insert(kb.upd)
+            continue
+        if not np.array_equal(q.export(1), idx.export(1)):
+            bad(f"batch {b}: U differs")
+        q.build_local_index()
+        om = idx.masks(plan)
+        if check_masks:
+            mb = _masks_equal(q, om, plan, kb.dv, kb.edge_ok)
+            if mb:
+                bad(f"batch {b}: masks differ {mb[:5]}")
+        cnt, per = q.matching()
+        ocnt, oper = idx.match(plan, kb.edge_ok, masks=om, threads=4)
+        if cnt != ocnt or not np.array_equal(per, oper):
+            bad(f"batch {b}: count {cnt} vs oracle {ocnt}")
+        q.batch_update_global_index()
+        idx.insert(kb.upd)
+        if not np.array_equal(q.export(0), idx.export(0)):
+            bad(f"batch {b}: M after commit differs")
+        res["total"] += cnt
+        res["counts"].append(cnt)
+        del st
+    q.close()
+    g.close()
+    return res
