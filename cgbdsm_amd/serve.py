@@ -312,3 +312,199 @@ class Server:
 
     def close(self):
         self.pool.shutdown()
+
+
+@dataclass
+class QueryRequest:
+    """A query registration for the device-Stage-1 server: the query itself
+    (labels, edges); its CaLiG index is built on the device over the
+    resident graph when it is admitted."""
+    name: str
+    labels: object
+    edges: object
+    arrive_s: float = 0.0
+    leave_after: int = None
+    footprint: int = None            # device bytes to reserve (None: measured at the first admission)
+
+
+class DeviceServer:
+    """Serving loop with Stage 1 on the device.
+
+    The data graph and the whole update stream stay resident in HBM
+    (cgbdsm_graph_*); every admitted query keeps its CaLiG index, main index
+    and update lists resident beside them.  Update batches arrive open-loop
+    (arrival time, size in stream edges) and are applied in stream order;
+    query registrations arrive at their own times.  Per batch:
+
+      * the graph applies the batch once;
+      * served: every live query's Stage 1 (cgbdsm_calig_update), index
+        maintenance, masks and frontier run concurrently (one host thread and
+        one stream per query), ONE persistent final-DFS launch counts them
+        all, and the commits run concurrently;
+        plain: the live queries one after another, each with its own launch;
+      * meanwhile, queries admitted at this batch boundary build their CaLiG
+        index over the graph as it now stands (cgbdsm_calig_attach, on their
+        own threads and streams, overlapping the live set's work) and join
+        the live set at the next batch.
+
+    Admission: a free slot (max_live) and room in the engine's memory budget
+    for the query's footprint next to the admitted queries' footprints; a
+    load that still fails with CGBDSM_ERR_OOM waits for a departure.  A
+    query's count for a batch depends only on its index before and after the
+    batch, so its counts in the mix equal its counts alone."""
+
+    def __init__(self, engine, graph, *, mode="served", layout="gpma", deadline_ms=120000.0, max_live=128,
+                 threads=16, tuning=None, log=None):
+        if mode not in ("served", "plain"):
+            raise ValueError("mode is 'served' or 'plain'")
+        self.eng, self.graph, self.mode, self.layout = engine, graph, mode, layout
+        self.deadline_ms, self.max_live = deadline_ms, max_live
+        self.pool = ThreadPoolExecutor(max_workers=threads)
+        self.log = log or (lambda m: None)
+        self.tuning = tuple(tuning) if tuning else None
+        self.footprint = None
+
+    def _attach(self, r):
+        from .host import Planner
+        ebits = 5 if 2 * len(r.edges) <= 32 else 7
+        pl = Planner(r.labels, r.edges, ebits=ebits)
+        dq = self.eng.query(pl.query_tables)
+        try:
+            used0, _ = self.eng.memory()
+            dq.set_layout(self.layout)
+            card, avg = dq.calig_attach(self.graph, r.labels)
+            dq.set_plan(pl.make_plan(card, avg))
+            dq.set_deadline(self.deadline_ms)
+            if self.tuning:
+                dq.set_dfs_tuning(*self.tuning)
+            used1, _ = self.eng.memory()
+            r.measured = used1 - used0
+        except Exception:
+            dq.close()
+            pl.close()
+            raise
+        return dq, pl
+
+    def _prepare(self, dq):
+        dq.calig_update()
+        dq.transfer_update_all()
+        if not dq.transfer_update_index():
+            return False
+        dq.build_local_index()
+        dq.build_frontier()
+        return True
+
+    def _plain_one(self, dq):
+        dq.calig_update()
+        c, _, _ = dq.run_batch()
+        return c
+
+    def _serve_batch(self, live):
+        out = {}
+        names = list(live)
+        qs = [live[n][0] for n in names]
+        if self.mode == "plain":
+            for n, dq in zip(names, qs):
+                out[n] = self._plain_one(dq)
+            return out
+        ok = list(self.pool.map(self._prepare, qs))
+        ks = [k for k, f in enumerate(ok) if f]
+        for k, f in enumerate(ok):
+            if not f:
+                out[names[k]] = 0
+        if ks:
+            if self.tuning is None:
+                from .tune import autotune
+                best, _, _ = autotune(self.eng, [qs[k] for k in ks], grabs=(2, 4, 8), blocks_per_cu=(0,))
+                self.tuning = best
+                for dq in qs:
+                    dq.set_dfs_tuning(*best)
+            for k, (c, _) in zip(ks, self.eng.matching_batch([qs[k] for k in ks])):
+                out[names[k]] = c
+            list(self.pool.map(lambda k: qs[k].batch_update_global_index(), ks))
+        for dq in qs:
+            dq.sync()
+        return out
+
+    def run(self, requests, arrivals):
+        """requests: QueryRequest list; arrivals: [(arrival_s, edges)] per
+        update batch in stream order (seconds after the loop starts, which is
+        after the queries registered at time <= 0 are attached).  Returns
+        counts {name: {batch: count}}, per-batch latency (ms, arrival -> every
+        live count available) and admission records."""
+        pending = sorted(requests, key=lambda r: r.arrive_s)
+        waiting, live = [], {}
+        counts = {r.name: {} for r in requests}
+        lat, admitted, oom = [], {}, 0
+        blocked = False
+
+        def fits(r):
+            used, budget = self.eng.memory()
+            need = r.footprint if r.footprint is not None else (self.footprint or 0)
+            return not budget or used + need <= budget
+
+        while pending and pending[0].arrive_s <= 0 and len(live) < self.max_live and fits(pending[0]):
+            r = pending.pop(0)
+            live[r.name] = self._attach(r)
+            self.footprint = max(self.footprint or 0, r.measured)
+            admitted[r.name] = {"arrived_s": r.arrive_s, "admitted_s": 0.0, "joined_batch": 0}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        t_first = arrivals[0][0] if arrivals else 0.0
+        start = 0
+        for b, (t_arr, edges) in enumerate(arrivals):
+            while now() < t_arr:
+                time.sleep(min(t_arr - now(), 0.001))
+            while pending and pending[0].arrive_s <= now():
+                waiting.append(pending.pop(0))
+            self.graph.apply_range(start, 2 * edges)
+            start += 2 * edges
+            # admissions at this boundary attach over the graph after batch b
+            joining = []
+            while waiting and not blocked and len(live) + len(joining) < self.max_live and fits(waiting[0]):
+                r = waiting.pop(0)
+                joining.append((r, self.pool.submit(self._attach, r)))
+                admitted[r.name] = {"arrived_s": r.arrive_s, "admitted_s": now(), "joined_batch": b + 1}
+            res = self._serve_batch(live) if live else {}
+            t_done = now()
+            for name, c in res.items():
+                counts[name][b] = c
+            lat.append(1e3 * (t_done - t_arr))
+            for r, fut in joining:
+                try:
+                    live[r.name] = fut.result()
+                    self.footprint = max(self.footprint or 0, r.measured)
+                except CgbdsmError as e:
+                    if e.code != OOM:
+                        raise
+                    oom += 1
+                    blocked = True
+                    waiting.insert(0, r)
+                    admitted.pop(r.name, None)
+            for name in [n for n, (dq, pl) in live.items() if self._leaves(requests, n, b)]:
+                dq, pl = live.pop(name)
+                dq.close()
+                pl.close()
+                blocked = False
+        t_end = now()
+        for dq, pl in live.values():
+            dq.close()
+            pl.close()
+        edges = sum(p for _, p in arrivals)
+        return {
+            "counts": counts, "latency_ms": lat, "admissions": admitted, "oom_retries": oom,
+            "update_edges_per_s": edges / max(t_end - t_first, 1e-9),
+            "p50_ms": statistics.median(lat) if lat else 0.0,
+            "p99_ms": sorted(lat)[min(len(lat) - 1, int(0.99 * len(lat)))] if lat else 0.0,
+            "dfs_tuning": self.tuning, "footprint_bytes": self.footprint,
+        }
+
+    @staticmethod
+    def _leaves(requests, name, b):
+        for r in requests:
+            if r.name == name:
+                return r.leave_after is not None and r.leave_after <= b
+        return False
+
+    def close(self):
+        self.pool.shutdown()
