@@ -384,3 +384,36 @@ def test_masks_match_definition():
                 assert ok == bit(ei, u0, v), (ei, d, v)
                 checked += 1
     assert checked > 0
+
+
+@pytest.mark.parametrize("kind,seed", [("dense", 21), ("sparse", 22), ("tree", 23)])
+def test_candidate_sets_after_every_batch(kind, seed):
+    """The host producer's post-batch CaLiG candidate sets cand[ui][uj] of
+    every lit pair (index/calig.h:28-33) are exactly the lit neighbours of
+    the query neighbour's label: the definition the device Stage 1 and the
+    index keys build from.  Checked for every lit pair after every batch."""
+    spec = synth_spec(9, 6, 3, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
+    host = HostSession(spec=spec, batch_edges=150, threads=2)
+    host.initial_keys()
+    ql, qe = host.query_info()
+    qn = [set() for _ in ql]
+    for a, b in qe.reshape(-1, 2):
+        qn[a].add(int(b))
+        qn[b].add(int(a))
+    checked = 0
+    for b in range(host.num_batches + 1):
+        if b:
+            host.next_batch()
+        offs, adj, lab = host.graph()
+        li = host.lit(lab.size)
+        for v in range(lab.size):
+            for ui in range(len(ql)):
+                if not (li[v] >> ui) & 1:
+                    continue
+                nb = adj[int(offs[v]):int(offs[v + 1])]
+                for uj in qn[ui]:
+                    want = sorted(int(x) for x in nb if lab[x] == ql[uj] and (li[x] >> uj) & 1)
+                    got = host.cand(v, ui, uj)
+                    assert got is not None and got.tolist() == want, (b, v, ui, uj)
+                    checked += 1
+    assert checked > 1000
