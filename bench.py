@@ -605,6 +605,100 @@ def cpu_baseline_config4(args, kept, w, log):
             "s2_ms_per_query": round(1e3 * s2_tot / len(per_q), 3)}
 
 
+def serve_device_bench(args, log):
+    """Serving loop with Stage 1 on the device (cgbdsm_amd.serve.DeviceServer)
+    on config 5: R-MAT s23, degree 24, 32 labels Zipf(1.1), 90 % of the
+    edges resident; update batches arrive as a Poisson process with
+    lognormal sizes (median --serve-median edges, sigma 1, capped at 2 M);
+    queries of 4-12 vertices (mixed kinds) register at a tenth of the batch
+    rate on top of --serve-initial registered at the start, at most 128
+    live, admitted against the device memory budget.  Four runs over the
+    same arrivals: open loop and closed loop (every batch waiting at the
+    start), each served and plain; every count a query gets must agree
+    across the runs.  A query whose count of a batch cannot be taken within
+    the deadline (or exceeds 64 bits) is evicted, by name, in every run."""
+    import random
+
+    import numpy as np
+
+    from cgbdsm_amd.engine import Engine
+    from cgbdsm_amd.host import Workload, synth_spec
+    from cgbdsm_amd.multi import sample_mix
+    from cgbdsm_amd.serve import DeviceServer, QueryRequest
+
+    rng = random.Random(7)
+    t0 = time.perf_counter()
+    spec = synth_spec(args.scale, args.avg_degree, args.labels, zipf_s=args.zipf, seed=1,
+                      initial_frac=args.initial_frac, stream_edges=args.stream_edges)
+    w = Workload(spec)
+    left = w.stream.size // 2
+    sizes = []
+    while left > 0 and len(sizes) < args.serve_batches:
+        k = min(left, max(1, min(2_000_000, int(round(args.serve_median * float(np.exp(rng.gauss(0.0, 1.0))))))))
+        sizes.append(k)
+        left -= k
+    t, arr = 0.0, []
+    for p in sizes:
+        arr.append((t, p))
+        t += rng.expovariate(args.serve_rate)
+    horizon = t
+    t, reg_t = 0.0, [0.0] * args.serve_initial
+    while True:
+        t += rng.expovariate(args.serve_rate / 10.0)
+        if t >= horizon:
+            break
+        reg_t.append(t)
+    specs = sample_mix(w, len(reg_t), args.mix_seed + 100)
+    log(f"serve: graph of {w.n} vertices, {w.edges.shape[0]} resident edges; {len(sizes)} batches "
+        f"(median {int(np.median(sizes))} edges, max {max(sizes)}), {len(specs)} registrations "
+        f"({args.serve_initial} at start), setup {time.perf_counter() - t0:.1f}s")
+    out = {"metric": "serving loop with device Stage 1: sustained update edges/s and p50/p99 per-batch latency, "
+                     "served vs plain (exact counts)",
+           "config": {"workload": f"config 5: rmat-s{args.scale}-deg{args.avg_degree:g}-{args.labels}labels-"
+                                  f"zipf{args.zipf:g}, {args.initial_frac:g} resident, Poisson batches at "
+                                  f"{args.serve_rate:g}/s, lognormal sizes (median {args.serve_median}, sigma 1, "
+                                  f"cap 2M), registrations at a tenth of the batch rate, <= 128 live"},
+           "batches": len(sizes), "batch_edges": sizes, "registrations": [sp.name for sp in specs],
+           "registration_s": [round(x, 4) for x in reg_t]}
+    counts, tuning = {}, None
+    for loop in ("open", "closed"):
+        for mode in ("served", "plain"):
+            eng = Engine(0, budget_bytes=int(args.serve_budget_gb * (1 << 30)))
+            g = eng.graph(w.labels, w.edges)
+            g.set_stream(w.stream)
+            reqs = [QueryRequest(sp.name, sp.labels, sp.edges, arrive_s=(a if loop == "open" else min(a, 0.0)))
+                    for sp, a in zip(specs, reg_t)]
+            arrivals = arr if loop == "open" else [(0.0, p) for p in sizes]
+            srv = DeviceServer(eng, g, mode=mode, deadline_ms=args.serve_deadline_ms, tuning=tuning,
+                               threads=16, max_live=128)
+            res = srv.run(reqs, arrivals)
+            if mode == "served":
+                tuning = res["dfs_tuning"]
+            srv.close()
+            g.close()
+            eng.close()
+            key = f"{loop}_{mode}"
+            counts[key] = res["counts"]
+            out[key] = {"update_edges_per_s": round(res["update_edges_per_s"], 3),
+                        "p50_batch_ms": round(res["p50_ms"], 3), "p99_batch_ms": round(res["p99_ms"], 3),
+                        "evicted": res["evicted"], "admitted": len(res["admissions"]),
+                        "footprint_mb": round((res["footprint_bytes"] or 0) / 2**20, 1)}
+            log(f"serve {key}: {res['update_edges_per_s']:.0f} update edges/s, p50 {res['p50_ms']:.1f} ms, "
+                f"p99 {res['p99_ms']:.1f} ms, evicted {list(res['evicted'])}")
+    agree = True
+    for name in out["registrations"]:
+        merged = {}
+        for c in counts.values():
+            for b, v in c.get(name, {}).items():
+                if merged.setdefault(b, v) != v:
+                    agree = False
+    out["counts_agree"] = agree
+    out["closed_served_vs_plain"] = round(out["closed_served"]["update_edges_per_s"] /
+                                          out["closed_plain"]["update_edges_per_s"], 4)
+    w.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -645,6 +739,12 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0, help="batch arrivals per second (open loop)")
     ap.add_argument("--serve-median", type=int, default=50_000, help="median batch size in edges")
     ap.add_argument("--serve-batches", type=int, default=24, help="batches drawn from the stream at most")
+    ap.add_argument("--serve-device", action="store_true",
+                    help="config 5 serving loop with device Stage 1 (R-MAT s23; prints its own JSON line)")
+    ap.add_argument("--serve-initial", type=int, default=32, help="queries registered before the first batch")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="device memory budget of the serving runs")
+    ap.add_argument("--serve-deadline-ms", type=float, default=3000.0,
+                    help="per-batch count deadline; a query that misses it is evicted")
     ap.add_argument("--index-stress", action="store_true",
                     help="index maintenance only, GAMMA vs GPMA (prints its own JSON line)")
     ap.add_argument("--multi-queries", type=int, default=8,
@@ -662,7 +762,16 @@ def main():
     from cgbdsm_amd.host import HostSession, synth_spec
 
     threads = min(16, os.cpu_count() or 1)
-    headline = not (args.single or args.index_stress or args.serve or args.multi_only)
+    headline = not (args.single or args.index_stress or args.serve or args.multi_only or args.serve_device)
+    if args.serve_device:
+        args.scale = args.scale or 23
+        args.avg_degree = args.avg_degree or 24
+        if args.labels == 16:
+            args.labels = 32
+        if args.zipf == 1.2:
+            args.zipf = 1.1
+        print(json.dumps(serve_device_bench(args, log)), flush=True)
+        return
     if args.scale is None:
         args.scale = 21 if headline else 20
     if args.avg_degree is None:

@@ -69,6 +69,9 @@ struct cgbdsm_query {
     // the best untuned value on the mix (2 measured 8 % slower there, 8 %
     // faster on the dense stream); cgbdsm_set_dfs_tuning changes both.
     uint32_t dfs_grab = 4, dfs_bpc = 0;
+    // Smallest candidate range a DFS wavefront splits off for idle ones, and
+    // the cap on the per-wave stable-intersection cache (0: all LDS left).
+    uint32_t dfs_range_min = 1024, dfs_cache_cap = 0;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;

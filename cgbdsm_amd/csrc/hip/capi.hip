@@ -305,6 +305,7 @@ StealQueue MakeSteal(cgbdsm_query* q, bool donate) {
     Q.seq = q->steal_seq;
     Q.cap = q->steal_cap;
     Q.donate_min = donate ? q->donate_min : 0;
+    Q.range_min = q->dfs_range_min;
     Q.abort = q->d_abort;
     Q.poll_ticks = std::max<long long>(1, q->ctx->wall_khz / 20);  // 50 us
     Q.host_ticks = (long long)q->ctx->wall_khz * 2;              // 2 ms
@@ -342,9 +343,12 @@ void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
     const int k = tuples ? -1 : KBegin(q, CGBDSM_K_DFS);
     uint32_t levels = 1;  // survivor stacks: qv - 1 levels cover every job
     for (const DfsJob& j : q->h_jobs) levels = std::max<uint32_t>(levels, j.T.qv - 1);
-    q->st.last_dfs_lds_cache = DfsLdsShape(levels, tuples != nullptr).stab_cap;
+    {
+        const uint32_t full = DfsLdsShape(levels, tuples != nullptr).stab_cap;
+        q->st.last_dfs_lds_cache = q->dfs_cache_cap && q->dfs_cache_cap < full ? q->dfs_cache_cap & ~63u : full;
+    }
     LaunchDfs(q->d_jobs.p, (uint32_t)q->h_jobs.size(), q->d_rowbase.p, n, q->d_cnt, MakeSteal(q, tuples == nullptr),
-              tuples, tup_cap, q->ctx->num_cus, q->dfs_bpc, q->dfs_grab, levels, q->s);
+              tuples, tup_cap, q->ctx->num_cus, q->dfs_bpc, q->dfs_grab, levels, q->dfs_cache_cap, q->s);
     KEnd(q, k);
     CGD_HIP(hipGetLastError());
     if (q->deadline_ms > 0) {
@@ -1106,6 +1110,14 @@ cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uin
         if (!q || rows_per_grab > 1024) Throw(CGBDSM_ERR_INVALID, "bad DFS tuning");
         q->dfs_grab = rows_per_grab ? rows_per_grab : 4;
         q->dfs_bpc = blocks_per_cu;
+    });
+}
+
+cgbdsm_status cgbdsm_set_dfs_steal(cgbdsm_query* q, uint32_t range_min, uint32_t cache_cap) {
+    return Guard([&] {
+        if (!q || range_min > (1u << 20)) Throw(CGBDSM_ERR_INVALID, "bad DFS stealing knobs");
+        q->dfs_range_min = range_min ? std::max<uint32_t>(range_min, 64) : 1024;
+        q->dfs_cache_cap = cache_cap;
     });
 }
 

@@ -91,6 +91,7 @@ struct StealQueue {
     unsigned long long* seq;  // seq[slot] = ticket + 1 once the item is published
     uint64_t cap;             // slots (power of two)
     uint32_t donate_min;      // remaining survivors needed before donating (0: never)
+    uint32_t range_min;       // candidates a donated range must at least hold
     const uint32_t* abort;    // host-mapped coherent flag, non-zero asks kernels to stop
     long long poll_ticks;     // wall-clock ticks between a wave's reads of shared queue words
     long long host_ticks;     // wall-clock ticks between a wave's reads of the host flag

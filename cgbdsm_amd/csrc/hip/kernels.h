@@ -76,9 +76,11 @@ struct DfsJob {
 // job j, row_base[njobs] = all rows); `cnt` holds the shared work queue.
 // surv_levels >= qv - 1 of every job (survivor stacks in LDS); blocks_per_cu
 // 0 = as many as fit.
+// cache_cap: u32 per wave of the stable-intersection cache (0: all the LDS
+// left; smaller values leave part of it unused).
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t num_cus, uint32_t blocks_per_cu,
-               uint32_t grab, uint32_t surv_levels, hipStream_t s);
+               uint32_t grab, uint32_t surv_levels, uint32_t cache_cap, hipStream_t s);
 // LDS plan of a final-DFS launch (see DfsLdsShapeT in match.hip).
 struct DfsShape {
     uint32_t blocks_per_cu, dyn_bytes, stab_cap;

@@ -545,7 +545,6 @@ __device__ __forceinline__ unsigned long long SatMul(unsigned long long a, unsig
 constexpr uint32_t kFull = 0xFFFFFFFFu;
 constexpr uint32_t kNoSel = 0xFFu;
 constexpr uint32_t kNoSkip = 0xFFu;
-constexpr uint32_t kRangeMin = 1024;  // candidates a donated range must at least hold
 
 __device__ __forceinline__ bool AbortRequested(const StealQueue& Q) {
     return __hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
@@ -1418,7 +1417,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             // into, so its ranges are split down to one step each (smaller
             // pieces would repeat the per-item set-up of the suffix sizes).
             const bool blev = bsfx && d + 1 == cps;
-            const uint32_t rmin = blev ? kWave : kRangeMin;
+            const uint32_t rmin = blev ? kWave : Q.range_min;
             if (Q.donate_min && idl_hint > 0 && limit - off >= 2 * rmin) {
                 unsigned long long h = 0, t = 0;
                 if (lane == 0) {
@@ -1640,9 +1639,10 @@ void LaunchQueueSync(Counters* cnt, hipStream_t s) { KQueueSync<<<1, 64, 0, s>>>
 
 void LaunchDfs(const DfsJob* jobs, uint32_t njobs, const unsigned long long* row_base, uint64_t n_rows, Counters* cnt,
                const StealQueue& Q, uint32_t* tuples, uint64_t tup_cap, uint32_t num_cus, uint32_t blocks_per_cu,
-               uint32_t grab, uint32_t surv_levels, hipStream_t s) {
+               uint32_t grab, uint32_t surv_levels, uint32_t cache_cap, hipStream_t s) {
     if (!n_rows || !njobs) return;
-    const DfsShape sh = DfsLdsShape(surv_levels, tuples != nullptr);
+    DfsShape sh = DfsLdsShape(surv_levels, tuples != nullptr);
+    if (cache_cap && cache_cap < sh.stab_cap) sh.stab_cap = cache_cap & ~63u;
     const uint32_t bpc = blocks_per_cu && blocks_per_cu < sh.blocks_per_cu ? blocks_per_cu : sh.blocks_per_cu;
     const unsigned g = Grid1(n_rows, kWpb, num_cus * bpc);
     grab = grab < 1 ? 1 : grab;

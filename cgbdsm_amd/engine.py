@@ -264,6 +264,12 @@ class DeviceQuery:
         _check(self.L.cgbdsm_set_dfs_tuning(self.h, int(rows_per_grab), int(blocks_per_cu)))
         self.dfs_tuning = (int(rows_per_grab), int(blocks_per_cu))
 
+    def set_dfs_steal(self, range_min=1024, cache_cap=0):
+        """Stealing chunk and stable-cache size of the final DFS (counts are
+        identical for every setting)."""
+        _check(self.L.cgbdsm_set_dfs_steal(self.h, int(range_min), int(cache_cap)))
+        self.dfs_steal = (int(range_min), int(cache_cap))
+
     def export(self, which):
         n = C.c_uint64()
         _check(self.L.cgbdsm_index_export(self.h, which, None, 0, C.byref(n)))

@@ -38,6 +38,7 @@ from dataclasses import dataclass, field
 from .engine import CgbdsmError
 
 OOM = 2
+OVERFLOW = 5
 
 
 @dataclass
@@ -399,13 +400,23 @@ class DeviceServer:
         c, _, _ = dq.run_batch()
         return c
 
-    def _serve_batch(self, live):
+    def _serve_batch(self, live, evicted):
+        """Counts of every live query for the batch the graph just applied.
+        A query whose count cannot be taken (deadline, or a count beyond 64
+        bits: CGBDSM_ERR_OVERFLOW) is evicted and recorded in `evicted`;
+        in served mode the batch's queries are then counted one by one to
+        find it."""
         out = {}
         names = list(live)
         qs = [live[n][0] for n in names]
         if self.mode == "plain":
             for n, dq in zip(names, qs):
-                out[n] = self._plain_one(dq)
+                try:
+                    out[n] = self._plain_one(dq)
+                except CgbdsmError as e:
+                    if e.code != OVERFLOW:
+                        raise
+                    evicted[n] = str(e)[:200]
             return out
         ok = list(self.pool.map(self._prepare, qs))
         ks = [k for k, f in enumerate(ok) if f]
@@ -419,9 +430,21 @@ class DeviceServer:
                 self.tuning = best
                 for dq in qs:
                     dq.set_dfs_tuning(*best)
-            for k, (c, _) in zip(ks, self.eng.matching_batch([qs[k] for k in ks])):
-                out[names[k]] = c
-            list(self.pool.map(lambda k: qs[k].batch_update_global_index(), ks))
+            try:
+                for k, (c, _) in zip(ks, self.eng.matching_batch([qs[k] for k in ks])):
+                    out[names[k]] = c
+            except CgbdsmError as e:
+                if e.code != OVERFLOW:
+                    raise
+                for k in ks:
+                    try:
+                        out[names[k]] = qs[k].matching()[0]
+                    except CgbdsmError as e2:
+                        if e2.code != OVERFLOW:
+                            raise
+                        evicted[names[k]] = str(e2)[:200]
+            list(self.pool.map(lambda k: qs[k].batch_update_global_index(),
+                               [k for k in ks if names[k] not in evicted]))
         for dq in qs:
             dq.sync()
         return out
@@ -435,7 +458,7 @@ class DeviceServer:
         pending = sorted(requests, key=lambda r: r.arrive_s)
         waiting, live = [], {}
         counts = {r.name: {} for r in requests}
-        lat, admitted, oom = [], {}, 0
+        lat, admitted, oom, evictions = [], {}, 0, {}
         blocked = False
 
         def fits(r):
@@ -465,11 +488,18 @@ class DeviceServer:
                 r = waiting.pop(0)
                 joining.append((r, self.pool.submit(self._attach, r)))
                 admitted[r.name] = {"arrived_s": r.arrive_s, "admitted_s": now(), "joined_batch": b + 1}
-            res = self._serve_batch(live) if live else {}
+            gone = {}
+            res = self._serve_batch(live, gone) if live else {}
             t_done = now()
             for name, c in res.items():
                 counts[name][b] = c
             lat.append(1e3 * (t_done - t_arr))
+            for name, why in gone.items():
+                dq, pl = live.pop(name)
+                dq.close()
+                pl.close()
+                evictions[name] = {"batch": b, "reason": why}
+                blocked = False
             for r, fut in joining:
                 try:
                     live[r.name] = fut.result()
@@ -496,7 +526,7 @@ class DeviceServer:
             "update_edges_per_s": edges / max(t_end - t_first, 1e-9),
             "p50_ms": statistics.median(lat) if lat else 0.0,
             "p99_ms": sorted(lat)[min(len(lat) - 1, int(0.99 * len(lat)))] if lat else 0.0,
-            "dfs_tuning": self.tuning, "footprint_bytes": self.footprint,
+            "dfs_tuning": self.tuning, "footprint_bytes": self.footprint, "evicted": evictions,
         }
 
     @staticmethod

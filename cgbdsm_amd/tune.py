@@ -55,3 +55,53 @@ def autotune(engine, queries, grabs=GRABS, blocks_per_cu=BLOCKS_PER_CU, repeats=
     lead.kernel_times(reset=True)
     lead.profile(False)
     return best, table, ref
+
+
+# Second stage: stealing knobs of the final DFS, searched one at a time
+# around the launch shape found first (coordinate descent: each candidate
+# costs one counted batch).  range_min is the smallest candidate range a
+# wavefront splits off for idle ones (the reference's fixed STEAL_CHUNK /
+# TASK_SHARE_BOUND, kernels/enumeration_balance.cu:11-17); donate_min the
+# survivors left at which it hands work to the queue; cache_cap the u32 per
+# wavefront of the LDS stable-intersection cache (0: all the LDS left).
+RANGE_MIN = (256, 1024, 4096)
+DONATE_MIN = (1, 4, 16)
+CACHE_CAP = (0, 1024, 256)
+
+
+def autotune_full(engine, queries, grabs=GRABS, blocks_per_cu=BLOCKS_PER_CU, range_min=RANGE_MIN,
+                  donate_min=DONATE_MIN, cache_cap=CACHE_CAP, log=None):
+    """autotune, then the stealing knobs one after another.  Returns
+    (knobs, table, counts): knobs = {rows_per_grab, blocks_per_cu, range_min,
+    donate_min, cache_cap}, table = {knob: {value: DFS ms}}; every candidate
+    must give identical counts, and the fastest setting is left applied."""
+    best, shape_table, ref = autotune(engine, queries, grabs=grabs, blocks_per_cu=blocks_per_cu, log=log)
+    knobs = {"rows_per_grab": best[0], "blocks_per_cu": best[1], "range_min": 1024, "donate_min": 4,
+             "cache_cap": 0}
+    table = {"shape": {f"{g}/{b}": ms for (g, b), ms in shape_table.items()}}
+    lead = queries[0]
+
+    def apply(k):
+        for q in queries:
+            q.set_dfs_steal(k["range_min"], k["cache_cap"])
+            q.set_match_params(bfs_row_limit=getattr(q, "bfs_row_limit", 65536), donate_min=k["donate_min"])
+
+    lead.profile(True)
+    for name, values in (("range_min", range_min), ("donate_min", donate_min), ("cache_cap", cache_cap)):
+        table[name] = {}
+        for v in values:
+            trial = dict(knobs, **{name: v})
+            apply(trial)
+            lead.kernel_times(reset=True)
+            got = _count(engine, queries)
+            ms = lead.kernel_times()["dfs"]["ms"]
+            if any(a[0] != b[0] or not np.array_equal(a[1], b[1]) for a, b in zip(got, ref)):
+                raise RuntimeError(f"DFS knob {name}={v} changed the counts")
+            table[name][v] = ms
+            if log:
+                log(f"autotune {name}={v}: DFS {ms:.2f} ms")
+        knobs[name] = min(table[name], key=table[name].get)
+    apply(knobs)
+    lead.kernel_times(reset=True)
+    lead.profile(False)
+    return knobs, table, ref

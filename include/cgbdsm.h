@@ -264,6 +264,17 @@ cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int pro
  * reference counterpart (fixed GRID_DIM/BLOCK_DIM, utils/config.h). */
 cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu);
 
+/* Stealing chunk and LDS split of the final DFS (autotuner knobs; counts
+ * are identical for every setting).  range_min = smallest candidate range a
+ * wavefront splits off for idle wavefronts (0: default 1024; at least 64;
+ * the reference's fixed STEAL_CHUNK / TASK_SHARE_BOUND,
+ * kernels/enumeration_balance.cu:11-17); cache_cap = u32 per wavefront of
+ * the stable-intersection cache in LDS (0: all the LDS the survivor stacks
+ * leave; smaller values trade cache hits for nothing else, since the block's
+ * LDS size stays).  For cgbdsm_matching_batch the first query's setting
+ * applies. */
+cgbdsm_status cgbdsm_set_dfs_steal(cgbdsm_query* q, uint32_t range_min, uint32_t cache_cap);
+
 /* Cooperative deadline for the final DFS of cgbdsm_matching (default 120 s,
  * 0 = none).
  * Past the deadline the host raises a flag the kernel polls; the call then

@@ -1,18 +1,42 @@
-"""Serving loop (cgbdsm_amd.serve): queries registered while batches are
+"""Serving loops (cgbdsm_amd.serve): queries registered while batches are
 being served, admitted against the device memory budget, brought up to the
 stream position on their own stream and counted in the batched launch.
 Every count a query gets in the mix must equal its count when it runs alone
-from the start of the stream."""
+from the start of the stream, and the oracle's count of that batch.  The
+device-Stage-1 server (DeviceServer) attaches late queries over the
+resident graph as it stands; its counts must equal the oracle's too."""
 import numpy as np
 import pytest
 
 from cgbdsm_amd.engine import Engine
-from cgbdsm_amd.host import HostSession, synth_spec
-from cgbdsm_amd.serve import Registration, Server
+from cgbdsm_amd.host import HostSession, Workload, synth_spec
+from cgbdsm_amd.serve import DeviceServer, QueryRequest, Registration, Server
+from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
 MIX = [("dense", 5, 2), ("tree", 5, 3), ("sparse", 6, 4), ("dense", 4, 5)]
+
+
+def _oracle_counts(plan, keys, dv, batches):
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    out = {}
+    for b, kb in enumerate(batches):
+        idx.set_dv(kb.dv)
+        idx.insert(kb.ins)
+        idx.remove(kb.dele)
+        idx.rebuild_update(kb.upd)
+        c = 0
+        if kb.upd.size:
+            c, _ = idx.match(plan, kb.edge_ok, masks=idx.masks(plan), threads=4)
+            idx.insert(kb.upd)
+        out[b] = int(c)
+    return out
+
+
+ORACLE = {}
 
 
 @pytest.fixture(scope="module")
@@ -24,7 +48,10 @@ def regs():
         keys, dv, card, avg = host.initial_keys()
         plan = host.make_plan(card, avg)
         batches = [host.next_batch() for _ in range(host.num_batches)]
-        out.append(Registration(f"q{i}-{size}{kind}", host.query_tables, plan, keys, dv, batches))
+        r = Registration(f"q{i}-{size}{kind}", host.query_tables, plan, keys, dv, batches)
+        ORACLE[r.name] = _oracle_counts(plan, keys, dv, batches)
+        r.query = host.query_info()
+        out.append(r)
     return out
 
 
@@ -49,6 +76,9 @@ def alone(regs):
     nb = min(len(r.batches) for r in regs)
     res = {r.name: _alone(eng, r, nb) for r in regs}
     eng.close()
+    for r in regs:  # alone on the device == the oracle
+        for b in range(nb):
+            assert res[r.name][0][b] == ORACLE[r.name][b], (r.name, b)
     return nb, res
 
 
@@ -108,3 +138,35 @@ def test_admission_waits_for_memory(regs, alone):
         for b, c in res["counts"][r.name].items():
             assert c == ref[regs[1].name][0][b]
     assert np.all(np.diff(sorted(res["counts"]["t2"])) == 1)
+
+
+@pytest.mark.parametrize("mode", ["served", "plain"])
+def test_device_server_counts_equal_the_oracle(regs, alone, mode):
+    """Stage 1 on the device: the graph and stream resident, two queries from
+    the start (one leaves), two attached over the graph at later batch
+    boundaries; every count equals the oracle's count of that batch."""
+    print(f"case device serve {mode}", flush=True)
+    nb, _ = alone
+    w = Workload(synth_spec(11, 8, 4, seed=2))
+    eng = Engine(0)
+    g = eng.graph(w.labels, w.edges)
+    g.set_stream(w.stream)
+    arrivals = [(0.03 * b, 120) for b in range(nb)]
+    reqs = [QueryRequest(r.name, r.query[0], r.query[1].reshape(-1, 2), arrive_s=a, leave_after=l)
+            for r, a, l in zip(regs, [0.0, 0.0, 0.07, 0.13], [None, 3, None, nb - 2])]
+    srv = DeviceServer(eng, g, mode=mode, threads=8)
+    res = srv.run(reqs, arrivals)
+    srv.close()
+    g.close()
+    eng.close()
+    w.close()
+    served = 0
+    for r in reqs:
+        got = res["counts"][r.name]
+        assert got, f"{r.name} was never served"
+        for b, c in got.items():
+            assert c == ORACLE[r.name][b], (r.name, b, c, ORACLE[r.name][b])
+            served += 1
+    assert res["admissions"][reqs[2].name]["joined_batch"] > 0
+    assert not res["evicted"]
+    assert served >= 2 * nb
