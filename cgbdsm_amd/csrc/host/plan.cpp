@@ -157,8 +157,9 @@ void Plan::GlobalOrder(const uint32_t* card, const float* avg) {
                 }
             }
             if (cand.empty()) throw Error("query graph is not connected");
-            std::stable_sort(cand.begin(), cand.end(),
-                             [](const auto& a, const auto& b) { return a.second < b.second; });
+            // std::sort with the reference's comparator (plan.cpp:580), so
+            // ties fall the same way with the same standard library.
+            std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
             size_t end = cand.size();
             for (size_t j = 0; j + 1 < cand.size(); j++) {
                 if (cand[j + 1].second >= (uint32_t)(cand[j].second * (2u << 9))) {
@@ -259,8 +260,9 @@ void Plan::GlobalOrder(const uint32_t* card, const float* avg) {
         vis[sel[0]] = true;
     }
     CartesianInfo(order);
-    // Backward neighbours by position, nearest first, then stably ordered by
-    // ascending average degree (plan.cpp:720-744).
+    // Backward neighbours by position, nearest first, then ordered by
+    // ascending average degree with std::sort as the reference does
+    // (plan.cpp:720-744).
     uint32_t cum = 0;
     global.bni_offs[0] = 0;
     for (uint32_t j = 0; j < V; j++) {
@@ -269,10 +271,9 @@ void Plan::GlobalOrder(const uint32_t* card, const float* avg) {
         for (int k = (int)j - 1; k >= 0; k--)
             if (q->Adjacent(u, global.vs[k])) global.bni[cum++] = (uint8_t)k;
         global.bni_offs[j + 1] = (uint8_t)cum;
-        std::stable_sort(global.bni + global.bni_offs[j], global.bni + global.bni_offs[j + 1],
-                         [&](uint8_t a, uint8_t b) {
-                             return avg[q->E(global.vs[a], u)] < avg[q->E(global.vs[b], u)];
-                         });
+        std::sort(global.bni + global.bni_offs[j], global.bni + global.bni_offs[j + 1], [&](uint8_t a, uint8_t b) {
+            return avg[q->E(global.vs[a], u)] < avg[q->E(global.vs[b], u)];
+        });
     }
 }
 

@@ -72,6 +72,7 @@ def test_device_stage1_call_order(eng):
     print("case s1 call order", flush=True)
     spec = synth_spec(9, 6, 3, seed=9, query_size=4, query_kind="tree", query_seed=9)
     host = HostSession(spec=spec, batch_edges=100, threads=2)
+    host.initial_keys()
     offs, adj, lab = host.graph()
     import numpy as np
     n = lab.size
