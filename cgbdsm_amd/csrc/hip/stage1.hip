@@ -475,7 +475,8 @@ __global__ void __launch_bounds__(kThreads) KS1CheckWave(GView G, QTab Q, SView 
 
 // Deaths of a round: clear the lit (and activated) bit, record the dead mask.
 __global__ void KS1Apply(QTab Q, SView S, const uint32_t* __restrict__ died, const uint16_t* __restrict__ bits,
-                         uint64_t n) {
+                         const unsigned long long* __restrict__ n_dev) {
+    const uint64_t n = *n_dev;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = died[i], v = p >> 4, ui = p & 15u;
         atomicAnd(&S.vstate[v], ~((1u << ui) | (1u << (16 + ui))));
@@ -483,10 +484,14 @@ __global__ void KS1Apply(QTab Q, SView S, const uint32_t* __restrict__ died, con
     }
 }
 
-// offs[i] = adjacency length of the vertex of pair i (before the scan).
-__global__ void KS1Deg(GView G, const uint32_t* __restrict__ L, uint64_t n, uint64_t* __restrict__ offs) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        offs[i] = G.rec[L[i] >> 4].len;
+// offs[i] = adjacency length of the vertex of pair i (before the scan); with
+// n_dev, entries from *n_dev up to n_upper are 0 (pairs the list does not hold).
+__global__ void KS1Deg(GView G, const uint32_t* __restrict__ L, uint64_t n_upper,
+                       const unsigned long long* __restrict__ n_dev, uint64_t* __restrict__ offs) {
+    const uint64_t n = n_dev ? *n_dev : n_upper;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_upper;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        offs[i] = i < n ? G.rec[L[i] >> 4].len : 0u;
 }
 
 // After deaths: queue the lit neighbour pairs they supported (only the pairs
@@ -963,11 +968,12 @@ void S1Set(cgbdsm_query* q, unsigned long long S1Cnt::*f, uint64_t v) {
 }
 
 // Offsets of the adjacency items of list L[0..n): offs[n] = total items.
-void S1Offsets(cgbdsm_query* q, const uint32_t* L, uint64_t n) {
+// With n_dev, the list holds *n_dev <= n pairs (the rest count as empty).
+void S1Offsets(cgbdsm_query* q, const uint32_t* L, uint64_t n, const unsigned long long* n_dev = nullptr) {
     S1Query* S = q->s1;
     Ensure(q, S->offs, n + 1);
     Ensure(q, q->scan_scratch, ScanScratchBytes(n + 2) + 64);
-    if (n) KS1Deg<<<Grid(n), 256, 0, q->s>>>(View(S->g), L, n, S->offs.p);
+    if (n) KS1Deg<<<Grid(n), 256, 0, q->s>>>(View(S->g), L, n, n_dev, S->offs.p);
     ScanU64(S->offs.p, S->offs.p, n, S->offs.p + n, q->scan_scratch.p, q->s);
 }
 
@@ -990,18 +996,19 @@ void S1DeletionRounds(cgbdsm_query* q, int src, uint64_t n, bool opt_only) {
                                                  &S->d_c->n_died, S->L[3].p, &S->d_c->n_big);
         KS1CheckWave<<<Grid(n, kWpb, 16384), kThreads, 0, q->s>>>(G, S->tab, V, S->L[3].p, &S->d_c->n_big,
                                                                  S->L[2].p, S->died_bits.p, &S->d_c->n_died);
+        // The deaths (at most n) are applied and their neighbours queued with
+        // the device count: one host round trip per round.
+        KS1Apply<<<Grid(n), 256, 0, q->s>>>(S->tab, V, S->L[2].p, S->died_bits.p, &S->d_c->n_died);
+        S1Offsets(q, S->L[2].p, n, &S->d_c->n_died);
+        const int nxt = cur == 0 ? 1 : 0;
+        FillU64((uint64_t*)&S->d_c->n_list[nxt], 0, 1, q->s);
+        KS1Expand<<<LbGrid(q), kThreads, 0, q->s>>>(G, S->tab, V, S->L[2].p, S->offs.p, n, opt_only ? 1u : 0u,
+                                                   S->L[nxt].p, &S->d_c->n_list[nxt]);
         S1Read(q);
         rounds++;
         const uint64_t nd = S->h_c->n_died;
         if (!nd) break;
         killed += nd;
-        KS1Apply<<<Grid(nd), 256, 0, q->s>>>(S->tab, V, S->L[2].p, S->died_bits.p, nd);
-        S1Offsets(q, S->L[2].p, nd);
-        const int nxt = cur == 0 ? 1 : 0;
-        FillU64((uint64_t*)&S->d_c->n_list[nxt], 0, 1, q->s);
-        KS1Expand<<<LbGrid(q), kThreads, 0, q->s>>>(G, S->tab, V, S->L[2].p, S->offs.p, nd, opt_only ? 1u : 0u,
-                                                   S->L[nxt].p, &S->d_c->n_list[nxt]);
-        S1Read(q);
         n = S->h_c->n_list[nxt];
         cur = nxt;
     }

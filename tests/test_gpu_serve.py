@@ -143,8 +143,9 @@ def test_admission_waits_for_memory(regs, alone):
 @pytest.mark.parametrize("mode", ["served", "plain"])
 def test_device_server_counts_equal_the_oracle(regs, alone, mode):
     """Stage 1 on the device: the graph and stream resident, two queries from
-    the start (one leaves), two attached over the graph at later batch
-    boundaries; every count equals the oracle's count of that batch."""
+    the start (one leaves after batch 3), two attached over the graph at
+    later batch boundaries (each joins at the batch after its admission);
+    every count equals the oracle's count of that batch."""
     print(f"case device serve {mode}", flush=True)
     nb, _ = alone
     w = Workload(synth_spec(11, 8, 4, seed=2))
@@ -153,7 +154,7 @@ def test_device_server_counts_equal_the_oracle(regs, alone, mode):
     g.set_stream(w.stream)
     arrivals = [(0.03 * b, 120) for b in range(nb)]
     reqs = [QueryRequest(r.name, r.query[0], r.query[1].reshape(-1, 2), arrive_s=a, leave_after=l)
-            for r, a, l in zip(regs, [0.0, 0.0, 0.07, 0.13], [None, 3, None, nb - 2])]
+            for r, a, l in zip(regs, [0.0, 0.0, 0.05, 0.10], [None, 3, None, None])]
     srv = DeviceServer(eng, g, mode=mode, threads=8)
     res = srv.run(reqs, arrivals)
     srv.close()
@@ -168,5 +169,7 @@ def test_device_server_counts_equal_the_oracle(regs, alone, mode):
             assert c == ORACLE[r.name][b], (r.name, b, c, ORACLE[r.name][b])
             served += 1
     assert res["admissions"][reqs[2].name]["joined_batch"] > 0
+    assert res["admissions"][reqs[3].name]["joined_batch"] > 0
+    assert max(res["counts"][reqs[1].name]) == 3
     assert not res["evicted"]
     assert served >= 2 * nb
