@@ -158,6 +158,7 @@ def dev_lib():
         L.cgbdsm_batch_update_global_index.argtypes = [P]
         L.cgbdsm_set_match_params.argtypes = [P, U64, U64, U32]
         L.cgbdsm_index_export.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
+        L.cgbdsm_index_export_static.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
         L.cgbdsm_mask_export.argtypes = [P, U32, U32, C.c_void_p, U64, C.POINTER(I)]
         L.cgbdsm_match_tuples.argtypes = [P, C.c_void_p, U64, C.POINTER(U64), C.c_void_p]
         L.cgbdsm_stats_get.argtypes = [P, C.POINTER(Stats)]
@@ -196,7 +197,7 @@ DEV_SYMBOLS = [
     "cgbdsm_set_match_params", "cgbdsm_index_export", "cgbdsm_mask_export", "cgbdsm_match_tuples",
     "cgbdsm_stats_get", "cgbdsm_run_batch", "cgbdsm_sync", "cgbdsm_set_deadline_ms", "cgbdsm_set_dfs_options",
     "cgbdsm_profile", "cgbdsm_query_set_layout", "cgbdsm_set_dfs_tuning", "cgbdsm_build_frontier",
-    "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next",
+    "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next", "cgbdsm_index_export_static",
     "cgbdsm_graph_create", "cgbdsm_graph_destroy", "cgbdsm_graph_set_stream", "cgbdsm_graph_apply_range",
     "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_export_lit",
     "cgbdsm_calig_old_ids", "cgbdsm_calig_stats", "cgbdsm_staged_keys", "cgbdsm_set_dfs_steal",

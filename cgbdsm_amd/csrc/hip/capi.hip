@@ -16,7 +16,33 @@
 using namespace cgd;
 using namespace cgapi;
 
+// Device scratch of cgbdsm_index_export_static, the export entry point that
+// allocates nothing: row lengths, the scan's partial sums and total, and the
+// exported keys all live in this array (32 MiB on every device the library
+// runs on, held until the process ends).  Host code never passes `scratch`
+// itself to a kernel: cgbdsm_ctx_create looks up its device address once per
+// device (StaticScratch) and every use goes through that pointer.
+constexpr uint64_t kStaticScratchWords = 4ull << 20;
+static __device__ __align__(256) uint64_t scratch[kStaticScratchWords];
+
 namespace cgapi {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up on first use.
+uint64_t* StaticScratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw ApiError{CGBDSM_ERR_INVALID, "device index beyond 64"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        CGD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw ApiError{CGBDSM_ERR_DEVICE, "static scratch has no device address"};
+        g_scratch_ptr[device] = (uint64_t*)p;
+    }
+    return g_scratch_ptr[device];
+}
 
 thread_local std::string g_err;
 
@@ -448,13 +474,19 @@ void PmaBulk(cgbdsm_query* q, const uint64_t* keys, uint64_t n) {
 }
 
 // Flat lists (M for matching) from the array: one streaming pass.
-void PmaExtract(cgbdsm_query* q) {
+// may_grow = false: a pool too small for the current entries fails with
+// CGBDSM_ERR_LIMIT instead of being reallocated (nothing is changed then).
+void PmaExtract(cgbdsm_query* q, bool may_grow = true) {
     auto& P = q->pma;
     const int k = KBegin(q, CGBDSM_K_GPMA_EXTRACT);
     ScanU32(P.cnt.p, P.gpos.p, P.nleaves + 1ull, (uint64_t*)Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
     ReadCounters(q);
     const uint64_t n = q->h_cnt->n_segs;
     if (q->pool.cap < std::max<uint64_t>(n, 1)) {
+        if (!may_grow) {
+            KEnd(q, k);
+            Throw(CGBDSM_ERR_LIMIT, "GPMA flat lists are stale and the list pool must grow to refresh them");
+        }
         Release(q, q->pool);
         const uint64_t want = n + n / 2 + (16ull << 20);
         q->pool.p = (uint32_t*)DevAlloc(q->ctx, want * sizeof(uint32_t));
@@ -645,6 +677,12 @@ cgbdsm_status cgbdsm_ctx_create(int device, uint64_t budget_bytes, cgbdsm_ctx** 
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             c->wall_khz = khz;
+        try {
+            StaticScratch(device);
+        } catch (...) {
+            delete c;
+            throw;
+        }
         *out = c;
     });
 }
@@ -1489,6 +1527,47 @@ cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uin
         std::sort(h.begin(), h.end());
         if (n_out) *n_out = total;
         if (out) memcpy(out, h.data(), std::min(cap, total) * 8);
+    });
+}
+
+cgbdsm_status cgbdsm_index_export_static(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out) {
+    return Guard([&] {
+        if (!q || (which != 0 && which != 1)) Throw(CGBDSM_ERR_INVALID, "bad query or index selector");
+        uint64_t* base = StaticScratch(q->ctx->device);
+        // Refreshing stale GPMA flat lists must not grow the list pool.
+        if (which == 0 && q->layout == CGBDSM_LAYOUT_GPMA && q->pma.stale) PmaExtract(q, false);
+        const uint64_t nrows = (uint64_t)q->dv * q->ndir;
+        // Scratch layout (u64 words): row lengths then offsets [nrows + 1],
+        // the scan total, the scan's tile sums, then the exported keys.
+        const uint64_t w_total = nrows + 1;
+        const uint64_t w_scan = w_total + 1;
+        const uint64_t scan_words = (ScanScratchBytes(nrows + 1) + 64 + 7) / 8;
+        const uint64_t w_keys = (w_scan + scan_words + 31) & ~31ull;
+        if (w_keys > kStaticScratchWords)
+            Throw(CGBDSM_ERR_LIMIT, "index has " + std::to_string(nrows) + " rows; the static export holds at most " +
+                                        std::to_string(kStaticScratchWords * 8) + " bytes of scratch");
+        uint64_t* lens = base;
+        uint64_t* tot = base + w_total;
+        LaunchRowLens(MakeTables(q), nrows, which, lens, q->s);
+        ScanU64(lens, lens, nrows, tot, base + w_scan, q->s);
+        uint64_t total = 0;
+        CGD_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, q->s));
+        CGD_HIP(hipStreamSynchronize(q->s));
+        if (!nrows) total = 0;
+        if (n_out) *n_out = total;
+        if (w_keys + total > kStaticScratchWords)
+            Throw(CGBDSM_ERR_LIMIT, "index has " + std::to_string(total) + " entries; the static export holds at most " +
+                                        std::to_string(kStaticScratchWords - w_keys) + " for " +
+                                        std::to_string(nrows) + " rows");
+        if (total && (!out || cap < total))
+            Throw(CGBDSM_ERR_INVALID, "out holds " + std::to_string(cap) + " keys, the index has " +
+                                          std::to_string(total) + " (*n_out is set)");
+        if (total) {
+            LaunchExport(MakeTables(q), nrows, which, lens, q->ushift, base + w_keys, q->s);
+            CGD_HIP(hipMemcpyAsync(out, base + w_keys, total * 8, hipMemcpyDeviceToHost, q->s));
+            CGD_HIP(hipStreamSynchronize(q->s));
+            std::sort(out, out + total);
+        }
     });
 }
 

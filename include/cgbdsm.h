@@ -312,6 +312,20 @@ cgbdsm_status cgbdsm_kernel_times(cgbdsm_query* q, cgbdsm_ktime* out, int reset)
  * *n_out receives the total number of entries (also when cap is too small). */
 cgbdsm_status cgbdsm_index_export(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out);
 
+/* cgbdsm_index_export without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Row lengths, scan partials and the exported keys live in one
+ * static device array of 32 MiB per device (4 Mi u64 words, held until the
+ * process ends): an index whose rows + entries (+ scan partials) exceed it
+ * fails with CGBDSM_ERR_LIMIT.  out must hold every entry: cap < *n_out
+ * fails with CGBDSM_ERR_INVALID after *n_out is set.  Calls no hipMalloc /
+ * hipFree; a GPMA index whose flat lists are stale is refreshed only when
+ * that needs no pool growth (else CGBDSM_ERR_LIMIT).  Every call on one
+ * device shares the array: calls that overlap on one device (several
+ * threads or streams) race and must be serialised by the caller. */
+cgbdsm_status cgbdsm_index_export_static(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out);
+
 /* Support-mask words for (edge ei, query vertex qv): ceil(dv/64) u64 words,
  * bit v%64 of word v/64.  *all_ones = 1 for tiles that are not rebuilt. */
 cgbdsm_status cgbdsm_mask_export(cgbdsm_query* q, uint32_t ei, uint32_t qv, uint64_t* words,
