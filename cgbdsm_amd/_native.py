@@ -168,6 +168,7 @@ def dev_lib():
         L.cgbdsm_set_dfs_options.argtypes = [P, C.c_int, C.c_int]
         L.cgbdsm_set_dfs_tuning.argtypes = [P, U32, U32]
         L.cgbdsm_set_dfs_steal.argtypes = [P, U32, U32]
+        L.cgbdsm_set_dfs_search.argtypes = [P, I]
         L.cgbdsm_build_frontier.argtypes = [P]
         L.cgbdsm_query_set_layout.argtypes = [P, C.c_int]
         L.cgbdsm_profile.argtypes = [P, I]
@@ -201,4 +202,5 @@ DEV_SYMBOLS = [
     "cgbdsm_graph_create", "cgbdsm_graph_destroy", "cgbdsm_graph_set_stream", "cgbdsm_graph_apply_range",
     "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_export_lit",
     "cgbdsm_calig_old_ids", "cgbdsm_calig_stats", "cgbdsm_staged_keys", "cgbdsm_set_dfs_steal",
+    "cgbdsm_set_dfs_search",
 ]

@@ -206,9 +206,11 @@ void InsertSorted(cgbdsm_query* q, const uint64_t* keys, const unsigned long lon
     for (int attempt = 0;; attempt++) {
         SetCounter(q, &Counters::flags, 0);
         SetCounter(q, &Counters::bytes, 0);
+        SetCounter(q, &Counters::n_long, 0);
         const int k = KBegin(q, CGBDSM_K_MERGE_INSERT);
+        // q->flags (segment discovery's scan) is free again: long-list queue.
         LaunchMergeInsert(MakeTables(q), keys, q->vbits, q->seg_row.p, q->seg_start.p, n_segs, n_upper, q->sc_val.p,
-                          q->sc_pos.p, q->pool.cap, q->d_cnt, q->s);
+                          q->sc_pos.p, q->flags.p, q->pool.cap, q->d_cnt, q->s);
         KEnd(q, k);
         ReadCounters(q);
         q->batch_idx_bytes += q->h_cnt->bytes;
@@ -412,7 +414,8 @@ void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
 
 DfsJob MakeJob(cgbdsm_query* q, const uint32_t* rows, uint32_t depth) {
     return DfsJob{MakeTables(q), rows, q->d_cnt, depth,
-                  (q->dfs_shortest ? kJobShortest : 0u) | (q->dfs_suffix ? kJobSuffix : 0u)};
+                  (q->dfs_shortest ? kJobShortest : 0u) | (q->dfs_suffix ? kJobSuffix : 0u) |
+                      (q->dfs_wsearch ? kJobWaveSearch : 0u)};
 }
 
 void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
@@ -1052,9 +1055,10 @@ cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
             LaunchSegments(q->k_alt.p, Cnt(q, &Counters::n_unique), q->n_del, q->vbits, q->flags.p, q->seg_row.p,
                            q->seg_start.p, Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
             SetCounter(q, &Counters::bytes, 0);
+            SetCounter(q, &Counters::n_long, 0);
             const int k = KBegin(q, CGBDSM_K_MERGE_REMOVE);
             LaunchMergeRemove(MakeTables(q), q->k_alt.p, q->vbits, q->seg_row.p, q->seg_start.p,
-                              Cnt(q, &Counters::n_segs), q->n_del, q->d_cnt, q->s);
+                              Cnt(q, &Counters::n_segs), q->n_del, q->flags.p, q->d_cnt, q->s);
             KEnd(q, k);
             ReadCounters(q);
             q->batch_idx_bytes += q->h_cnt->bytes;
@@ -1148,6 +1152,13 @@ cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uin
         if (!q || rows_per_grab > 1024) Throw(CGBDSM_ERR_INVALID, "bad DFS tuning");
         q->dfs_grab = rows_per_grab ? rows_per_grab : 4;
         q->dfs_bpc = blocks_per_cu;
+    });
+}
+
+cgbdsm_status cgbdsm_set_dfs_search(cgbdsm_query* q, int wave_search) {
+    return Guard([&] {
+        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
+        q->dfs_wsearch = wave_search != 0;
     });
 }
 

@@ -61,6 +61,7 @@ struct alignas(128) Counters {
     unsigned long long pma_level;     // GPMA: highest rebalanced window level + 1 (since reset)
     unsigned long long wave_ticks;    // final DFS: summed wave lifetimes (wall-clock ticks)
     unsigned long long idle_ticks;    // final DFS: summed time waves waited for work
+    unsigned long long n_long;        // index merges: touched lists left to one wavefront each
     // Words polled by many wavefronts at once, one 128-byte L2 line each.
     struct alignas(128) Line {
         unsigned long long v;

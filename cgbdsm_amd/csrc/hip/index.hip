@@ -7,7 +7,15 @@
 // Set semantics on insert (as index/cpu_index_mirror.cpp:155): values already
 // present are dropped before the merge, so lists never hold duplicates.
 //
-// Merge-path per list, one wavefront per list:
+// A touched list whose old length plus delta length is at most kShortWork
+// (nearly all of them: lists hold a handful of entries) is merged by ONE lane,
+// many lists per wavefront (KMergeInsertShort / KMergeRemoveShort):
+//   insert: a forward two-pointer pass counts the delta values not yet
+//   present, then one back-to-front pass writes the merged list (in place when
+//   it fits its capacity: every store lands at or above the reads still to
+//   come; else into a fresh pool block, the untouched head copied after);
+//   remove: one forward compaction pass (stores land at or below the reads).
+// Longer lists are queued (a ballot append) for one wavefront each:
 //   1. dedup: each lane binary-searches its delta value in the old list; the
 //      survivors B' are compacted with a ballot and their final positions
 //      (rank in old list + rank in B') are recorded.
@@ -80,19 +88,167 @@ __device__ __forceinline__ uint32_t LowerBoundKeys(const uint64_t* __restrict__ 
     return lo;
 }
 
-// One wavefront per touched list: M[row] <- M[row] u B (B sorted, unique).
+constexpr uint32_t kShortWork = 48;
+
+// Appends segment s (lanes with push set) to the queue of long lists.
+__device__ __forceinline__ void PushLong(bool push, uint64_t s, uint32_t* __restrict__ long_segs,
+                                         unsigned long long* __restrict__ n_long) {
+    const uint64_t m = __ballot(push);
+    unsigned long long base = 0;
+    if (m && LaneId() == 0) base = atomicAdd(n_long, (unsigned long long)__popcll(m));
+    base = __shfl(base, 0, kWave);
+    if (push) long_segs[base + RankBelow(m)] = (uint32_t)s;
+}
+
+__device__ __forceinline__ void FoldCounters(unsigned long long bytes, unsigned long long touched,
+                                             Counters* __restrict__ cnt) {
+    bytes = __shfl(WaveInclusiveScan(bytes), 63, kWave);
+    touched = __shfl(WaveInclusiveScan(touched), 63, kWave);
+    if (LaneId() == 0) {
+        if (bytes) atomicAdd(&cnt->bytes, bytes);
+        if (touched) atomicAdd(&cnt->rows_touched, touched);
+    }
+}
+
+// One lane per short touched list: M[row] <- M[row] u B (B sorted, unique).
+__global__ void __launch_bounds__(kThreads) KMergeInsertShort(Tables T, const uint64_t* __restrict__ keys,
+                                                              uint32_t vbits, const uint32_t* __restrict__ seg_row,
+                                                              const uint64_t* __restrict__ seg_start,
+                                                              const unsigned long long* __restrict__ n_segs_dev,
+                                                              uint64_t pool_cap, uint32_t* __restrict__ long_segs,
+                                                              Counters* __restrict__ cnt) {
+    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bytes = 0, touched = 0;
+    // The base is uniform over the block, so every lane reaches the ballot.
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_segs; base += stride) {
+        const uint64_t s = base + threadIdx.x;
+        const bool valid = s < n_segs;
+        uint32_t row = 0, nb = 0;
+        uint64_t bs = 0;
+        MRec rec{0, 0, 0};
+        if (valid) {
+            row = seg_row[s];
+            bs = seg_start[s];
+            nb = (uint32_t)(seg_start[s + 1] - bs);
+            rec = T.m_rec[row];
+        }
+        const bool is_long = valid && rec.len + nb > kShortWork;
+        PushLong(is_long, s, long_segs, &cnt->n_long);
+        if (!valid || is_long) continue;
+        const uint64_t* B = keys + bs;
+        uint32_t* A = T.pool + rec.off;
+        const uint32_t na = rec.len;
+        // Forward pass: delta values not yet in the list.
+        uint32_t nbp = 0;
+        for (uint32_t j = 0, i = 0; j < nb; j++) {
+            const uint32_t b = (uint32_t)(B[j] & vmask);
+            while (i < na && A[i] < b) i++;
+            nbp += (i < na && A[i] == b) ? 0u : 1u;
+        }
+        bytes += 8ull * nb + 4ull * na;
+        if (nbp == 0) continue;
+        const uint32_t newlen = na + nbp;
+        uint32_t* C = A;
+        if (newlen > rec.cap) {
+            const uint32_t ncap = CapFor(newlen);
+            const unsigned long long off = atomicAdd(&cnt->pool_top, (unsigned long long)ncap);
+            if (off + ncap > pool_cap) {
+                atomicOr(&cnt->flags, kFlagPool);
+                continue;
+            }
+            C = T.pool + off;
+            rec.off = off;
+            rec.cap = ncap;
+        }
+        // Back-to-front merge; w - i = delta values still to place.
+        uint32_t i = na, w = newlen;
+        for (uint32_t j = nb; j > 0; j--) {
+            const uint32_t b = (uint32_t)(B[j - 1] & vmask);
+            while (i > 0) {
+                const uint32_t a = A[i - 1];
+                if (a <= b) break;
+                C[--w] = a;
+                i--;
+            }
+            if (i > 0 && A[i - 1] == b) continue;
+            C[--w] = b;
+        }
+        bytes += 8ull * (na - i) + 4ull * nbp;
+        if (C != A)
+            for (uint32_t k = 0; k < i; k++) C[k] = A[k];
+        if (C != A) bytes += 8ull * i;
+        rec.len = newlen;
+        T.m_rec[row] = rec;
+        touched++;
+    }
+    FoldCounters(bytes, touched, cnt);
+}
+
+// One lane per short touched list: M[row] <- M[row] \ D (D sorted).
+__global__ void __launch_bounds__(kThreads) KMergeRemoveShort(Tables T, const uint64_t* __restrict__ keys,
+                                                              uint32_t vbits, const uint32_t* __restrict__ seg_row,
+                                                              const uint64_t* __restrict__ seg_start,
+                                                              const unsigned long long* __restrict__ n_segs_dev,
+                                                              uint32_t* __restrict__ long_segs,
+                                                              Counters* __restrict__ cnt) {
+    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bytes = 0, touched = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_segs; base += stride) {
+        const uint64_t s = base + threadIdx.x;
+        const bool valid = s < n_segs;
+        uint32_t row = 0, nd = 0, na = 0;
+        uint64_t bs = 0, off = 0;
+        if (valid) {
+            row = seg_row[s];
+            bs = seg_start[s];
+            nd = (uint32_t)(seg_start[s + 1] - bs);
+            const MRec rec = T.m_rec[row];
+            na = rec.len;
+            off = rec.off;
+        }
+        const bool is_long = valid && na + nd > kShortWork;
+        PushLong(is_long, s, long_segs, &cnt->n_long);
+        if (!valid || is_long) continue;
+        bytes += 8ull * nd;
+        if (na == 0) continue;
+        const uint64_t* D = keys + bs;
+        uint32_t* A = T.pool + off;
+        uint32_t w = 0;
+        for (uint32_t i = 0, j = 0; i < na; i++) {
+            const uint32_t a = A[i];
+            while (j < nd && (uint32_t)(D[j] & vmask) < a) j++;
+            if (j < nd && (uint32_t)(D[j] & vmask) == a) continue;
+            if (w != i) A[w] = a;
+            w++;
+        }
+        bytes += 4ull * na + 4ull * w;
+        if (w != na) {
+            touched++;
+            T.m_rec[row].len = w;
+        }
+    }
+    FoldCounters(bytes, touched, cnt);
+}
+
+// One wavefront per long touched list: M[row] <- M[row] u B (B sorted, unique).
 __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_t* __restrict__ keys, uint32_t vbits,
                                                          const uint32_t* __restrict__ seg_row,
                                                          const uint64_t* __restrict__ seg_start,
-                                                         const unsigned long long* __restrict__ n_segs_dev,
+                                                         const uint32_t* __restrict__ long_segs,
+                                                         const unsigned long long* __restrict__ n_long_dev,
                                                          uint32_t* __restrict__ sc_val, uint32_t* __restrict__ sc_pos,
                                                          uint64_t pool_cap, Counters* __restrict__ cnt) {
     const uint32_t lane = LaneId();
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
-    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t n_long = *n_long_dev;
     const uint64_t vmask = (1ull << vbits) - 1ull;
     unsigned long long bytes = 0, touched = 0;
-    for (uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; s < n_segs; s += nw) {
+    for (uint64_t t = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; t < n_long; t += nw) {
+        const uint64_t s = long_segs[t];
         const uint32_t row = seg_row[s];
         const uint64_t bs = seg_start[s];
         const uint32_t nb = (uint32_t)(seg_start[s + 1] - bs);
@@ -180,18 +336,20 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
     }
 }
 
-// One wavefront per touched list: M[row] <- M[row] \ D (D sorted).
+// One wavefront per long touched list: M[row] <- M[row] \ D (D sorted).
 __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_t* __restrict__ keys, uint32_t vbits,
                                                          const uint32_t* __restrict__ seg_row,
                                                          const uint64_t* __restrict__ seg_start,
-                                                         const unsigned long long* __restrict__ n_segs_dev,
+                                                         const uint32_t* __restrict__ long_segs,
+                                                         const unsigned long long* __restrict__ n_long_dev,
                                                          Counters* __restrict__ cnt) {
     const uint32_t lane = LaneId();
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
-    const uint64_t n_segs = *n_segs_dev;
+    const uint64_t n_long = *n_long_dev;
     const uint64_t vmask = (1ull << vbits) - 1ull;
     unsigned long long bytes = 0, touched = 0;
-    for (uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; s < n_segs; s += nw) {
+    for (uint64_t t = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; t < n_long; t += nw) {
+        const uint64_t s = long_segs[t];
         const uint32_t row = seg_row[s];
         const uint64_t bs = seg_start[s];
         const uint32_t nd = (uint32_t)(seg_start[s + 1] - bs);
@@ -352,20 +510,25 @@ void LaunchSegments(const uint64_t* keys, const unsigned long long* n_dev, uint6
     KWriteSegs<<<GridFor(n_upper), 256, 0, s>>>(keys, n_dev, vbits, flags, n_segs_dev, seg_row, seg_start);
 }
 
+// Both merges expect cnt->n_long zeroed; long_segs holds n_upper entries.
 void LaunchMergeInsert(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
                        const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint32_t* sc_val,
-                       uint32_t* sc_pos, uint64_t pool_cap, Counters* cnt, hipStream_t s) {
+                       uint32_t* sc_pos, uint32_t* long_segs, uint64_t pool_cap, Counters* cnt, hipStream_t s) {
     if (!n_upper) return;
-    KMergeInsert<<<GridFor(n_upper, kWavesPerBlock, 16384), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
-                                                                             n_segs, sc_val, sc_pos, pool_cap, cnt);
+    KMergeInsertShort<<<GridFor(n_upper, kThreads, 8192), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
+                                                                           n_segs, pool_cap, long_segs, cnt);
+    KMergeInsert<<<GridFor(n_upper, kWavesPerBlock, 2048), kThreads, 0, s>>>(
+        T, keys, vbits, seg_row, seg_start, long_segs, &cnt->n_long, sc_val, sc_pos, pool_cap, cnt);
 }
 
 void LaunchMergeRemove(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
-                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, Counters* cnt,
-                       hipStream_t s) {
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper,
+                       uint32_t* long_segs, Counters* cnt, hipStream_t s) {
     if (!n_upper) return;
-    KMergeRemove<<<GridFor(n_upper, kWavesPerBlock, 16384), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
-                                                                             n_segs, cnt);
+    KMergeRemoveShort<<<GridFor(n_upper, kThreads, 8192), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
+                                                                           n_segs, long_segs, cnt);
+    KMergeRemove<<<GridFor(n_upper, kWavesPerBlock, 2048), kThreads, 0, s>>>(T, keys, vbits, seg_row, seg_start,
+                                                                            long_segs, &cnt->n_long, cnt);
 }
 
 void LaunchClearU(URec* u_rec, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,

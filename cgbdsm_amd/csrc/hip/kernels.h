@@ -22,12 +22,15 @@ void LaunchUnique(const uint64_t* in, uint64_t* out, uint64_t n, uint32_t* flags
 void LaunchSegments(const uint64_t* keys, const unsigned long long* n_dev, uint64_t n_upper, uint32_t vbits,
                     uint32_t* flags, uint32_t* seg_row, uint64_t* seg_start, unsigned long long* n_segs_dev,
                     void* scan_scratch, hipStream_t s);
+// Merges of sorted keys into (insert) or out of (remove) the lists they
+// touch: one lane per short list, one wavefront per long one (queued in
+// long_segs, n_upper entries; cnt->n_long must be zero at the launch).
 void LaunchMergeInsert(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
                        const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, uint32_t* sc_val,
-                       uint32_t* sc_pos, uint64_t pool_cap, Counters* cnt, hipStream_t s);
+                       uint32_t* sc_pos, uint32_t* long_segs, uint64_t pool_cap, Counters* cnt, hipStream_t s);
 void LaunchMergeRemove(const Tables& T, const uint64_t* keys, uint32_t vbits, const uint32_t* seg_row,
-                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper, Counters* cnt,
-                       hipStream_t s);
+                       const uint64_t* seg_start, const unsigned long long* n_segs, uint64_t n_upper,
+                       uint32_t* long_segs, Counters* cnt, hipStream_t s);
 void LaunchClearU(URec* u_rec, const uint32_t* u_rows, const unsigned long long* n_dev, uint64_t n_upper,
                   hipStream_t s);
 void LaunchBuildU(URec* u_rec, const uint64_t* keys, uint64_t n, uint32_t vbits, uint32_t* vals,
@@ -64,6 +67,7 @@ void LaunchExtend(const Tables& T, const uint32_t* in, uint64_t n_in, uint32_t d
 enum : uint32_t {
     kJobShortest = 1u,  // scan the shortest backward-neighbour list when lists are sets
     kJobSuffix = 2u,    // count the product suffix of the order in closed form
+    kJobWaveSearch = 4u,  // search long backward-neighbour lists with the whole wavefront
 };
 struct DfsJob {
     Tables T;

@@ -624,6 +624,83 @@ __device__ __forceinline__ bool AcceptSel(const Ctx& X, const Tables& T, uint32_
     return true;
 }
 
+// Lists at least this long are searched by the whole wavefront at once.
+constexpr uint32_t kWaveSearchMin = 256;
+
+// Membership test of every lane's candidate c (lanes with `act`) in one list
+// L shared by the wavefront (every lane calls this).  A long base part is cut
+// into 64 equal buckets: lane l loads the last entry of bucket l (64
+// independent loads, one round trip), a candidate finds its bucket among
+// those pivots with shuffles and binary-searches only inside it, which saves
+// log2(64) dependent loads per candidate.  Short parts and the appended part
+// are searched per lane as InList does.
+__device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, uint32_t& probes) {
+    bool found = false;
+    if (L.bl >= kWaveSearchMin) {
+        const uint32_t lane = LaneId();
+        const uint32_t piv = L.base[(uint32_t)(((uint64_t)(lane + 1) * L.bl) >> 6) - 1];
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t step = 32; step >= 1; step >>= 1) {
+            const uint32_t pv = __shfl(piv, (int)(pos + step - 1), kWave);
+            if (pv < c) pos += step;
+        }
+        const uint32_t top = __shfl(piv, kWave - 1, kWave);
+        if (act) {
+            probes++;
+            if (c <= top) {
+                uint32_t lo = (uint32_t)(((uint64_t)pos * L.bl) >> 6);
+                uint32_t hi = (uint32_t)(((uint64_t)(pos + 1) * L.bl) >> 6);
+                while (lo < hi) {
+                    const uint32_t m = (lo + hi) >> 1;
+                    probes++;
+                    if (L.base[m] < c) lo = m + 1;
+                    else hi = m;
+                }
+                found = lo < L.bl && L.base[lo] == c;
+            }
+        }
+    } else if (act) {
+        uint32_t lo = 0, hi = L.bl;
+        while (lo < hi) {
+            const uint32_t m = (lo + hi) >> 1;
+            probes++;
+            if (L.base[m] < c) lo = m + 1;
+            else hi = m;
+        }
+        found = lo < L.bl && L.base[lo] == c;
+    }
+    if (act && !found && L.al) {
+        uint32_t lo = 0, hi = L.al;
+        while (lo < hi) {
+            const uint32_t m = (lo + hi) >> 1;
+            probes++;
+            if (L.app[m] < c) lo = m + 1;
+            else hi = m;
+        }
+        found = lo < L.al && L.app[lo] == c;
+    }
+    return found;
+}
+
+// AcceptSel for a step of candidates, every lane calling (lanes with `act`
+// hold a candidate); the backward-neighbour lists are searched with
+// InListWave while any lane still holds an accepted candidate.
+__device__ __forceinline__ bool AcceptSelWave(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched,
+                                              uint32_t c, bool act, uint32_t skip, uint32_t ninj,
+                                              uint32_t& probes) {
+    bool ok = act && TileBit(X, T, Vs(X, d), c);
+    for (uint32_t i = 0; i < ninj; i++)
+        if (matched[i] == c) ok = false;
+    const uint32_t nbn = Nbn(X, d);
+    for (uint32_t k = 0; k < nbn; k++) {
+        if (k == skip) continue;
+        if (!__ballot(ok)) break;
+        ok = InListWave(BnList(X, T, d, k, matched), c, ok, probes);
+    }
+    return ok;
+}
+
 // Backward neighbour of position d with the shortest visible list (lowest
 // index on ties).  Scanning it instead of the first one enumerates the same
 // set when no list holds a value twice.
@@ -1139,7 +1216,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     Counters* jc = nullptr;
     Ctx X = MakeCtx(T, &s_plan[wv]);
     StabCache sc{stab, stab_cap, 0, 0, 0, 0xFFu, 0, 0};
-    bool sets = false, cart = false, bsfx = false;
+    bool sets = false, cart = false, bsfx = false, wsearch = false;
     auto flush_pe = [&]() {
         if (!jc) return;
         for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave)
@@ -1172,6 +1249,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         cps = T.P->cp_start;
         cart = !kTuples && lists_are_sets && (opts & kJobSuffix) && cps < qv;
         bsfx = cart && T.P->sfx_batch;
+        wsearch = (opts & kJobWaveSearch) != 0;
     };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     // Candidates scanned and binary-search probes: 32-bit per lane (two
@@ -1461,7 +1539,13 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             bool ok = false;
             steps++;
             step_lanes += limit - off < kWave ? limit - off : kWave;
-            if (i < limit) {
+            if (wsearch) {
+                if (i < limit) {
+                    c = At(L, i);
+                    my_scanned++;
+                }
+                ok = AcceptSelWave(X, T, d, matched, c, i < limit, sel, d, my_probes);
+            } else if (i < limit) {
                 c = At(L, i);
                 my_scanned++;
                 ok = AcceptSel(X, T, d, matched, c, sel, d, my_probes);

@@ -938,8 +938,9 @@ void GraphMerge(cgbdsm_graph* g, const uint64_t* keys, uint64_t nk) {
                    g->s);
     for (int attempt = 0;; attempt++) {
         FillU64((uint64_t*)&g->d_cnt->flags, 0, 1, g->s);
+        FillU64((uint64_t*)&g->d_cnt->n_long, 0, 1, g->s);
         LaunchMergeInsert(GraphTables(g), keys, g->vbits, g->seg_row.p, g->seg_start.p, n_s, nk, g->sc_val.p,
-                          g->sc_pos.p, g->pool.cap, g->d_cnt, g->s);
+                          g->sc_pos.p, g->flags.p, g->pool.cap, g->d_cnt, g->s);
         GReadCounters(g);
         if (!(g->h_cnt->flags & kFlagPool)) return;
         if (attempt >= 4) Throw(CGBDSM_ERR_OOM, "graph adjacency pool overflow after growth");

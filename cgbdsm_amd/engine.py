@@ -264,6 +264,10 @@ class DeviceQuery:
         _check(self.L.cgbdsm_set_dfs_tuning(self.h, int(rows_per_grab), int(blocks_per_cu)))
         self.dfs_tuning = (int(rows_per_grab), int(blocks_per_cu))
 
+    def set_dfs_search(self, wave=True):
+        """Search long backward-neighbour lists with the whole wavefront."""
+        _check(self.L.cgbdsm_set_dfs_search(self.h, 1 if wave else 0))
+
     def set_dfs_steal(self, range_min=1024, cache_cap=0):
         """Stealing chunk and stable-cache size of the final DFS (counts are
         identical for every setting)."""

@@ -264,6 +264,13 @@ cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int pro
  * reference counterpart (fixed GRID_DIM/BLOCK_DIM, utils/config.h). */
 cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu);
 
+/* Final-DFS list search (counts are identical either way; no reference
+ * counterpart): wave_search != 0 searches backward-neighbour lists of 256
+ * entries or more with the whole wavefront (each lane loads one of 64
+ * bucket pivots, each candidate then binary-searches one bucket) instead
+ * of one full binary search per lane. */
+cgbdsm_status cgbdsm_set_dfs_search(cgbdsm_query* q, int wave_search);
+
 /* Stealing chunk and LDS split of the final DFS (autotuner knobs; counts
  * are identical for every setting).  range_min = smallest candidate range a
  * wavefront splits off for idle wavefronts (0: default 1024; at least 64;
