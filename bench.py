@@ -77,6 +77,44 @@ def cpu_baseline(keys, dv, plan, batches, budget_s, threads):
     return edges, secs, counts
 
 
+def cpu_s2_sampled(keys, dv, plan, kb, threads, target_s, log=None):
+    """Oracle Stage 2 time of one batch, estimated from a uniform sample of
+    its update keys: the sample grows 4x at a time until its masks + count
+    take `target_s` seconds (or it is the whole batch); the estimate is the
+    sample's time x (update keys / sampled keys).  Index maintenance of the
+    batch (insert, remove) is timed in full.  Returns (seconds, fraction
+    sampled, count of the sample)."""
+    import numpy as np
+
+    from tests import oracle_lib as O
+
+    idx = O.Index(plan.ndir, 5)
+    idx.set_dv(dv)
+    idx.init(keys)
+    t0 = time.perf_counter()
+    idx.set_dv(kb.dv)
+    idx.insert(kb.ins)
+    idx.remove(kb.dele)
+    t_idx = time.perf_counter() - t0
+    total = int(kb.upd.size)
+    if not total:
+        return t_idx, 1.0, 0
+    perm = np.random.default_rng(0).permutation(total)
+    k = min(total, 64)
+    while True:
+        sample = np.sort(kb.upd[perm[:k]])
+        t0 = time.perf_counter()
+        idx.rebuild_update(sample)
+        masks = idx.masks(plan)
+        cnt, _ = idx.match(plan, kb.edge_ok, masks=masks, threads=threads)
+        t = time.perf_counter() - t0
+        if log:
+            log(f"cpu [S2] sample {k}/{total} update keys: {t:.2f}s, {int(cnt)} matches")
+        if t >= target_s or k == total:
+            return t_idx + t * total / k, k / total, int(cnt)
+        k = min(total, k * 4)
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy-kernel
@@ -453,32 +491,47 @@ def config4(args, log):
     specs = sample_mix(w, args.queries, args.mix_seed)
     log(f"workload: {w.n} vertices, {w.edges.shape[0]} initial edges, {w.stream.size // 2} stream edges, "
         f"{len(specs)} queries, generated in {time.perf_counter() - t_setup:.1f}s")
-    eng = Engine(0)
+    budget = int(args.budget_gb * 2**30)
+    eng = Engine(0, budget_bytes=budget)
     g = eng.graph(w.labels, w.edges)
     g.set_stream(w.stream)
+    # Plain copies of every query first (at most half the budget, so the
+    # served copies of the kept ones fit beside them with room to grow).
+    errors = {}
     t_att = time.perf_counter()
-    plain = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.probe_ms)
-    served = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.deadline_ms)
+    plain = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.probe_ms, errors=errors,
+                     stop_bytes=budget // 2)
+    for m in plain.members:
+        m.q.set_dfs_search(args.wave_search)
     attach_s = time.perf_counter() - t_att
-    log(f"{2 * len(specs)} device queries attached (CaLiG static filter + initial index on the GPU) in "
-        f"{attach_s:.1f}s")
+    log(f"{len(plain.members)} device queries attached (CaLiG static filter + initial index on the GPU) in "
+        f"{attach_s:.1f}s, {eng.memory()[0] / 2**30:.1f} GiB held")
     bi = 2 * args.batch_edges
     nb = (w.stream.size + bi - 1) // bi
-    warm = max(1, min(args.warmup, nb - 1))
+    warm = max(2, min(args.warmup, nb - 2))
     steps = max(1, min(args.steps, nb - warm))
     # Batch 0: which counts can be taken (plain, each query with the probe deadline).
     g.apply_range(0, bi)
-    errors = {}
     t0 = time.perf_counter()
     c_plain = plain.run_plain(errors)
     probe_s = time.perf_counter() - t0
+    plain.drop(set(errors))
+    kept_specs = [m.spec for m in plain.members]
+    # Served copies of the kept queries, attached to the graph after batch 0:
+    # the lit set is the greatest fixpoint on the current graph, so they
+    # start from the index the plain copies hold after committing batch 0.
+    t_att = time.perf_counter()
+    served = QuerySet(eng, g, kept_specs, layout=args.layout, deadline_ms=args.deadline_ms, errors=errors,
+                      stop_bytes=int(budget * 0.85))
+    for m in served.members:
+        m.q.set_dfs_search(args.wave_search)
+    plain.drop(set(errors))
+    attach_s += time.perf_counter() - t_att
     excluded = {k: v.split(":")[0].strip() + ": " + v.split(";")[0].split(":", 1)[-1].strip()[:80]
                 for k, v in errors.items()}
-    plain.drop(set(errors))
-    served.drop(set(errors))
     for name in errors:
         log(f"excluded {name}: {errors[name][:160]}")
-    c_plain = [c for c in c_plain if c is not None]
+    log(f"served copies of {len(served.members)} queries attached; {eng.memory()[0] / 2**30:.1f} GiB held")
     tuned = {}
 
     def tune(qs):
@@ -488,13 +541,12 @@ def config4(args, log):
         tuned.update(rows_per_grab=best[0], blocks_per_cu=best[1],
                      dfs_ms={f"{a}/{b}": round(ms, 3) for (a, b), ms in table.items()})
 
-    c_served, _ = served.run_served(tune=tune)
-    counts_equal = [c_plain == c_served]
     for m in plain.members:
         m.q.set_deadline(args.deadline_ms)
+    counts_equal = []
     for b in range(1, warm):
         g.apply_range(b * bi, bi)
-        counts_equal.append(plain.run_plain() == served.run_served()[0])
+        counts_equal.append(plain.run_plain() == served.run_served(tune=tune if b == 1 else None)[0])
     log(f"{len(plain.members)} queries kept, {len(errors)} excluded; batch 0 plain {probe_s:.2f}s")
     lat_plain, lat_served, lat_apply, dfs_ms, dfs_bytes, s1_ms, matches, edges = [], [], [], [], [], [], 0, 0
     for b in range(warm, warm + steps):
@@ -554,7 +606,18 @@ def config4(args, log):
     g.close()
     eng.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_config4(args, kept, w, log)
+        cb = cpu_baseline_config4(args, kept, w, log)
+        out["cpu_baseline"] = cb
+        # The reference's plain GPU path runs Stage 1 (CaLiG) on the host for
+        # each query: this run's plain batches with the device Stage-1 time
+        # replaced by the host Stage-1 time measured on the CPU sample.
+        host_s1 = cb["s1_ms_per_query"] * len(kept)
+        ref_ms = statistics.mean(lat_plain) - statistics.mean(s1_ms) + host_s1
+        out["plain_host_stage1_estimate"] = {
+            "update_edges_per_s": round(args.batch_edges / (1e-3 * ref_ms), 3),
+            "ms_per_batch": round(ref_ms, 3),
+            "method": f"plain ms per batch - device Stage-1 ms + {len(kept)} x host Stage-1 ms per query "
+                      f"({cb['s1_ms_per_query']} ms, mean of the CPU sample)"}
     w.close()
     return out
 
@@ -587,12 +650,13 @@ def cpu_baseline_config4(args, kept, w, log):
         t1 = time.perf_counter()
         kb = host.next_batch()
         s1 = time.perf_counter() - t1
-        edges, s2, counts = cpu_baseline(keys, dv, plan, [kb], 0.0, threads)
-        sample.append(f"{sp.name} (setup {setup:.1f}s untimed)")
+        log(f"cpu baseline {sp.name}: setup {setup:.1f}s, [S1] {1e3 * s1:.0f} ms")
+        s2, frac, _ = cpu_s2_sampled(keys, dv, plan, kb, threads, args.cpu_seconds / 6, log)
+        sample.append(f"{sp.name} (setup {setup:.1f}s untimed, [S2] from {100 * frac:.3g} % of the update keys)")
         per_q.append(s1 + s2)
         s1_tot += s1
         s2_tot += s2
-        log(f"cpu baseline {sp.name}: [S1] {1e3 * s1:.0f} ms, CPU [S2] {1e3 * s2:.0f} ms, count {counts[0]}")
+        log(f"cpu baseline {sp.name}: [S1] {1e3 * s1:.0f} ms, CPU [S2] {1e3 * s2:.0f} ms (estimated)")
         host.close()
         if s1_tot + s2_tot > args.cpu_seconds:
             break
@@ -600,7 +664,9 @@ def cpu_baseline_config4(args, kept, w, log):
     return {"value": round(args.batch_edges / (mean * len(kept)), 3), "unit": "update edges/s",
             "cores": threads, "nproc": total, "numa_nodes": nodes, "kind": "port",
             "sample": f"batch 0 ({args.batch_edges} edges) for {len(per_q)} of the {len(kept)} kept queries: "
-                      + ", ".join(sample) + f"; job rate = batch edges / (mean per-query time x {len(kept)})",
+                      + ", ".join(sample) + f"; [S2] = index maintenance + the masks and count of a uniform sample "
+                      f"of the update keys, scaled by the inverse fraction; job rate = batch edges / "
+                      f"(mean per-query time x {len(kept)})",
             "s1_ms_per_query": round(1e3 * s1_tot / len(per_q), 3),
             "s2_ms_per_query": round(1e3 * s2_tot / len(per_q), 3)}
 
@@ -715,6 +781,7 @@ def main():
     ap.add_argument("--initial-frac", type=float, default=0.9)
     ap.add_argument("--stream-edges", type=int, default=0, help="edges in the update stream (0: all the rest)")
     ap.add_argument("--queries", type=int, default=64, help="registered queries of the headline mix")
+    ap.add_argument("--budget-gb", type=float, default=240.0, help="device memory budget of the headline (GiB)")
     ap.add_argument("--mix-seed", type=int, default=1, help="seed of the headline query mix")
     ap.add_argument("--probe-ms", type=float, default=1000.0,
                     help="headline: a query whose exact count of the first batch takes longer is excluded")
@@ -732,6 +799,8 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
+    ap.add_argument("--wave-search", action="store_true",
+                    help="final DFS searches long backward-neighbour lists with the whole wavefront")
     ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
                     help="main-index layout (both give identical lists; GPMA maintains them faster)")
     ap.add_argument("--serve", action="store_true",
@@ -822,6 +891,7 @@ def main():
     q.set_deadline(args.deadline_ms)
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
     q.set_dfs_options(shortest_list=not args.no_shortest_list, product_suffix=not args.no_product)
+    q.set_dfs_search(args.wave_search)
     enq_ms = []
     for kb in batches[: warm + steps]:
         t_e = time.perf_counter()

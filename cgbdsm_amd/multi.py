@@ -24,6 +24,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .engine import CgbdsmError
+
+OOM = 2  # CGBDSM_ERR_OOM
 from .host import Planner
 
 
@@ -76,16 +78,33 @@ class Member:
 class QuerySet:
     """One device query per spec, attached to `graph` (device Stage 1)."""
 
-    def __init__(self, eng, graph, specs, layout="gpma", deadline_ms=60_000, threads=16):
+    def __init__(self, eng, graph, specs, layout="gpma", deadline_ms=60_000, threads=16, errors=None,
+                 stop_bytes=0):
+        """With `errors` (a dict), a query whose attach fails for device
+        memory (CGBDSM_ERR_OOM), or that would start once the engine holds
+        more than `stop_bytes` (0: no limit), is recorded there by name and
+        left out instead of raising."""
         self.eng, self.graph = eng, graph
         self.members = []
         self.pool = ThreadPoolExecutor(max_workers=max(1, min(threads, len(specs))))
         for sp in specs:
+            if errors is not None and stop_bytes and eng.memory()[0] > stop_bytes:
+                errors[sp.name] = (f"OOM: device memory: the engine holds {eng.memory()[0] / 2**30:.1f} GiB, "
+                                   f"over the {stop_bytes / 2**30:.1f} GiB left for this set")
+                continue
             ebits = 5 if 2 * sp.qe <= 32 else 7
             pl = Planner(sp.labels, sp.edges, ebits=ebits)
             q = eng.query(pl.query_tables)
             q.set_layout(layout)
-            card, avg = q.calig_attach(graph, sp.labels)
+            try:
+                card, avg = q.calig_attach(graph, sp.labels)
+            except CgbdsmError as e:
+                q.close()
+                pl.close()
+                if errors is None or e.code != OOM:
+                    raise
+                errors[sp.name] = str(e)
+                continue
             q.set_plan(pl.make_plan(card, avg))
             q.set_deadline(deadline_ms)
             self.members.append(Member(sp, pl, q))

@@ -17,10 +17,10 @@ walk query, device Stage 1: GAMMA and GPMA main and update indexes equal
 after every apply and commit on all 10 batches, and equal to the oracle's
 index fed with the device's own keys on the first two (the device keys
 themselves are pinned by the device-vs-host Stage-1 tests at smaller
-scales); counts equal across the layouts, and with first-list scans on the
-first two batches.  (The oracle's count of one 1 M-edge batch of this
-stream takes the CPU many minutes, so counts are pinned against the oracle
-at the smaller scales instead.)"""
+scales).  The configuration measures index maintenance; its query has more
+than 9e14 new embeddings in the first batch (the final DFS had not finished
+after 120 s), so no counts are taken here: counts are pinned against the
+oracle at the smaller scales."""
 import numpy as np
 import pytest
 
@@ -159,35 +159,20 @@ def test_config3_index_stress():
             idx.remove(staged["gamma"]["del"])
             idx.rebuild_update(staged["gamma"]["upd"])
             assert np.array_equal(m_g, idx.export(0)), b
-        counts = []
         for q in qs.values():
-            if not q.transfer_update_index():
-                counts.append((0, None))
-                q.batch_update_global_index()
-                continue
-            q.build_local_index()
-            counts.append(q.matching())
+            q.transfer_update_index()
         assert np.array_equal(qs["gamma"].export(1), qs["gpma"].export(1)), b
         if check:
             assert np.array_equal(qs["gamma"].export(1), idx.export(1)), b
-        assert counts[0][0] == counts[1][0], b
-        if check and counts[0][1] is not None:
-            # the same batch again with first-list scans (another code path)
-            q = qs["gamma"]
-            q.set_dfs_options(shortest_list=False)
-            c2, per2 = q.matching()
-            q.set_dfs_options(shortest_list=True)
-            assert c2 == counts[0][0] and np.array_equal(per2, counts[0][1]), b
         for q in qs.values():
-            if q.stats()["batches"] <= b:
-                q.batch_update_global_index()
+            q.batch_update_global_index()
         if check:
             idx.insert(staged["gamma"]["upd"])
         m_g = qs["gamma"].export(0)
         assert np.array_equal(m_g, qs["gpma"].export(0)), b
         if check:
             assert np.array_equal(m_g, idx.export(0)), b
-        print(f"config3 batch {b}: {counts[0][0]} matches, {m_g.size} index entries", flush=True)
+        print(f"config3 batch {b}: {m_g.size} index entries, {staged['gamma']['upd'].size} update keys", flush=True)
     for q in qs.values():
         q.close()
     g.close()
