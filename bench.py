@@ -502,7 +502,7 @@ def config4(args, log):
     plain = QuerySet(eng, g, specs, layout=args.layout, deadline_ms=args.probe_ms, errors=errors,
                      stop_bytes=budget // 2)
     for m in plain.members:
-        m.q.set_dfs_search(args.wave_search)
+        m.q.set_dfs_search(args.wave_search > 0, args.wave_search or 256)
     attach_s = time.perf_counter() - t_att
     log(f"{len(plain.members)} device queries attached (CaLiG static filter + initial index on the GPU) in "
         f"{attach_s:.1f}s, {eng.memory()[0] / 2**30:.1f} GiB held")
@@ -524,7 +524,7 @@ def config4(args, log):
     served = QuerySet(eng, g, kept_specs, layout=args.layout, deadline_ms=args.deadline_ms, errors=errors,
                       stop_bytes=int(budget * 0.85))
     for m in served.members:
-        m.q.set_dfs_search(args.wave_search)
+        m.q.set_dfs_search(args.wave_search > 0, args.wave_search or 256)
     plain.drop(set(errors))
     attach_s += time.perf_counter() - t_att
     excluded = {k: v.split(":")[0].strip() + ": " + v.split(";")[0].split(":", 1)[-1].strip()[:80]
@@ -799,8 +799,9 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
-    ap.add_argument("--wave-search", action="store_true",
-                    help="final DFS searches long backward-neighbour lists with the whole wavefront")
+    ap.add_argument("--wave-search", type=int, default=0,
+                    help="final DFS searches backward-neighbour lists of at least this many entries (a power "
+                         "of two >= 64) with the whole wavefront; 0: off")
     ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
                     help="main-index layout (both give identical lists; GPMA maintains them faster)")
     ap.add_argument("--serve", action="store_true",
@@ -891,7 +892,7 @@ def main():
     q.set_deadline(args.deadline_ms)
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
     q.set_dfs_options(shortest_list=not args.no_shortest_list, product_suffix=not args.no_product)
-    q.set_dfs_search(args.wave_search)
+    q.set_dfs_search(args.wave_search > 0, args.wave_search or 256)
     enq_ms = []
     for kb in batches[: warm + steps]:
         t_e = time.perf_counter()

@@ -182,6 +182,7 @@ def dev_lib():
         L.cgbdsm_graph_info.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
         L.cgbdsm_calig_attach.argtypes = [P, P, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cgbdsm_calig_update.argtypes = [P]
+        L.cgbdsm_calig_match_deleted.argtypes = [P, C.POINTER(C.c_ulonglong), C.c_void_p]
         L.cgbdsm_calig_export_lit.argtypes = [P, C.c_void_p, U32]
         L.cgbdsm_calig_old_ids.argtypes = [P, C.c_void_p, U32, C.POINTER(U32)]
         L.cgbdsm_calig_stats.argtypes = [P, C.POINTER(C.c_double), C.c_void_p]
@@ -200,7 +201,8 @@ DEV_SYMBOLS = [
     "cgbdsm_profile", "cgbdsm_query_set_layout", "cgbdsm_set_dfs_tuning", "cgbdsm_build_frontier",
     "cgbdsm_kernel_times", "cgbdsm_batch_enqueue", "cgbdsm_batch_stage_next", "cgbdsm_index_export_static",
     "cgbdsm_graph_create", "cgbdsm_graph_destroy", "cgbdsm_graph_set_stream", "cgbdsm_graph_apply_range",
-    "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_export_lit",
+    "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_match_deleted",
+    "cgbdsm_calig_export_lit",
     "cgbdsm_calig_old_ids", "cgbdsm_calig_stats", "cgbdsm_staged_keys", "cgbdsm_set_dfs_steal",
     "cgbdsm_set_dfs_search",
 ]

@@ -72,7 +72,7 @@ struct cgbdsm_query {
     // Smallest candidate range a DFS wavefront splits off for idle ones, and
     // the cap on the per-wave stable-intersection cache (0: all LDS left).
     uint32_t dfs_range_min = 1024, dfs_cache_cap = 0;
-    bool dfs_wsearch = false;  // long backward-neighbour lists searched by the whole wavefront
+    uint32_t dfs_wsearch = 0;  // 0, or log2 of the shortest backward-neighbour list searched by the whole wavefront
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;

@@ -415,7 +415,7 @@ void RunDfsJobs(cgbdsm_query* q, uint32_t* tuples, uint64_t tup_cap) {
 DfsJob MakeJob(cgbdsm_query* q, const uint32_t* rows, uint32_t depth) {
     return DfsJob{MakeTables(q), rows, q->d_cnt, depth,
                   (q->dfs_shortest ? kJobShortest : 0u) | (q->dfs_suffix ? kJobSuffix : 0u) |
-                      (q->dfs_wsearch ? kJobWaveSearch : 0u)};
+                      (q->dfs_wsearch ? kJobWaveSearch | (q->dfs_wsearch << kJobWaveShift) : 0u)};
 }
 
 void RunDfs(cgbdsm_query* q, const uint32_t* rows, uint64_t n, uint32_t depth, uint32_t* tuples, uint64_t tup_cap) {
@@ -1157,8 +1157,13 @@ cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uin
 
 cgbdsm_status cgbdsm_set_dfs_search(cgbdsm_query* q, int wave_search) {
     return Guard([&] {
-        if (!q) Throw(CGBDSM_ERR_INVALID, "null query");
-        q->dfs_wsearch = wave_search != 0;
+        // 64 buckets of at least one entry each: lists of 64 entries or more.
+        if (!q || wave_search < 0 || (wave_search > 1 && (wave_search < 64 || (wave_search & (wave_search - 1)))))
+            Throw(CGBDSM_ERR_INVALID, "wave_search must be 0, 1 or a power of two >= 64");
+        uint32_t lg = 0;
+        if (wave_search == 1) lg = 8;  // lists of 256 entries or more
+        while (wave_search > 1 && (1 << lg) < wave_search) lg++;
+        q->dfs_wsearch = (uint32_t)lg;
     });
 }
 

@@ -68,6 +68,7 @@ enum : uint32_t {
     kJobShortest = 1u,  // scan the shortest backward-neighbour list when lists are sets
     kJobSuffix = 2u,    // count the product suffix of the order in closed form
     kJobWaveSearch = 4u,  // search long backward-neighbour lists with the whole wavefront
+    kJobWaveShift = 8u,   // bits 8..12: log2 of the shortest list searched so (0: 256)
 };
 struct DfsJob {
     Tables T;

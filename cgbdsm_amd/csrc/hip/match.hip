@@ -634,9 +634,9 @@ constexpr uint32_t kWaveSearchMin = 256;
 // those pivots with shuffles and binary-searches only inside it, which saves
 // log2(64) dependent loads per candidate.  Short parts and the appended part
 // are searched per lane as InList does.
-__device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, uint32_t& probes) {
+__device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, uint32_t& probes, uint32_t wmin) {
     bool found = false;
-    if (L.bl >= kWaveSearchMin) {
+    if (L.bl >= wmin) {
         const uint32_t lane = LaneId();
         const uint32_t piv = L.base[(uint32_t)(((uint64_t)(lane + 1) * L.bl) >> 6) - 1];
         uint32_t pos = 0;
@@ -688,7 +688,7 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
 // InListWave while any lane still holds an accepted candidate.
 __device__ __forceinline__ bool AcceptSelWave(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched,
                                               uint32_t c, bool act, uint32_t skip, uint32_t ninj,
-                                              uint32_t& probes) {
+                                              uint32_t& probes, uint32_t wmin) {
     bool ok = act && TileBit(X, T, Vs(X, d), c);
     for (uint32_t i = 0; i < ninj; i++)
         if (matched[i] == c) ok = false;
@@ -696,7 +696,7 @@ __device__ __forceinline__ bool AcceptSelWave(const Ctx& X, const Tables& T, uin
     for (uint32_t k = 0; k < nbn; k++) {
         if (k == skip) continue;
         if (!__ballot(ok)) break;
-        ok = InListWave(BnList(X, T, d, k, matched), c, ok, probes);
+        ok = InListWave(BnList(X, T, d, k, matched), c, ok, probes, wmin);
     }
     return ok;
 }
@@ -1217,6 +1217,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     Ctx X = MakeCtx(T, &s_plan[wv]);
     StabCache sc{stab, stab_cap, 0, 0, 0, 0xFFu, 0, 0};
     bool sets = false, cart = false, bsfx = false, wsearch = false;
+    uint32_t wmin = kWaveSearchMin;
     auto flush_pe = [&]() {
         if (!jc) return;
         for (uint32_t t = lane; t < (uint32_t)kMaxQE; t += kWave)
@@ -1250,6 +1251,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         cart = !kTuples && lists_are_sets && (opts & kJobSuffix) && cps < qv;
         bsfx = cart && T.P->sfx_batch;
         wsearch = (opts & kJobWaveSearch) != 0;
+        // At least 64 entries: every lane's bucket pivot index is >= 0.
+        wmin = (opts >> kJobWaveShift) & 31u ? 1u << ((opts >> kJobWaveShift) & 31u) : kWaveSearchMin;
+        wmin = wmin < (uint32_t)kWave ? (uint32_t)kWave : wmin;
     };
     uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
     // Candidates scanned and binary-search probes: 32-bit per lane (two
@@ -1544,7 +1548,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                     c = At(L, i);
                     my_scanned++;
                 }
-                ok = AcceptSelWave(X, T, d, matched, c, i < limit, sel, d, my_probes);
+                ok = AcceptSelWave(X, T, d, matched, c, i < limit, sel, d, my_probes, wmin);
             } else if (i < limit) {
                 c = At(L, i);
                 my_scanned++;

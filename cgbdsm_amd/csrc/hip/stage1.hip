@@ -88,7 +88,7 @@ struct alignas(128) S1Cnt {
     unsigned long long n_died, n_big, n_keys, n_upd, ovf, items, scan_total, newv;
     unsigned long long dirmask[2];  // directed query edges that received an update key
     unsigned long long pairs;       // label-compatible pairs (attach)
-    unsigned long long pad[1];
+    unsigned long long n_killed;    // pairs lit before the batch that died in it
 };
 
 namespace {
@@ -188,19 +188,28 @@ __device__ __forceinline__ LbItem LbLocate(const uint64_t* __restrict__ offs, ui
 // ---------------------------------------------------------------- graph kernels
 // Both orientations of every valid pair of stream ints [start, start + 2 * np)
 // as keys (src << vbits) | dst (invalid pairs: ~0, sorted to the end).
+// Both directions of every pair of the batch: insertions into keys, and
+// deletions -- a pair of negative ids (-x-1, -y-1) removes edge (x, y)
+// (index/calig.cpp:980-989) -- into dkeys; ~0 for the other kind and for
+// pairs that name no valid edge.
 __global__ void KGraphPairKeys(const int32_t* __restrict__ ints, uint64_t start, uint64_t np, uint32_t n,
-                               uint32_t vbits, uint64_t* __restrict__ keys) {
+                               uint32_t vbits, uint64_t* __restrict__ keys, uint64_t* __restrict__ dkeys) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (uint64_t)gridDim.x * blockDim.x) {
         const int32_t a = ints[start + 2 * i], b = ints[start + 2 * i + 1];
         const bool ok = a >= 0 && b >= 0 && (uint32_t)a < n && (uint32_t)b < n && a != b;
         keys[2 * i] = ok ? ((uint64_t)a << vbits) | (uint32_t)b : ~0ull;
         keys[2 * i + 1] = ok ? ((uint64_t)b << vbits) | (uint32_t)a : ~0ull;
+        const int64_t x = -(int64_t)a - 1, y = -(int64_t)b - 1;
+        const bool del = a < 0 && b < 0 && x < (int64_t)n && y < (int64_t)n && x != y;
+        dkeys[2 * i] = del ? ((uint64_t)x << vbits) | (uint64_t)y : ~0ull;
+        dkeys[2 * i + 1] = del ? ((uint64_t)y << vbits) | (uint64_t)x : ~0ull;
     }
 }
 
-// keep[i] = sorted unique key i is a valid edge absent from the graph.
+// keep[i] = sorted unique key i is a valid edge absent from (present: in)
+// the graph.
 __global__ void KGraphAbsent(const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ n_dev,
-                             uint64_t n_upper, uint32_t vbits, GView G, uint32_t* __restrict__ keep) {
+                             uint64_t n_upper, uint32_t vbits, GView G, uint32_t present, uint32_t* __restrict__ keep) {
     const uint64_t n = *n_dev;
     const uint64_t vmask = (1ull << vbits) - 1ull;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_upper;
@@ -209,7 +218,7 @@ __global__ void KGraphAbsent(const uint64_t* __restrict__ keys, const unsigned l
         if (i < n && keys[i] != ~0ull) {
             const uint32_t a = (uint32_t)(keys[i] >> vbits), b = (uint32_t)(keys[i] & vmask);
             const MRec r = G.rec[a];
-            k = SortedContains(G.pool + r.off, r.len, b) ? 0u : 1u;
+            k = SortedContains(G.pool + r.off, r.len, b) == (present != 0) ? 1u : 0u;
         }
         keep[i] = k;
     }
@@ -474,13 +483,169 @@ __global__ void __launch_bounds__(kThreads) KS1CheckWave(GView G, QTab Q, SView 
 }
 
 // Deaths of a round: clear the lit (and activated) bit, record the dead mask.
+// With `killed`, deaths of pairs lit before the batch (possible only when
+// the batch deletes edges) are also listed there and marked in bits 16..27
+// of `queued` (their index entries leave the main index).
 __global__ void KS1Apply(QTab Q, SView S, const uint32_t* __restrict__ died, const uint16_t* __restrict__ bits,
-                         const unsigned long long* __restrict__ n_dev) {
+                         const unsigned long long* __restrict__ n_dev, uint32_t* __restrict__ killed,
+                         unsigned long long* __restrict__ n_killed) {
+    const uint64_t n = *n_dev;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        bool old_pair = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = died[i];
+            const uint32_t v = p >> 4, ui = p & 15u;
+            const uint32_t old = atomicAnd(&S.vstate[v], ~((1u << ui) | (1u << (16 + ui))));
+            DeadSet(S.dead, v, ui, Q.qv, bits[i]);
+            if (killed && !((Opt(old) >> ui) & 1u)) {
+                old_pair = true;
+                atomicOr(&S.queued[v], 1u << (16 + ui));
+            }
+        }
+        if (killed) {
+            const uint64_t at = WaveReserve(old_pair ? 1u : 0u, n_killed);
+            if (old_pair) killed[at] = p;
+        }
+    }
+}
+
+// Deleted edge a -> b: lit pairs (a, ui) from before the batch with a query
+// neighbour labelled like b may have lost their support; queued for the
+// deletion rounds (pairs lit in this batch are on the list already).
+__global__ void KS1SeedDeleted(QTab Q, SView S, const uint64_t* __restrict__ dkeys, uint64_t nd, uint32_t vbits,
+                               uint32_t* __restrict__ out, unsigned long long* __restrict__ n_out) {
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nd; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t a = 0, cand = 0;
+        if (i < nd) {
+            a = (uint32_t)(dkeys[i] >> vbits);
+            const uint32_t sa = S.vstate[a], cb = Code(S.vstate[(uint32_t)(dkeys[i] & vmask)]);
+            if (cb != kCodeNone) {
+                uint32_t l = Lit(sa) & ~Opt(sa);
+                while (l) {
+                    const uint32_t u = __ffs(l) - 1;
+                    l &= l - 1;
+                    if (Q.uj_of[u][cb]) cand |= 1u << u;
+                }
+            }
+        }
+        for (uint32_t u = 0; u < Q.qv; u++) {
+            const bool want = ((cand >> u) & 1u) && Enqueue(S.queued, a, u);
+            const uint64_t at = WaveReserve(want ? 1u : 0u, n_out);
+            if (want) out[at] = (a << 4) | u;
+        }
+    }
+}
+
+// Deleted edge a -> b: the main-index entries it carried, a -> b of
+// direction (ui, uj) for every pair (a, ui), (b, uj) lit before the batch
+// (the reverse entries come from the key b -> a).  Run before the deletion
+// rounds, while lit & ~activated is still the lit set before the batch.
+__global__ void KS1DelEdgeKeys(QTab Q, SView S, const uint64_t* __restrict__ dkeys, uint64_t nd, uint32_t vbits,
+                               uint64_t* __restrict__ keys, uint64_t cap, unsigned long long* __restrict__ n_keys,
+                               unsigned long long* __restrict__ ovf, unsigned long long* __restrict__ dirmask) {
+    const uint64_t vmask = (1ull << vbits) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long dm0 = 0, dm1 = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nd; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t a = 0, b = 0, la = 0, lb = 0, cb = kCodeNone, cnt = 0;
+        if (i < nd) {
+            a = (uint32_t)(dkeys[i] >> vbits);
+            b = (uint32_t)(dkeys[i] & vmask);
+            const uint32_t sa = S.vstate[a], sb = S.vstate[b];
+            cb = Code(sb);
+            la = Lit(sa) & ~Opt(sa);
+            lb = Lit(sb) & ~Opt(sb);
+            if (cb != kCodeNone) {
+                uint32_t l = la;
+                while (l) {
+                    const uint32_t u = __ffs(l) - 1;
+                    l &= l - 1;
+                    cnt += (uint32_t)__popc(Q.uj_of[u][cb] & lb);
+                }
+            }
+        }
+        uint64_t at = WaveReserve(cnt, n_keys);
+        if (cnt) {
+            if (at + cnt > cap) {
+                atomicOr(ovf, 1ull);
+            } else {
+                const uint64_t na = S.new_of[a], nb = S.new_of[b];
+                uint32_t l = la;
+                while (l) {
+                    const uint32_t ui = __ffs(l) - 1;
+                    l &= l - 1;
+                    uint32_t m = Q.uj_of[ui][cb] & lb;
+                    while (m) {
+                        const uint32_t uj = __ffs(m) - 1;
+                        m &= m - 1;
+                        const uint32_t d = Q.eidx[ui][uj];
+                        keys[at++] = ((uint64_t)d << Q.ushift) | (na << 32) | nb;
+                        if (d < 64) dm0 |= 1ull << d;
+                        else dm1 |= 1ull << (d - 64);
+                    }
+                }
+            }
+        }
+    }
+    if (dm0) atomicOr(&dirmask[0], dm0);
+    if (dm1) atomicOr(&dirmask[1], dm1);
+}
+
+// Pairs lit before the batch that died in it: the main-index entries they
+// shared with the pairs lit before the batch, both directions, over the
+// current adjacency (entries over deleted edges come from KS1DelEdgeKeys).
+// When both ends died, the end with the smaller vertex emits them.
+__global__ void __launch_bounds__(kThreads) KS1EmitKilled(GView G, QTab Q, SView S, const uint32_t* __restrict__ K,
+                                                          const uint64_t* __restrict__ offs, uint64_t np,
+                                                          uint64_t* __restrict__ keys, uint64_t cap,
+                                                          unsigned long long* __restrict__ n_keys,
+                                                          unsigned long long* __restrict__ ovf) {
+    const uint64_t total = offs[np];
+    const uint64_t nw = (uint64_t)gridDim.x * kWpb;
+    for (uint64_t t = (uint64_t)blockIdx.x * kWpb + threadIdx.x / kWave; t * kWave < total; t += nw) {
+        const LbItem it = LbLocate(offs, np, t * kWave, total);
+        uint32_t m = 0, v = 0, ui = 0, vj = 0;
+        if (it.valid) {
+            const uint32_t p = K[it.p];
+            v = p >> 4;
+            ui = p & 15u;
+            vj = G.pool[G.rec[v].off + it.j];
+            const uint32_t s = S.vstate[vj];
+            const uint32_t c = Code(s);
+            const uint32_t kb = (S.queued[vj] >> 16) & 0xFFFu;
+            m = c == kCodeNone ? 0u : (Q.uj_of[ui][c] & ((Lit(s) & ~Opt(s)) | kb));
+            if (vj < v) m &= ~kb;
+        }
+        const uint32_t cnt = 2u * (uint32_t)__popc(m);
+        uint64_t at = WaveReserve(cnt, n_keys);
+        if (cnt) {
+            if (at + cnt > cap) {
+                atomicOr(ovf, 1ull);
+            } else {
+                const uint64_t nu = S.new_of[v], nv = S.new_of[vj];
+                while (m) {
+                    const uint32_t uj = __ffs(m) - 1;
+                    m &= m - 1;
+                    keys[at++] = ((uint64_t)Q.eidx[ui][uj] << Q.ushift) | (nu << 32) | nv;
+                    keys[at++] = ((uint64_t)Q.eidx[uj][ui] << Q.ushift) | (nv << 32) | nu;
+                }
+            }
+        }
+    }
+}
+
+__global__ void KS1ClearKilled(SView S, const uint32_t* __restrict__ K, const unsigned long long* __restrict__ n_dev) {
     const uint64_t n = *n_dev;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t p = died[i], v = p >> 4, ui = p & 15u;
-        atomicAnd(&S.vstate[v], ~((1u << ui) | (1u << (16 + ui))));
-        DeadSet(S.dead, v, ui, Q.qv, bits[i]);
+        const uint32_t p = K[i];
+        atomicAnd(&S.queued[p >> 4], ~(1u << (16 + (p & 15u))));
     }
 }
 
@@ -608,8 +773,9 @@ __global__ void __launch_bounds__(kThreads) KS1Push(GView G, QTab Q, SView S, co
     }
 }
 
-// Pairs of `opt` still lit (the batch's new pairs) -> Y; their vertices with
-// no other lit bit from before the batch are new to the compressed space.
+// Pairs of `opt` still lit (the batch's new pairs) -> Y; their vertices
+// without a compressed id yet are new to the compressed space (a vertex
+// keeps its id when deletions leave it without lit pairs).
 __global__ void KS1Survivors(SView S, const uint32_t* __restrict__ opt, const unsigned long long* __restrict__ n_dev,
                              uint32_t* __restrict__ Y, unsigned long long* __restrict__ n_y,
                              uint32_t* __restrict__ newv) {
@@ -625,7 +791,7 @@ __global__ void KS1Survivors(SView S, const uint32_t* __restrict__ opt, const un
             const uint32_t s = S.vstate[v];
             if ((Opt(s) >> u) & 1u) {
                 keep = true;
-                if ((Lit(s) & ~Opt(s)) == 0) atomicOr(&newv[v >> 5], 1u << (v & 31));
+                if (S.new_of[v] == ~0u) atomicOr(&newv[v >> 5], 1u << (v & 31));
             }
         }
         const uint64_t at = WaveReserve(keep ? 1u : 0u, n_y);
@@ -818,6 +984,8 @@ struct cgbdsm_graph {
     uint64_t b_start = 0, b_pairs = 0;  // the latest batch: ints [b_start, b_start + 2 * b_pairs)
     DBuf<uint64_t> newe;           // its new directed edges, sorted (src << vbits) | dst
     uint64_t n_new = 0;
+    DBuf<uint64_t> dele, kd;       // its deleted directed edges (sorted, as newe); deletion key scratch
+    uint64_t n_dele = 0;
     DBuf<uint64_t> k, alt, seg_start;
     DBuf<uint32_t> flags, seg_row, sc_val, sc_pos, hist;
     DBuf<uint8_t> scan;
@@ -839,6 +1007,10 @@ struct S1Query {
     DBuf<uint32_t> L2[2];          // activation: touched list of the next round, activated list
     DBuf<uint16_t> died_bits;
     DBuf<uint64_t> offs;
+    DBuf<uint64_t> kd;  // main-index entries a batch's deletions remove: [deleted edges | dead pairs]
+    uint64_t n_dedge = 0;              // entries of the deleted edges (the negative update keys)
+    uint8_t neg_ok[kMaxQE] = {};       // edge_ok of the negative pass
+    DBuf<uint64_t> sv_ins, sv_del, sv_upd;  // the positive batch's key buffers during the negative pass
     S1Cnt* d_c = nullptr;
     S1Cnt* h_c = nullptr;
     uint32_t dv = 0;
@@ -928,6 +1100,18 @@ void GraphLayout(cgbdsm_graph* g, uint64_t nk) {
                      g->alt.p, g->d_cnt, g->s);
 }
 
+// Removes the sorted deleted-edge keys (all present) from the adjacency lists.
+void GraphRemove(cgbdsm_graph* g, const uint64_t* keys, uint64_t nk) {
+    if (!nk) return;
+    unsigned long long* n_s = &g->d_cnt->n_segs;
+    FillU64((uint64_t*)&g->d_cnt->n_unique, nk, 1, g->s);
+    LaunchSegments(keys, &g->d_cnt->n_unique, nk, g->vbits, g->flags.p, g->seg_row.p, g->seg_start.p, n_s, g->scan.p,
+                   g->s);
+    FillU64((uint64_t*)&g->d_cnt->n_long, 0, 1, g->s);
+    LaunchMergeRemove(GraphTables(g), keys, g->vbits, g->seg_row.p, g->seg_start.p, n_s, nk, g->flags.p, g->d_cnt,
+                      g->s);
+}
+
 // Inserts the sorted new-edge keys into the adjacency lists (pool grows and
 // the merge is relaunched on overflow; merged lists see nothing left).
 void GraphMerge(cgbdsm_graph* g, const uint64_t* keys, uint64_t nk) {
@@ -983,7 +1167,8 @@ unsigned LbGrid(cgbdsm_query* q) { return (unsigned)q->ctx->num_cus * 8; }
 // Deletion rounds from list L[src] (n pairs, all lit): check, kill the pairs
 // that fail, queue the pairs their deaths may break, repeat until a round
 // kills nothing.  opt_only: only pairs lit in this batch are re-checked.
-void S1DeletionRounds(cgbdsm_query* q, int src, uint64_t n, bool opt_only) {
+// killed_list: list (and mark) the deaths of pairs lit before the batch.
+void S1DeletionRounds(cgbdsm_query* q, int src, uint64_t n, bool opt_only, uint32_t* killed_list = nullptr) {
     S1Query* S = q->s1;
     const GView G = View(S->g);
     const SView V = SV(S);
@@ -999,7 +1184,8 @@ void S1DeletionRounds(cgbdsm_query* q, int src, uint64_t n, bool opt_only) {
                                                                  S->L[2].p, S->died_bits.p, &S->d_c->n_died);
         // The deaths (at most n) are applied and their neighbours queued with
         // the device count: one host round trip per round.
-        KS1Apply<<<Grid(n), 256, 0, q->s>>>(S->tab, V, S->L[2].p, S->died_bits.p, &S->d_c->n_died);
+        KS1Apply<<<Grid(n), 256, 0, q->s>>>(S->tab, V, S->L[2].p, S->died_bits.p, &S->d_c->n_died,
+                                            killed_list, &S->d_c->n_killed);
         S1Offsets(q, S->L[2].p, n, &S->d_c->n_died);
         const int nxt = cur == 0 ? 1 : 0;
         FillU64((uint64_t*)&S->d_c->n_list[nxt], 0, 1, q->s);
@@ -1034,6 +1220,52 @@ uint64_t S1EmitPairs(cgbdsm_query* q, const uint32_t* Y, uint64_t n, bool both) 
         if (attempt) Throw(CGBDSM_ERR_DEVICE, "index key emission overflowed twice");
         Ensure(q, q->k_ins, nk + 1024);
     }
+}
+
+// Main-index entries over the graph's deleted edges (lit set before the
+// batch) -> S->kd[0..); returns their number.
+uint64_t S1EmitDelEdges(cgbdsm_query* q) {
+    S1Query* S = q->s1;
+    const cgbdsm_graph* g = S->g;
+    Ensure(q, S->kd, std::max<uint64_t>(S->kd.cap, 1024));
+    for (int attempt = 0;; attempt++) {
+        S1Set(q, &S1Cnt::n_keys, 0);
+        S1Set(q, &S1Cnt::ovf, 0);
+        FillU64((uint64_t*)&S->d_c->dirmask[0], 0, 2, q->s);
+        KS1DelEdgeKeys<<<Grid(g->n_dele), 256, 0, q->s>>>(S->tab, SV(S), g->dele.p, g->n_dele, g->vbits, S->kd.p,
+                                                          S->kd.cap, &S->d_c->n_keys, &S->d_c->ovf,
+                                                          &S->d_c->dirmask[0]);
+        S1Read(q);
+        const uint64_t nk = S->h_c->n_keys;
+        if (!S->h_c->ovf) {
+            auto on = [&](uint32_t d) { return (S->h_c->dirmask[d >> 6] >> (d & 63)) & 1ull; };
+            for (uint32_t e = 0; e < q->qe; e++) S->neg_ok[e] = on(q->hplan.sa[e]) && on(q->hplan.sb[e]) ? 1 : 0;
+            return nk;
+        }
+        if (attempt) Throw(CGBDSM_ERR_DEVICE, "deleted-edge key emission overflowed twice");
+        Ensure(q, S->kd, nk + 1024);
+    }
+}
+
+// Entries of the nk killed pairs in L2[1] -> S->kd[base..); clears their
+// marks; returns the new end of S->kd.
+uint64_t S1EmitKilled(cgbdsm_query* q, uint64_t base, uint64_t nk) {
+    S1Query* S = q->s1;
+    S1Offsets(q, S->L2[1].p, nk);
+    uint64_t end = base;
+    for (int attempt = 0;; attempt++) {
+        FillU64((uint64_t*)&S->d_c->n_keys, base, 1, q->s);
+        S1Set(q, &S1Cnt::ovf, 0);
+        KS1EmitKilled<<<LbGrid(q), kThreads, 0, q->s>>>(View(S->g), S->tab, SV(S), S->L2[1].p, S->offs.p, nk, S->kd.p,
+                                                       S->kd.cap, &S->d_c->n_keys, &S->d_c->ovf);
+        S1Read(q);
+        end = S->h_c->n_keys;
+        if (!S->h_c->ovf) break;
+        if (attempt) Throw(CGBDSM_ERR_DEVICE, "killed-pair key emission overflowed twice");
+        Ensure(q, S->kd, end + 1024, true, base);
+    }
+    KS1ClearKilled<<<Grid(nk), 256, 0, q->s>>>(SV(S), S->L2[1].p, &S->d_c->n_killed);
+    return end;
 }
 
 // New compressed ids for the vertices flagged in S->newv; returns the new dv.
@@ -1094,6 +1326,7 @@ void S1Free(cgbdsm_query* q) {
         Release(q, *b);
     Release(q, S->died_bits);
     Release(q, S->offs);
+    for (auto* b : {&S->kd, &S->sv_ins, &S->sv_del, &S->sv_upd}) Release(q, *b);
     if (S->d_c) DevFree(q->ctx, S->d_c, sizeof(S1Cnt));
     if (S->h_c) (void)hipHostFree(S->h_c);
     delete S;
@@ -1184,7 +1417,7 @@ cgbdsm_status cgbdsm_graph_destroy(cgbdsm_graph* g) {
         GRelease(c, g->pool);
         GRelease(c, g->stream);
         GRelease(c, g->newe);
-        for (auto* b : {&g->k, &g->alt, &g->seg_start}) GRelease(c, *b);
+        for (auto* b : {&g->k, &g->alt, &g->seg_start, &g->dele, &g->kd}) GRelease(c, *b);
         for (auto* b : {&g->flags, &g->seg_row, &g->sc_val, &g->sc_pos, &g->hist}) GRelease(c, *b);
         GRelease(c, g->scan);
         if (g->d_cnt) DevFree(c, g->d_cnt, sizeof(Counters));
@@ -1219,23 +1452,38 @@ cgbdsm_status cgbdsm_graph_apply_range(cgbdsm_graph* g, uint64_t start, uint64_t
         const uint64_t np = (end - start) / 2;  // main_calig.cpp:220-222 / calig.cpp:973-976
         const uint64_t nk = 2 * np;
         GEnsure(g, g->k, std::max<uint64_t>(nk, 1));
+        GEnsure(g, g->kd, std::max<uint64_t>(nk, 1));
         GKeyScratch(g, nk);
         GEnsure(g, g->newe, std::max<uint64_t>(nk, 1));
-        uint64_t n_new = 0;
+        GEnsure(g, g->dele, std::max<uint64_t>(nk, 1));
+        uint64_t n_new = 0, n_dele = 0;
         if (np) {
-            KGraphPairKeys<<<Grid(np), 256, 0, g->s>>>(g->stream.p, start, np, g->n, g->vbits, g->k.p);
-            RadixSortU64(g->k.p, g->alt.p, nk, 2 * g->vbits + 1, g->hist.p, g->s);
+            KGraphPairKeys<<<Grid(np), 256, 0, g->s>>>(g->stream.p, start, np, g->n, g->vbits, g->k.p, g->kd.p);
+            // Both kinds are judged against the graph before the batch
+            // (calig.cpp:980-998): insertions of absent edges, deletions of
+            // present ones; so no edge is both.
+            const uint32_t kb = 2 * g->vbits + 1;
+            RadixSortU64(g->k.p, g->alt.p, nk, kb, g->hist.p, g->s);
             LaunchUnique(g->k.p, g->alt.p, nk, g->flags.p, g->scan.p, &g->d_cnt->n_unique, g->s);
-            KGraphAbsent<<<Grid(nk), 256, 0, g->s>>>(g->alt.p, &g->d_cnt->n_unique, nk, g->vbits, View(g),
+            KGraphAbsent<<<Grid(nk), 256, 0, g->s>>>(g->alt.p, &g->d_cnt->n_unique, nk, g->vbits, View(g), 0u,
                                                      g->sc_val.p);
             ScanU32(g->sc_val.p, g->sc_pos.p, nk, (uint64_t*)&g->d_cnt->n_segs, g->scan.p, g->s);
             KCompactKeys<<<Grid(nk), 256, 0, g->s>>>(g->alt.p, nk, g->sc_val.p, g->sc_pos.p, g->newe.p);
+            RadixSortU64(g->kd.p, g->alt.p, nk, kb, g->hist.p, g->s);
+            LaunchUnique(g->kd.p, g->alt.p, nk, g->flags.p, g->scan.p, &g->d_cnt->n_unique, g->s);
+            KGraphAbsent<<<Grid(nk), 256, 0, g->s>>>(g->alt.p, &g->d_cnt->n_unique, nk, g->vbits, View(g), 1u,
+                                                     g->sc_val.p);
+            ScanU32(g->sc_val.p, g->sc_pos.p, nk, (uint64_t*)&g->d_cnt->u_rows, g->scan.p, g->s);
+            KCompactKeys<<<Grid(nk), 256, 0, g->s>>>(g->alt.p, nk, g->sc_val.p, g->sc_pos.p, g->dele.p);
             GReadCounters(g);
             n_new = g->h_cnt->n_segs;
+            n_dele = g->h_cnt->u_rows;
+            GraphRemove(g, g->dele.p, n_dele);
             GraphMerge(g, g->newe.p, n_new);
         }
         g->n_new = n_new;
-        g->m_dir += n_new;
+        g->n_dele = n_dele;
+        g->m_dir = g->m_dir + n_new - n_dele;
         g->b_start = start;
         g->b_pairs = np;
         g->batches++;
@@ -1355,11 +1603,28 @@ cgbdsm_status cgbdsm_calig_update(cgbdsm_query* q) {
                                                      &S->d_c->n_list[0]);
         }
         const uint64_t n_opt = S->h_c->n_list[2];
-        // 2. Deletion rounds over the optimistic pairs.
-        if (n_opt) {
-            CGD_HIP(hipMemcpyAsync(S->L[0].p, S->L[5].p, n_opt * 4, hipMemcpyDeviceToDevice, q->s));
-            FillU64((uint64_t*)&S->d_c->n_list[0], n_opt, 1, q->s);
-            S1DeletionRounds(q, 0, n_opt, true);
+        const uint64_t nd = g->n_dele;
+        // 1b. Deleted edges: the main-index entries they carried (lit set
+        // before the batch: lit and not activated in it).
+        uint64_t n_kd = nd ? S1EmitDelEdges(q) : 0;
+        S->n_dedge = n_kd;
+        // 2. Deletion rounds over the optimistic pairs and, when edges were
+        // deleted, the pairs from before the batch that lost one (then any
+        // lit pair may die, so every lit neighbour of a death is re-checked).
+        if (n_opt) CGD_HIP(hipMemcpyAsync(S->L[0].p, S->L[5].p, n_opt * 4, hipMemcpyDeviceToDevice, q->s));
+        FillU64((uint64_t*)&S->d_c->n_list[0], n_opt, 1, q->s);
+        uint64_t n_chk = n_opt;
+        if (nd) {
+            KS1SeedDeleted<<<Grid(nd), 256, 0, q->s>>>(S->tab, V, g->dele.p, nd, g->vbits, S->L[0].p,
+                                                       &S->d_c->n_list[0]);
+            S1Set(q, &S1Cnt::n_killed, 0);
+            S1Read(q);
+            n_chk = S->h_c->n_list[0];
+        }
+        if (n_chk) S1DeletionRounds(q, 0, n_chk, nd == 0, nd ? S->L2[1].p : nullptr);
+        if (nd) {
+            const uint64_t nk = S->h_c->n_killed;
+            if (nk) n_kd = S1EmitKilled(q, n_kd, nk);
         }
         // 3. Survivors, new compressed ids, keys.
         FillU64((uint64_t*)&S->d_c->n_list[3], 0, 1, q->s);
@@ -1392,14 +1657,76 @@ cgbdsm_status cgbdsm_calig_update(cgbdsm_query* q) {
             Ensure(q, q->k_del, n_upd + 1024);
         }
         if (ny) KS1ClearOpt<<<Grid(ny), 256, 0, q->s>>>(V, S->L[4].p, &S->d_c->n_list[3]);
+        if (n_kd) {
+            // Del = the update keys (removed until the commit) + the entries
+            // the deletions remove for good.
+            Ensure(q, q->k_del, n_upd + n_kd, true, n_upd);
+            CGD_HIP(hipMemcpyAsync(q->k_del.p + n_upd, S->kd.p, n_kd * 8, hipMemcpyDeviceToDevice, q->s));
+        }
         uint8_t edge_ok[kMaxQE] = {};
         auto dir_on = [&](uint32_t d) { return (S->h_c->dirmask[d >> 6] >> (d & 63)) & 1ull; };
         for (uint32_t e = 0; e < q->qe; e++) edge_ok[e] = dir_on(q->hplan.sa[e]) && dir_on(q->hplan.sb[e]) ? 1 : 0;
         S->dv = dv;
         S->batches++;
-        StageDeviceKeys(q, n_ins, n_upd, n_upd, edge_ok, std::max<uint32_t>(dv, q->dv));
+        StageDeviceKeys(q, n_ins, n_upd + n_kd, n_upd, edge_ok, std::max<uint32_t>(dv, q->dv));
         CGD_HIP(hipStreamSynchronize(q->s));
         S->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+cgbdsm_status cgbdsm_calig_match_deleted(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge) {
+    return Guard([&] {
+        if (!q || !q->s1) Throw(CGBDSM_ERR_INVALID, "query has no device Stage 1 (cgbdsm_calig_attach)");
+        if (q->phase != 1 || q->s1->batches == 0)
+            Throw(CGBDSM_ERR_INVALID, "cgbdsm_calig_match_deleted must follow cgbdsm_calig_update, before Stage 2");
+        S1Query* S = q->s1;
+        if (num_matches) *num_matches = 0;
+        if (per_edge) memset(per_edge, 0, q->qe * sizeof(uint64_t));
+        const uint64_t nneg = S->n_dedge;
+        if (!nneg) return;
+        auto chk = [](cgbdsm_status st) {
+            if (st != CGBDSM_OK) Throw(st, g_err);
+        };
+        // Park the positive batch, stage the deleted edges' entries as both
+        // the Del and the update keys (M loses them for good; they are never
+        // committed back), run Stage 2, then restore the positive batch.
+        const uint64_t n_ins = q->n_ins, n_del = q->n_del, n_upd = q->n_upd;
+        const auto st_save = q->st;
+        uint8_t ok_save[kMaxQE];
+        memcpy(ok_save, q->edge_ok, sizeof(ok_save));
+        std::swap(q->k_ins, S->sv_ins);
+        std::swap(q->k_del, S->sv_del);
+        std::swap(q->k_upd, S->sv_upd);
+        auto restore = [&] {
+            std::swap(q->k_ins, S->sv_ins);
+            std::swap(q->k_del, S->sv_del);
+            std::swap(q->k_upd, S->sv_upd);
+            q->n_ins = n_ins;
+            q->n_del = n_del;
+            q->n_upd = n_upd;
+            memcpy(q->edge_ok, ok_save, sizeof(ok_save));
+            q->st.last_ins = st_save.last_ins;
+            q->st.last_del = st_save.last_del;
+            q->st.last_upd = st_save.last_upd;
+            q->phase = 1;
+        };
+        try {
+            StageDeviceKeys(q, 0, nneg, nneg, S->neg_ok, q->dv);
+            CGD_HIP(hipMemcpyAsync(q->k_del.p, S->kd.p, nneg * 8, hipMemcpyDeviceToDevice, q->s));
+            CGD_HIP(hipMemcpyAsync(q->k_upd.p, S->kd.p, nneg * 8, hipMemcpyDeviceToDevice, q->s));
+            chk(cgbdsm_transfer_update_all(q));
+            int ne = 0;
+            chk(cgbdsm_transfer_update_index(q, &ne));
+            if (ne) {
+                chk(cgbdsm_build_local_index(q));
+                chk(cgbdsm_matching(q, num_matches, per_edge));
+            }
+        } catch (...) {
+            restore();
+            throw;
+        }
+        restore();
+        S->n_dedge = 0;  // counted; a second call finds nothing left to count
     });
 }
 

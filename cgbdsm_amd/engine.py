@@ -166,6 +166,14 @@ class DeviceQuery:
         """Device Stage 1 of the graph's latest batch; stages it."""
         _check(self.L.cgbdsm_calig_update(self.h))
 
+    def calig_match_deleted(self):
+        """Embeddings the staged batch's deletions destroy: (count, per_edge),
+        (0, zeros) without deletions.  Call right after calig_update."""
+        n = C.c_ulonglong(0)
+        per = np.zeros(N.MAX_QE, dtype=np.uint64)
+        _check(self.L.cgbdsm_calig_match_deleted(self.h, C.byref(n), per.ctypes.data))
+        return int(n.value), per[: self.qe].copy()
+
     def calig_lit(self):
         li = np.zeros(self.graph.n, dtype=np.uint32)
         _check(self.L.cgbdsm_calig_export_lit(self.h, li.ctypes.data, self.graph.n))
@@ -264,9 +272,10 @@ class DeviceQuery:
         _check(self.L.cgbdsm_set_dfs_tuning(self.h, int(rows_per_grab), int(blocks_per_cu)))
         self.dfs_tuning = (int(rows_per_grab), int(blocks_per_cu))
 
-    def set_dfs_search(self, wave=True):
-        """Search long backward-neighbour lists with the whole wavefront."""
-        _check(self.L.cgbdsm_set_dfs_search(self.h, 1 if wave else 0))
+    def set_dfs_search(self, wave=True, min_len=256):
+        """Search backward-neighbour lists of min_len entries or more (a
+        power of two >= 64) with the whole wavefront."""
+        _check(self.L.cgbdsm_set_dfs_search(self.h, int(min_len) if wave else 0))
 
     def set_dfs_steal(self, range_min=1024, cache_cap=0):
         """Stealing chunk and stable-cache size of the final DFS (counts are

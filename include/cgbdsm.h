@@ -265,10 +265,11 @@ cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int pro
 cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu);
 
 /* Final-DFS list search (counts are identical either way; no reference
- * counterpart): wave_search != 0 searches backward-neighbour lists of 256
+ * counterpart): wave_search = 1 searches backward-neighbour lists of 256
  * entries or more with the whole wavefront (each lane loads one of 64
  * bucket pivots, each candidate then binary-searches one bucket) instead
- * of one full binary search per lane. */
+ * of one full binary search per lane; a power of two >= 64 sets that
+ * shortest list length instead; 0 turns it off (the default). */
 cgbdsm_status cgbdsm_set_dfs_search(cgbdsm_query* q, int wave_search);
 
 /* Stealing chunk and LDS split of the final DFS (autotuner knobs; counts
@@ -366,8 +367,15 @@ cgbdsm_status cgbdsm_sync(cgbdsm_query* q);
  * the Stage-2 calls follow unchanged.  Lit bits equal the greatest fixpoint
  * of the CaLiG check on the current graph; keys use the same boundary format
  * and compressed ids (ascending original id among newly lit vertices).
- * Insertion streams only: pairs with a negative id are skipped, as the host
- * producer skips them.
+ * Deletion streams: a pair of negative ids (-a-1, -b-1) deletes edge (a, b)
+ * (CaLiG::updateIndex's deletion branch, index/calig.cpp:980-989, 477-535);
+ * lit pairs that lose their support go dark, and the staged Del keys also
+ * remove the main-index entries of the deleted edges and of those pairs.
+ * Positive matches count the batch's insertions on the graph after the
+ * batch; cgbdsm_calig_match_deleted counts the embeddings its deletions
+ * destroy.  (The reference's ConstructUpdate indexes vertices with the raw
+ * ids, so its GPU stage runs insertion streams only.)  The host producer
+ * (cgh_*) skips negative ids.
  * ------------------------------------------------------------------------ */
 typedef struct cgbdsm_graph cgbdsm_graph;
 
@@ -389,7 +397,8 @@ cgbdsm_status cgbdsm_graph_set_stream(cgbdsm_graph* g, const int32_t* ints, uint
  * consumed the previous batch. */
 cgbdsm_status cgbdsm_graph_apply_range(cgbdsm_graph* g, uint64_t start, uint64_t count);
 /* Batches applied, directed adjacency entries, new directed edges of the
- * latest batch. */
+ * latest batch.  Insertions of present edges and deletions of absent ones
+ * are dropped, both judged against the graph before the batch. */
 cgbdsm_status cgbdsm_graph_info(cgbdsm_graph* g, uint64_t* batches, uint64_t* directed_edges, uint64_t* last_new);
 
 /* CaLiG::constructCand + staticFilter (index/calig.cpp:325-356, 705-717)
@@ -405,6 +414,15 @@ cgbdsm_status cgbdsm_calig_attach(cgbdsm_query* q, cgbdsm_graph* g, const int32_
  * PrepareUpdateAll / PrepareUpdateIndex (index/calig_helper_gamma.cu:363-445),
  * ending with the batch staged (as cgbdsm_batch_stage). */
 cgbdsm_status cgbdsm_calig_update(cgbdsm_query* q);
+/* Negative matches of the staged batch (call right after
+ * cgbdsm_calig_update, before the Stage-2 calls): the embeddings of the
+ * graph before the batch that use one of its deleted edges, with the
+ * per-edge split of cgbdsm_matching (the largest query-edge index mapped to
+ * a deleted edge).  Runs Stage 2 with the deleted edges' entries as the
+ * update index and leaves the positive batch staged; 0 for a batch without
+ * deletions.  No reference counterpart (the reference's GPU stage matches
+ * insertions only; CaLiG's turn-off path is index/calig.cpp:410-535). */
+cgbdsm_status cgbdsm_calig_match_deleted(cgbdsm_query* q, unsigned long long* num_matches, uint64_t* per_edge);
 /* Lit bitmask per original vertex (n = the graph's vertex count). */
 cgbdsm_status cgbdsm_calig_export_lit(cgbdsm_query* q, uint32_t* li, uint32_t n);
 /* Compressed id -> original vertex id; *n_out = compressed vertices. */

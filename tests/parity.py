@@ -44,7 +44,7 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
     if dfs_options:
         q.set_dfs_options(**dfs_options)
     if wave_search:
-        q.set_dfs_search(True)
+        q.set_dfs_search(True, 256 if wave_search is True else int(wave_search))
     dcard, davg = q.index_init(keys, dv)
     assert np.array_equal(dcard, card), (dcard, card)
     assert np.array_equal(davg, avg), (davg, avg)
@@ -222,3 +222,88 @@ def run_stream_s1(host, eng, batch_ints, *, layout=None, check_masks=True, check
     q.close()
     g.close()
     return res
+
+
+def deletion_stream(n, adj, stream, be, rng, frac=0.4):
+    """Batches of `be` insertions from the flat `stream` plus deletions (as
+    pairs of negative ids -a-1, -b-1) of about frac * be edges present
+    before the batch (earlier insertions among them), two of them listed
+    twice, and three random pairs (mostly absent edges), shuffled together.
+    `adj` is the initial adjacency (lists per vertex)."""
+    cur = {(min(a, b), max(a, b)) for a in range(n) for b in adj[a]}
+    out = []
+    for s0 in range(0, len(stream) - 1, 2 * be):
+        batch = [(stream[i], stream[i + 1]) for i in range(s0, min(s0 + 2 * be, len(stream) - 1), 2)]
+        present = sorted(cur)
+        k = min(len(present), int(frac * be))
+        dels = [present[i] for i in rng.choice(len(present), size=k, replace=False)] if k else []
+        dels += dels[:2]
+        dels += [(int(rng.integers(n)), int(rng.integers(n))) for _ in range(3)]
+        pairs = batch + [(-a - 1, -b - 1) for a, b in dels]
+        pairs = [pairs[i] for i in rng.permutation(len(pairs))]
+        gone = {(min(-a - 1, -b - 1), max(-a - 1, -b - 1)) for a, b in pairs if a < 0 and b < 0}
+        added = {(min(a, b), max(a, b)) for a, b in pairs if a >= 0 and b >= 0 and a != b}
+        cur = (cur - (gone & cur)) | (added - cur)
+        out.append(pairs)
+    return out
+
+
+def apply_pairs(cur, pairs, n):
+    """Edge set after one batch of `pairs` (both kinds judged against `cur`,
+    the set before it); returns (new set, inserted edges, deleted edges)."""
+    ins, dele = set(), set()
+    for a, b in pairs:
+        if a < 0 and b < 0:
+            x, y = -a - 1, -b - 1
+            e = (min(x, y), max(x, y))
+            if x != y and x < n and y < n and e in cur:
+                dele.add(e)
+        elif a >= 0 and b >= 0 and a != b and a < n and b < n:
+            e = (min(a, b), max(a, b))
+            if e not in cur:
+                ins.add(e)
+    return (cur - dele) | ins, sorted(ins), sorted(dele)
+
+
+def csr_of(n, edges):
+    """(offs u64, adj u32) of an undirected edge set."""
+    lists = [[] for _ in range(n)]
+    for a, b in edges:
+        lists[a].append(b)
+        lists[b].append(a)
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    adj = []
+    for v in range(n):
+        lists[v].sort()
+        adj.extend(lists[v])
+        offs[v + 1] = offs[v] + len(lists[v])
+    return offs, np.array(adj if adj else [0], dtype=np.uint32)
+
+
+def scratch_index(offs, adj, lab, ql, qe, lit, old_ids, plan):
+    """The main index built from scratch over a graph and a lit set: the
+    key (dir(ui, uj) << (64 - ebits)) | (id(v) << 32) | id(w) of every edge
+    v-w and lit pairs (v, ui), (w, uj) with ui-uj a query edge, ids the
+    compressed ids (old_ids[id] = original vertex); sorted."""
+    new_of = {int(v): i for i, v in enumerate(old_ids)}
+    qv = len(ql)
+    qn = [[] for _ in range(qv)]
+    for a, b in np.asarray(qe).reshape(-1, 2):
+        qn[int(a)].append(int(b))
+        qn[int(b)].append(int(a))
+    ushift = 64 - plan.ebits
+    keys = []
+    for v in range(lab.size):
+        lv = int(lit[v])
+        if not lv:
+            continue
+        for w in adj[int(offs[v]):int(offs[v + 1])]:
+            lw = int(lit[int(w)])
+            for ui in range(qv):
+                if not (lv >> ui) & 1:
+                    continue
+                for uj in qn[ui]:
+                    if (lw >> uj) & 1:
+                        e = plan.eidx[ui * 12 + uj]
+                        keys.append((e << ushift) | (new_of[v] << 32) | new_of[int(w)])
+    return np.array(sorted(keys), dtype=np.uint64)

@@ -89,15 +89,17 @@ def test_matching_paths(eng, params, dfs):
     assert res["mismatches"] == 0
 
 
-@pytest.mark.parametrize("kind,labels,seed", [("dense", 2, 21), ("tree", 1, 22), ("sparse", 2, 23)])
-def test_wave_search_parity(eng, kind, labels, seed):
-    """Backward-neighbour lists of 256 entries or more searched by the whole
-    wavefront (hubs of a scale-12, degree-16 graph with one or two labels):
-    index, counts, per-edge split and tuples as the oracle's."""
-    print(f"case wave search {kind}", flush=True)
+@pytest.mark.parametrize("kind,labels,seed,min_len", [("dense", 2, 21, 256), ("tree", 1, 22, 256),
+                                                      ("sparse", 2, 23, 256), ("dense", 2, 24, 64),
+                                                      ("sparse", 3, 25, 128)])
+def test_wave_search_parity(eng, kind, labels, seed, min_len):
+    """Backward-neighbour lists of min_len entries or more searched by the
+    whole wavefront (hubs of a scale-12, degree-16 graph with one to three
+    labels): index, counts, per-edge split and tuples as the oracle's."""
+    print(f"case wave search {kind} {min_len}", flush=True)
     spec = synth_spec(12, 16, labels, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
     host = HostSession(spec=spec, batch_edges=400, threads=4)
-    res = parity.run_stream(host, eng, check_masks=False, wave_search=True, max_batches=4, log=_log)
+    res = parity.run_stream(host, eng, check_masks=False, wave_search=min_len, max_batches=4, log=_log)
     assert res["batches"] > 0 and res["mismatches"] == 0
 
 
