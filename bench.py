@@ -736,7 +736,7 @@ def serve_device_bench(args, log):
                     for sp, a in zip(specs, reg_t)]
             arrivals = arr if loop == "open" else [(0.0, p) for p in sizes]
             srv = DeviceServer(eng, g, mode=mode, deadline_ms=args.serve_deadline_ms, tuning=tuning,
-                               threads=16, max_live=128)
+                               threads=16, max_live=128, log=lambda m, k=f"{loop}_{mode}": log(f"serve {k} {m}"))
             res = srv.run(reqs, arrivals)
             if mode == "served":
                 tuning = res["dfs_tuning"]

@@ -201,7 +201,19 @@ __device__ __forceinline__ uint32_t Pair(const Ctx& X, uint32_t d, uint32_t k) {
 
 __device__ __forceinline__ bool TileBit(const Ctx& X, const Tables& T, uint32_t qv, uint32_t v) {
     if ((X.ones >> qv) & 1u) return true;
-    return (T.masks[((uint64_t)X.ei * kMaxQV + qv) * T.W + (v >> 6)] >> (v & 63)) & 1ull;
+    return (LdG(T.masks + ((uint64_t)X.ei * kMaxQV + qv) * T.W + (v >> 6)) >> (v & 63)) & 1ull;
+}
+
+// Index records read as plain words through global loads (a struct copy
+// through the cast pointer can come out as flat loads once inlined).
+__device__ __forceinline__ URec LdURec(const URec* p) {
+    const uint64_t w = LdG((const uint64_t*)p);
+    return URec{(uint32_t)w, (uint32_t)(w >> 32)};
+}
+__device__ __forceinline__ MRec LdMRec(const MRec* p) {
+    typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+    const U4 w = LdG((const U4*)p);
+    return MRec{(uint64_t)w.x | ((uint64_t)w.y << 32), w.z, w.w};
 }
 
 // List of directed query edge `dir` at data vertex `owner` under the
@@ -211,17 +223,17 @@ __device__ __forceinline__ List Resolve(const Ctx& X, const Tables& T, uint32_t 
     List L;
     const uint64_t r = (uint64_t)owner * T.ndir + dir;
     if (dir == X.sa || dir == X.sb) {
-        const URec u = T.u_rec[r];
+        const URec u = LdURec(T.u_rec + r);
         L.base = T.u_vals + u.off;
         L.bl = u.len;
     } else {
-        const MRec m = T.m_rec[r];
+        const MRec m = LdMRec(T.m_rec + r);
         L.base = T.pool + m.off;
         L.bl = m.len;
     }
     const bool app = dir < 64 ? ((X.app0 >> dir) & 1ull) : ((X.app1 >> (dir - 64)) & 1ull);
     if (app) {
-        const URec u = T.u_rec[r];
+        const URec u = LdURec(T.u_rec + r);
         L.app = T.u_vals + u.off;
         L.al = u.len;
     } else {
@@ -231,7 +243,7 @@ __device__ __forceinline__ List Resolve(const Ctx& X, const Tables& T, uint32_t 
     return L;
 }
 
-__device__ __forceinline__ uint32_t At(const List& L, uint32_t i) { return i < L.bl ? L.base[i] : L.app[i - L.bl]; }
+__device__ __forceinline__ uint32_t At(const List& L, uint32_t i) { return i < L.bl ? LdG(L.base + i) : LdG(L.app + (i - L.bl)); }
 
 // Candidate c for position d given positions 0..d-1 in `matched` (LDS,
 // uniform): support-mask bit, injective, present in the list of every
@@ -291,7 +303,7 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
         uint64_t m0 = 0;
         for (uint32_t k = 0; k < n; k++) {
             const uint64_t word =
-                ((ones0 >> k) & 1ull) ? ~0ull : T.masks[((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w];
+                ((ones0 >> k) & 1ull) ? ~0ull : LdG(T.masks + ((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w);
             m0 |= ((word >> lane) & 1ull) << k;
         }
         if (v >= T.dv) m0 = 0;
@@ -300,12 +312,12 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
         uint32_t lm = 0, lu = 0, ou = 0;
         uint64_t om = 0;
         if (m0 & use_m) {
-            const MRec mr = T.m_rec[r];
+            const MRec mr = LdMRec(T.m_rec + r);
             om = mr.off;
             lm = mr.len;
         }
         if (m0 & use_u) {
-            const URec ur = T.u_rec[r];
+            const URec ur = LdURec(T.u_rec + r);
             ou = ur.off;
             lu = ur.len;
         }
@@ -334,10 +346,10 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
                 const uint32_t lmo = s_lenm[wv][lo];
                 uint64_t cand;
                 if (o < lmo) {
-                    nb = T.pool[s_offm[wv][lo] + o];
+                    nb = LdG(T.pool + s_offm[wv][lo] + o);
                     cand = s_m0[wv][lo] & use_m;
                 } else {
-                    nb = T.u_vals[s_offu[wv][lo] + (o - lmo)];
+                    nb = LdG(T.u_vals + s_offu[wv][lo] + (o - lmo));
                     cand = s_m0[wv][lo] & use_u;
                 }
                 v0 = (uint32_t)(w * kWave + lo);
@@ -400,15 +412,15 @@ __global__ void __launch_bounds__(kThreads) KSeed1(Tables T, EiList EL, uint32_t
         uint64_t m0 = 0;
         for (uint32_t k = 0; k < n; k++) {
             const uint64_t word =
-                ((ones0 >> k) & 1ull) ? ~0ull : T.masks[((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w];
+                ((ones0 >> k) & 1ull) ? ~0ull : LdG(T.masks + ((uint64_t)EL.ei[k] * kMaxQV + q0) * T.W + w);
             m0 |= ((word >> lane) & 1ull) << k;
         }
         if (v >= T.dv) m0 = 0;
         if (!__ballot(m0 != 0)) continue;
         const uint64_t r = (uint64_t)v * T.ndir + idx;
         uint64_t emit = 0;
-        if ((m0 & use_m) && T.m_rec[r].len) emit |= m0 & use_m;
-        if ((m0 & use_u) && T.u_rec[r].len) emit |= m0 & use_u;
+        if ((m0 & use_m) && LdMRec(T.m_rec + r).len) emit |= m0 & use_m;
+        if ((m0 & use_u) && LdURec(T.u_rec + r).len) emit |= m0 & use_u;
         const uint32_t nr = (uint32_t)__popcll(emit);
         const uint32_t incl = WaveInclusiveScan(nr);
         const uint32_t tot = Uniform(__shfl(incl, kWave - 1, kWave));
@@ -586,12 +598,12 @@ __device__ __forceinline__ bool InList(const List& L, uint32_t c, uint32_t& prob
     while (lo < hi) {
         const uint32_t m = (lo + hi) >> 1;
         probes++;
-        if (L.base[m] < c) lo = m + 1;
+        if (LdG(L.base + m) < c) lo = m + 1;
         else hi = m;
     }
     if (lo < L.bl) {
         probes++;
-        if (L.base[lo] == c) return true;
+        if (LdG(L.base + lo) == c) return true;
     }
     if (!L.al) return false;
     lo = 0;
@@ -599,12 +611,12 @@ __device__ __forceinline__ bool InList(const List& L, uint32_t c, uint32_t& prob
     while (lo < hi) {
         const uint32_t m = (lo + hi) >> 1;
         probes++;
-        if (L.app[m] < c) lo = m + 1;
+        if (LdG(L.app + m) < c) lo = m + 1;
         else hi = m;
     }
     if (lo < L.al) {
         probes++;
-        return L.app[lo] == c;
+        return LdG(L.app + lo) == c;
     }
     return false;
 }
@@ -638,7 +650,7 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
     bool found = false;
     if (L.bl >= wmin) {
         const uint32_t lane = LaneId();
-        const uint32_t piv = L.base[(uint32_t)(((uint64_t)(lane + 1) * L.bl) >> 6) - 1];
+        const uint32_t piv = LdG(L.base + ((uint32_t)(((uint64_t)(lane + 1) * L.bl) >> 6) - 1));
         uint32_t pos = 0;
 #pragma unroll
         for (uint32_t step = 32; step >= 1; step >>= 1) {
@@ -654,10 +666,10 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
                 while (lo < hi) {
                     const uint32_t m = (lo + hi) >> 1;
                     probes++;
-                    if (L.base[m] < c) lo = m + 1;
+                    if (LdG(L.base + m) < c) lo = m + 1;
                     else hi = m;
                 }
-                found = lo < L.bl && L.base[lo] == c;
+                found = lo < L.bl && LdG(L.base + lo) == c;
             }
         }
     } else if (act) {
@@ -665,20 +677,20 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
         while (lo < hi) {
             const uint32_t m = (lo + hi) >> 1;
             probes++;
-            if (L.base[m] < c) lo = m + 1;
+            if (LdG(L.base + m) < c) lo = m + 1;
             else hi = m;
         }
-        found = lo < L.bl && L.base[lo] == c;
+        found = lo < L.bl && LdG(L.base + lo) == c;
     }
     if (act && !found && L.al) {
         uint32_t lo = 0, hi = L.al;
         while (lo < hi) {
             const uint32_t m = (lo + hi) >> 1;
             probes++;
-            if (L.app[m] < c) lo = m + 1;
+            if (LdG(L.app + m) < c) lo = m + 1;
             else hi = m;
         }
-        found = lo < L.al && L.app[lo] == c;
+        found = lo < L.al && LdG(L.app + lo) == c;
     }
     return found;
 }
@@ -1126,7 +1138,7 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
                                 if (k != kp && k != kmin) ok = InList(BnList(X, T, j, k, matched), c, probes);
                         }
                     } else {
-                        const uint32_t c = o < B.bl[lo] ? B.base[lo][o] : B.app[lo][o - B.bl[lo]];
+                        const uint32_t c = o < B.bl[lo] ? LdG(B.base[lo] + o) : LdG(B.app[lo] + (o - B.bl[lo]));
                         if (use_sc) {
                             ok = StabContains(sc, c);
                         } else {
@@ -1330,9 +1342,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 }
                 use_job(jlo);
                 const uint64_t lr = r - row_base[jlo];
-                const uint32_t w0 = Uniform(rows[lr * D]);
+                const uint32_t w0 = Uniform(LdG(rows + lr * D));
                 ei = w0 >> tag_shift;
-                const uint32_t mv = lane == 0 ? (w0 & v_mask) : (lane < D ? rows[lr * D + lane] : 0u);
+                const uint32_t mv = lane == 0 ? (w0 & v_mask) : (lane < D ? LdG(rows + lr * D + lane) : 0u);
                 // A row whose tag or vertex ids are out of range (frontier
                 // written out of order with this launch) is not followed into
                 // the index: it is dropped and flagged, and the call fails.

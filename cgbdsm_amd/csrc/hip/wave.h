@@ -40,6 +40,16 @@ __device__ __forceinline__ T WaveBroadcast(T v, int src) {
     return __shfl(v, src, 64);
 }
 
+// Load through a pointer that came from memory (a table in a kernel
+// argument, a job record, LDS), whose address space the compiler cannot
+// infer: as a global load.  A flat load would also count against the LDS /
+// scalar counter, so every wait for a shuffle or an LDS read would wait for
+// the outstanding memory loads too.  Only for pointers into device memory.
+template <class T>
+__device__ __forceinline__ T LdG(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);

@@ -494,6 +494,8 @@ class DeviceServer:
             for name, c in res.items():
                 counts[name][b] = c
             lat.append(1e3 * (t_done - t_arr))
+            self.log(f"batch {b}: {edges} edges, {lat[-1]:.1f} ms after arrival, {len(res)} live counts"
+                     + (f", evicting {sorted(gone)}" if gone else ""))
             for name, why in gone.items():
                 dq, pl = live.pop(name)
                 dq.close()
