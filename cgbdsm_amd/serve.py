@@ -424,13 +424,15 @@ class DeviceServer:
             if not f:
                 out[names[k]] = 0
         if ks:
-            if self.tuning is None:
-                from .tune import autotune
-                best, _, _ = autotune(self.eng, [qs[k] for k in ks], grabs=(2, 4, 8), blocks_per_cu=(0,))
-                self.tuning = best
-                for dq in qs:
-                    dq.set_dfs_tuning(*best)
             try:
+                # The launch shape is tuned on the first batch whose batched
+                # count succeeds (the autotuner counts the batch once per shape).
+                if self.tuning is None:
+                    from .tune import autotune
+                    best, _, _ = autotune(self.eng, [qs[k] for k in ks], grabs=(2, 4, 8), blocks_per_cu=(0,))
+                    self.tuning = best
+                    for dq in qs:
+                        dq.set_dfs_tuning(*best)
                 for k, (c, _) in zip(ks, self.eng.matching_batch([qs[k] for k in ks])):
                     out[names[k]] = c
             except CgbdsmError as e:

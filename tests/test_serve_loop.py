@@ -95,3 +95,82 @@ def test_catching_query_is_stopped_before_its_handle_closes():
     closes = [c for c in eng.calls if c == ("late", "close")]
     assert len(closes) == 1
     assert len(res["counts"]["live"]) == n
+
+
+class _OvfQuery:
+    """Device query stand-in for DeviceServer._serve_batch: `heavy` ones
+    fail their count with CGBDSM_ERR_OVERFLOW (a missed deadline)."""
+
+    def __init__(self, name, count, heavy):
+        self.name, self.count, self.heavy = name, count, heavy
+        self.committed = 0
+
+    def calig_update(self):
+        pass
+
+    def transfer_update_all(self):
+        pass
+
+    def transfer_update_index(self):
+        return True
+
+    def build_local_index(self):
+        pass
+
+    def build_frontier(self):
+        pass
+
+    def set_dfs_tuning(self, *a):
+        self.tuning = a
+
+    def profile(self, on):
+        pass
+
+    def kernel_times(self, reset=False):
+        return {"dfs": {"ms": 1.0}}
+
+    def matching(self):
+        from cgbdsm_amd.engine import CgbdsmError
+        if self.heavy:
+            raise CgbdsmError(5, "final DFS exceeded the deadline")
+        return self.count, [self.count]
+
+    def batch_update_global_index(self):
+        self.committed += 1
+
+    def sync(self):
+        pass
+
+
+class _OvfEngine:
+    def matching_batch(self, qs):
+        from cgbdsm_amd.engine import CgbdsmError
+        if any(q.heavy for q in qs):
+            raise CgbdsmError(5, "final DFS exceeded the deadline")
+        return [(q.count, [q.count]) for q in qs]
+
+    def memory(self):
+        return 0, 0
+
+
+def test_served_batch_evicts_the_query_that_misses_its_deadline():
+    """Served mode: the batched count (and the autotuner's first count) of a
+    batch holding a query that misses the deadline fails; the queries are
+    then counted one by one, the failing one is evicted by name and not
+    committed, the others keep their counts and are committed, and the
+    launch shape is tuned at the next batch without it."""
+    from cgbdsm_amd.serve import DeviceServer
+
+    srv = DeviceServer(_OvfEngine(), None, mode="served", threads=2)
+    live = {"a": (_OvfQuery("a", 3, False), None), "b": (_OvfQuery("b", 0, True), None),
+            "c": (_OvfQuery("c", 7, False), None)}
+    gone = {}
+    out = srv._serve_batch(live, gone)
+    assert out == {"a": 3, "c": 7}
+    assert list(gone) == ["b"]
+    assert live["a"][0].committed == 1 and live["c"][0].committed == 1 and live["b"][0].committed == 0
+    assert srv.tuning is None
+    del live["b"]
+    out = srv._serve_batch(live, {})
+    assert out == {"a": 3, "c": 7} and srv.tuning is not None
+    srv.close()
