@@ -424,6 +424,10 @@ class DeviceServer:
             if not f:
                 out[names[k]] = 0
         if ks:
+            # The batched launch (led by its first query) counts every query's
+            # batch: its deadline is the per-query deadline times the queries.
+            lead = qs[ks[0]]
+            lead.set_deadline(self.deadline_ms * len(ks))
             try:
                 # The launch shape is tuned on the first batch whose batched
                 # count succeeds (the autotuner counts the batch once per shape).
@@ -438,6 +442,7 @@ class DeviceServer:
             except CgbdsmError as e:
                 if e.code != OVERFLOW:
                     raise
+                lead.set_deadline(self.deadline_ms)
                 for k in ks:
                     try:
                         out[names[k]] = qs[k].matching()[0]
@@ -445,6 +450,7 @@ class DeviceServer:
                         if e2.code != OVERFLOW:
                             raise
                         evicted[names[k]] = str(e2)[:200]
+            lead.set_deadline(self.deadline_ms)
             list(self.pool.map(lambda k: qs[k].batch_update_global_index(),
                                [k for k in ks if names[k] not in evicted]))
         for dq in qs:

@@ -123,6 +123,9 @@ class _OvfQuery:
     def set_dfs_tuning(self, *a):
         self.tuning = a
 
+    def set_deadline(self, ms):
+        self.deadline = ms
+
     def profile(self, on):
         pass
 
@@ -161,7 +164,7 @@ def test_served_batch_evicts_the_query_that_misses_its_deadline():
     launch shape is tuned at the next batch without it."""
     from cgbdsm_amd.serve import DeviceServer
 
-    srv = DeviceServer(_OvfEngine(), None, mode="served", threads=2)
+    srv = DeviceServer(_OvfEngine(), None, mode="served", threads=2, deadline_ms=50.0)
     live = {"a": (_OvfQuery("a", 3, False), None), "b": (_OvfQuery("b", 0, True), None),
             "c": (_OvfQuery("c", 7, False), None)}
     gone = {}
@@ -173,4 +176,6 @@ def test_served_batch_evicts_the_query_that_misses_its_deadline():
     del live["b"]
     out = srv._serve_batch(live, {})
     assert out == {"a": 3, "c": 7} and srv.tuning is not None
+    # the batched launch's deadline covered both queries; restored after it
+    assert live["a"][0].deadline == 50.0
     srv.close()
