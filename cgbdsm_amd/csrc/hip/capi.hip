@@ -602,6 +602,8 @@ void IndexInitDevice(cgbdsm_query* q, uint64_t n, uint32_t dv_count, uint32_t* c
     LaunchSegments(q->k_ins.p, Cnt(q, &Counters::n_unique), n, q->vbits, q->flags.p, q->seg_row.p, q->seg_start.p,
                    Cnt(q, &Counters::n_segs), q->scan_scratch.p, q->s);
     LaunchInitCaps(q->seg_start.p, Cnt(q, &Counters::n_segs), n, q->caps.p, q->s);
+    // An empty initial index (no lit pair) still scans one capacity: zero it.
+    if (!n) FillU64(q->caps.p, 0, 1, q->s);
     // exclusive scan of capacities -> offsets; total into pool_top
     // (flags hold the head positions needed by the layout kernel)
     ScanU64(q->caps.p, q->caps.p, std::max<uint64_t>(n, 1), (uint64_t*)Cnt(q, &Counters::pool_top),

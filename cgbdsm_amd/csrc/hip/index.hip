@@ -110,6 +110,19 @@ __device__ __forceinline__ void FoldCounters(unsigned long long bytes, unsigned 
     }
 }
 
+// Number of values below `a` among n < 64 sorted values held one per lane
+// (lane i holds value i, lanes >= n hold ~0u): a search by shuffles, no
+// memory round trip.  Every lane of the wavefront calls it.
+__device__ __forceinline__ uint32_t RankInLanes(uint32_t held, uint32_t a) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t pv = __shfl(held, (int)(pos + step - 1), kWave);
+        if (pv < a) pos += step;
+    }
+    return pos;
+}
+
 // One lane per short touched list: M[row] <- M[row] u B (B sorted, unique).
 __global__ void __launch_bounds__(kThreads) KMergeInsertShort(Tables T, const uint64_t* __restrict__ keys,
                                                               uint32_t vbits, const uint32_t* __restrict__ seg_row,
@@ -277,6 +290,10 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
         const uint32_t newlen = na + nbp;
         const uint32_t* Bv = sc_val + bs;
         const uint32_t* Bp = sc_pos + bs;
+        // Fewer than 64 new values: held one per lane, so a moved entry's
+        // shift (new values below it) is found by shuffles.
+        const bool in_lanes = nbp < (uint32_t)kWave;
+        const uint32_t held = in_lanes && lane < nbp ? Bv[lane] : ~0u;
         uint32_t* C = A;
         if (newlen > rec.cap) {
             const uint32_t ncap = CapFor(newlen);
@@ -290,10 +307,9 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
             C = T.pool + off;
             for (uint32_t k = 0; k < na; k += kWave) {
                 const uint32_t i = k + lane;
-                if (i < na) {
-                    const uint32_t a = A[i];
-                    C[i + LowerBound(Bv, nbp, a)] = a;
-                }
+                const uint32_t a = i < na ? A[i] : 0u;
+                const uint32_t r = in_lanes ? RankInLanes(held, a) : (i < na ? LowerBound(Bv, nbp, a) : 0u);
+                if (i < na) C[i + r] = a;
             }
             bytes += 4ull * na + 4ull * na;
             if (lane == 0) {
@@ -307,11 +323,8 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
                 const uint32_t lo = hi > i0 + kWave ? hi - kWave : i0;
                 const uint32_t i = hi - 1 - lane;
                 const bool valid = lane < hi - lo;
-                uint32_t a = 0, r = 0;
-                if (valid) {
-                    a = A[i];
-                    r = LowerBound(Bv, nbp, a);
-                }
+                const uint32_t a = valid ? A[i] : 0u;
+                const uint32_t r = in_lanes ? RankInLanes(held, a) : (valid ? LowerBound(Bv, nbp, a) : 0u);
                 WaveSync();
                 if (valid) A[i + r] = a;
                 WaveSync();
@@ -360,12 +373,20 @@ __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_
         if (na == 0) continue;
         const uint32_t i0 = LowerBound(A, na, (uint32_t)(D[0] & vmask));
         uint32_t w = i0;
+        // Fewer than 64 removed values: held one per lane (see KMergeInsert).
+        const bool in_lanes = nd < (uint32_t)kWave;
+        const uint32_t held = in_lanes && lane < nd ? (uint32_t)(D[lane] & vmask) : ~0u;
         for (uint32_t k = i0; k < na; k += kWave) {
             const uint32_t i = k + lane;
             const bool valid = i < na;
             const uint32_t a = valid ? A[i] : 0u;
             bool del = false;
-            if (valid) {
+            if (in_lanes) {
+                // every lane shuffles (a lane left out would not supply its value)
+                const uint32_t p = RankInLanes(held, a);
+                const uint32_t at = __shfl(held, (int)(p < (uint32_t)kWave ? p : kWave - 1), kWave);
+                del = valid && p < nd && at == a;
+            } else if (valid) {
                 const uint32_t p = LowerBoundKeys(D, nd, a, vmask);
                 del = p < nd && (uint32_t)(D[p] & vmask) == a;
             }

@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 from cgbdsm_amd.engine import Engine
-from cgbdsm_amd.host import HostSession, synth_spec
+from cgbdsm_amd.host import HostSession, Planner, Workload, synth_spec
 from tests import oracle_lib as O
 from tests.parity import apply_pairs, csr_of, deletion_stream, scratch_index
 
@@ -101,3 +101,90 @@ def test_deletion_stream(eng, case):
     assert total > 0 and negative > 0
     q.close()
     g.close()
+
+
+def test_deletion_stream_batched_launch(eng):
+    """Queries on one mixed stream over the unfiltered graph, counted
+    together by one batched final-DFS launch (cgbdsm_matching_batch) after
+    their negative counts: every count and per-edge split equals the
+    oracle's anchored counts.  One query's label pattern leaves its initial
+    lit set nearly empty on this graph (an index that starts empty)."""
+    print("case deletions batched", flush=True)
+    w = Workload(synth_spec(11, 10, 4, seed=6))
+    n = w.labels.size
+    lists = [[] for _ in range(n)]
+    for a, b in w.edges.tolist():
+        lists[a].append(b)
+        lists[b].append(a)
+    cur = {(min(a, b), max(a, b)) for a, b in w.edges.tolist() if a != b}
+    batches = deletion_stream(n, lists, w.stream.reshape(-1).tolist(), 200, np.random.default_rng(6), 0.6)
+    g = eng.graph(w.labels, np.array(sorted(cur), dtype=np.uint32).reshape(-1, 2))
+    g.set_stream(np.array([x for pairs in batches for ab in pairs for x in ab], dtype=np.int32))
+    qs = []
+    for size, kind, seed in ((5, "tree", 7), (5, "sparse", 8), (4, "dense", 9)):
+        ql, qe = w.query(size, kind, seed)
+        pl = Planner(ql, qe)
+        q = eng.query(pl.query_tables)
+        card, avg = q.calig_attach(g, ql)
+        q.set_plan(pl.make_plan(card, avg))
+        qs.append((q, ql, qe.reshape(-1), pl))
+    start, total = 0, 0
+    for b, pairs in enumerate(batches):
+        o0, a0 = csr_of(n, cur)
+        g.apply_range(start, 2 * len(pairs))
+        start += 2 * len(pairs)
+        cur, ins, dele = apply_pairs(cur, pairs, n)
+        o1, a1 = csr_of(n, cur)
+        live = []
+        for q, ql, qe, _ in qs:
+            q.calig_update()
+            nc, nper = q.calig_match_deleted()
+            onc, onper = O.anchored_count(o0, a0, w.labels, ql, qe, np.array(dele, dtype=np.uint32).reshape(-1, 2))
+            assert nc == onc and np.array_equal(nper, onper), (b, nc, onc)
+            q.transfer_update_all()
+            if q.transfer_update_index():
+                q.build_local_index()
+                live.append((q, ql, qe))
+        got = eng.matching_batch([q for q, _, _ in live]) if live else []
+        for (q, ql, qe), (c, per) in zip(live, got):
+            oc, oper = O.anchored_count(o1, a1, w.labels, ql, qe, np.array(ins, dtype=np.uint32).reshape(-1, 2))
+            assert c == oc and np.array_equal(per, oper), (b, c, oc)
+            total += c
+        for q, _, _, _ in qs:
+            q.batch_update_global_index()
+        print(f"batched batch {b}: +{len(ins)} -{len(dele)} edges, {len(live)} queries counted", flush=True)
+    assert total > 0
+    for q, _, _, pl in qs:
+        q.close()
+        pl.close()
+    g.close()
+    w.close()
+
+
+def test_empty_initial_index(eng):
+    """A query with a label the graph does not have: the static filter
+    leaves nothing lit, the initial index is empty (its one scanned
+    capacity must read as zero, whatever the reused device memory held),
+    and every batch counts 0 in both directions."""
+    print("case empty initial index", flush=True)
+    w = Workload(synth_spec(9, 6, 3, seed=11))
+    ql, qe = w.query(4, "tree", 11)
+    ql = ql.copy()
+    ql[0] = 7  # no vertex has label 7
+    pl = Planner(ql, qe)
+    g = eng.graph(w.labels, w.edges)
+    g.set_stream(w.stream)
+    q = eng.query(pl.query_tables)
+    card, avg = q.calig_attach(g, ql)
+    assert not card.any()
+    q.set_plan(pl.make_plan(card, avg))
+    for b in range(2):
+        g.apply_range(b * 100, 100)
+        q.calig_update()
+        assert q.calig_match_deleted()[0] == 0
+        assert q.run_batch()[0] == 0
+        assert q.export(0).size == 0
+    q.close()
+    pl.close()
+    g.close()
+    w.close()
