@@ -824,9 +824,9 @@ def main():
     ap.add_argument("--bfs-row-limit", type=int, default=65536)
     ap.add_argument("--no-shortest-list", action="store_true", help="final DFS scans the first backward neighbour")
     ap.add_argument("--no-product", action="store_true", help="final DFS enumerates the product suffix")
-    ap.add_argument("--wave-search", type=int, default=0,
-                    help="final DFS searches backward-neighbour lists of at least this many entries (a power "
-                         "of two >= 64) with the whole wavefront; 0: off")
+    ap.add_argument("--wave-search", type=int, default=64,
+                    help="final DFS searches shared lists with the whole wavefront: at most 64 entries held one "
+                         "per lane, this many or more (a power of two >= 64) by pivots; 0: off")
     ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
                     help="main-index layout (both give identical lists; GPMA maintains them faster)")
     ap.add_argument("--serve", action="store_true",

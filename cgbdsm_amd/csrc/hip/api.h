@@ -72,7 +72,10 @@ struct cgbdsm_query {
     // Smallest candidate range a DFS wavefront splits off for idle ones, and
     // the cap on the per-wave stable-intersection cache (0: all LDS left).
     uint32_t dfs_range_min = 1024, dfs_cache_cap = 0;
-    uint32_t dfs_wsearch = 0;  // 0, or log2 of the shortest backward-neighbour list searched by the whole wavefront
+    // Wave-cooperative list search of the final DFS: 0 off, else log2 of the
+    // shortest list searched with pivots (lists of at most 64 entries are
+    // always held one entry per lane when on).  Default 64.
+    uint32_t dfs_wsearch = 6;
     double deadline_ms = 120000;  // safety net; cgbdsm_set_deadline_ms(0) disables
     StealItem* steal_items = nullptr;
     unsigned long long* steal_seq = nullptr;

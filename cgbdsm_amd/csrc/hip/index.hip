@@ -110,19 +110,6 @@ __device__ __forceinline__ void FoldCounters(unsigned long long bytes, unsigned 
     }
 }
 
-// Number of values below `a` among n < 64 sorted values held one per lane
-// (lane i holds value i, lanes >= n hold ~0u): a search by shuffles, no
-// memory round trip.  Every lane of the wavefront calls it.
-__device__ __forceinline__ uint32_t RankInLanes(uint32_t held, uint32_t a) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1) {
-        const uint32_t pv = __shfl(held, (int)(pos + step - 1), kWave);
-        if (pv < a) pos += step;
-    }
-    return pos;
-}
-
 // One lane per short touched list: M[row] <- M[row] u B (B sorted, unique).
 __global__ void __launch_bounds__(kThreads) KMergeInsertShort(Tables T, const uint64_t* __restrict__ keys,
                                                               uint32_t vbits, const uint32_t* __restrict__ seg_row,

@@ -695,9 +695,33 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
     return found;
 }
 
+// InListWave, with a base part of at most 64 entries loaded whole (one
+// coalesced load, one entry per lane) and searched by shuffles: the lanes'
+// candidates need no dependent memory round trip at all.  Every lane calls.
+__device__ __forceinline__ bool InListLanes(const List& L, uint32_t c, bool act, uint32_t& probes, uint32_t wmin) {
+    if (L.bl > (uint32_t)kWave) return InListWave(L, c, act, probes, wmin);
+    const uint32_t lane = LaneId();
+    const uint32_t held = lane < L.bl ? LdG(L.base + lane) : ~0u;
+    const uint32_t p = RankInLanes(held, c);
+    const uint32_t at = __shfl(held, (int)p, kWave);
+    bool found = act && p < L.bl && at == c;
+    if (act) probes++;
+    if (act && !found && L.al) {
+        uint32_t lo = 0, hi = L.al;
+        while (lo < hi) {
+            const uint32_t m = (lo + hi) >> 1;
+            probes++;
+            if (LdG(L.app + m) < c) lo = m + 1;
+            else hi = m;
+        }
+        found = lo < L.al && LdG(L.app + lo) == c;
+    }
+    return found;
+}
+
 // AcceptSel for a step of candidates, every lane calling (lanes with `act`
 // hold a candidate); the backward-neighbour lists are searched with
-// InListWave while any lane still holds an accepted candidate.
+// InListLanes while any lane still holds an accepted candidate.
 __device__ __forceinline__ bool AcceptSelWave(const Ctx& X, const Tables& T, uint32_t d, const uint32_t* matched,
                                               uint32_t c, bool act, uint32_t skip, uint32_t ninj,
                                               uint32_t& probes, uint32_t wmin) {
@@ -708,7 +732,7 @@ __device__ __forceinline__ bool AcceptSelWave(const Ctx& X, const Tables& T, uin
     for (uint32_t k = 0; k < nbn; k++) {
         if (k == skip) continue;
         if (!__ballot(ok)) break;
-        ok = InListWave(BnList(X, T, d, k, matched), c, ok, probes, wmin);
+        ok = InListLanes(BnList(X, T, d, k, matched), c, ok, probes, wmin);
     }
     return ok;
 }
@@ -1016,7 +1040,7 @@ struct BxLds {
 // sibling in LDS.  Equal to SuffixCount per sibling (identical counts).
 __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_t p, const uint32_t* matched,
                                           uint64_t m, uint32_t x, uint32_t* csize, uint32_t& cvalid, StabCache& sc,
-                                          BxLds& B, uint32_t& scanned, uint32_t& probes) {
+                                          BxLds& B, uint32_t& scanned, uint32_t& probes, uint32_t wmin) {
     const uint32_t lane = LaneId();
     const uint32_t qv = X.qv;
     const uint32_t pcls = Cls(X, p);
@@ -1053,7 +1077,18 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
                 cvalid |= 1u << j;
             }
             long long val = (long long)raw - (long long)hits;
-            if (same && prod && val > 0) {
+            // Is the sibling itself in C_j?  C_j's lists do not depend on
+            // the sibling, so with the wave search on, the siblings (sorted,
+            // one per lane) are searched in each list by the whole wavefront.
+            const bool act = same && prod && val > 0;
+            if (wmin && __ballot(act)) {
+                bool in = act && TileBit(X, T, qj, x);
+                for (uint32_t k = 0; k < nbn; k++) {
+                    if (!__ballot(in)) break;
+                    in = InListLanes(BnList(X, T, j, k, matched), x, in, probes, wmin);
+                }
+                val -= in ? 1 : 0;
+            } else if (!wmin && act) {
                 bool in = TileBit(X, T, qj, x);
                 for (uint32_t k = 0; k < nbn && in; k++) in = InList(BnList(X, T, j, k, matched), x, probes);
                 val -= in ? 1 : 0;
@@ -1572,7 +1607,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             if (blev) {
                 if (m) {
                     count = SatAdd(count, BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv],
-                                                      my_scanned, my_probes));
+                                                      my_scanned, my_probes, wsearch ? wmin : 0u));
                     batched++;
                 }
                 if (lane == 0) lvl[d].off = off + kWave;

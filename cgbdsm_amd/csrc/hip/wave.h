@@ -50,6 +50,20 @@ __device__ __forceinline__ T LdG(const T* p) {
     return *(const __attribute__((address_space(1))) T*)p;
 }
 
+// Number of values below `a` among at most 64 sorted values held one per
+// lane (lane i holds value i; lanes past the count hold ~0u), by six
+// shuffles: no memory round trip.  At most 63 (a value above all 64 held
+// ones reads as 63: callers compare the value there).  Every lane calls it.
+__device__ __forceinline__ uint32_t RankInLanes(uint32_t held, uint32_t a) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t pv = __shfl(held, (int)(pos + step - 1), 64);
+        if (pv < a) pos += step;
+    }
+    return pos;
+}
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);

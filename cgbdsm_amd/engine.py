@@ -272,9 +272,10 @@ class DeviceQuery:
         _check(self.L.cgbdsm_set_dfs_tuning(self.h, int(rows_per_grab), int(blocks_per_cu)))
         self.dfs_tuning = (int(rows_per_grab), int(blocks_per_cu))
 
-    def set_dfs_search(self, wave=True, min_len=256):
-        """Search backward-neighbour lists of min_len entries or more (a
-        power of two >= 64) with the whole wavefront."""
+    def set_dfs_search(self, wave=True, min_len=64):
+        """Wave-cooperative list search of the final DFS (on by default,
+        min_len 64): lists of at most 64 entries held one per lane, lists of
+        min_len entries or more (a power of two >= 64) searched by pivots."""
         _check(self.L.cgbdsm_set_dfs_search(self.h, int(min_len) if wave else 0))
 
     def set_dfs_steal(self, range_min=1024, cache_cap=0):

@@ -265,11 +265,14 @@ cgbdsm_status cgbdsm_set_dfs_options(cgbdsm_query* q, int shortest_list, int pro
 cgbdsm_status cgbdsm_set_dfs_tuning(cgbdsm_query* q, uint32_t rows_per_grab, uint32_t blocks_per_cu);
 
 /* Final-DFS list search (counts are identical either way; no reference
- * counterpart): wave_search = 1 searches backward-neighbour lists of 256
- * entries or more with the whole wavefront (each lane loads one of 64
- * bucket pivots, each candidate then binary-searches one bucket) instead
- * of one full binary search per lane; a power of two >= 64 sets that
- * shortest list length instead; 0 turns it off (the default). */
+ * counterpart).  On, every membership test of the lanes' candidates in a
+ * list the whole wavefront shares is done by all lanes at once: a list of
+ * at most 64 entries is loaded whole (one entry per lane) and searched by
+ * shuffles; a list of wave_search entries or more is cut into 64 buckets
+ * (each lane loads one pivot, each candidate binary-searches one bucket);
+ * lists in between are binary-searched per lane.  wave_search = a power of
+ * two >= 64 (the default is 64), 1 = 256, 0 = off (one binary search per
+ * lane and list). */
 cgbdsm_status cgbdsm_set_dfs_search(cgbdsm_query* q, int wave_search);
 
 /* Stealing chunk and LDS split of the final DFS (autotuner knobs; counts

@@ -34,7 +34,7 @@ def _masks_equal(q, idx_masks, plan, dv, edge_ok):
 
 
 def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, match_params=None,
-               dfs_options=None, layout=None, max_batches=None, log=None, wave_search=False):
+               dfs_options=None, layout=None, max_batches=None, log=None, wave_search=None):
     keys, dv, card, avg = host.initial_keys()
     q = eng.query(host.query_tables)
     if layout:
@@ -43,8 +43,8 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
         q.set_match_params(**match_params)
     if dfs_options:
         q.set_dfs_options(**dfs_options)
-    if wave_search:
-        q.set_dfs_search(True, 256 if wave_search is True else int(wave_search))
+    if wave_search is not None:  # None: the library default (on, 64)
+        q.set_dfs_search(bool(wave_search), 256 if wave_search is True else int(wave_search or 64))
     dcard, davg = q.index_init(keys, dv)
     assert np.array_equal(dcard, card), (dcard, card)
     assert np.array_equal(davg, avg), (davg, avg)

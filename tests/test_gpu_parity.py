@@ -91,11 +91,14 @@ def test_matching_paths(eng, params, dfs):
 
 @pytest.mark.parametrize("kind,labels,seed,min_len", [("dense", 2, 21, 256), ("tree", 1, 22, 256),
                                                       ("sparse", 2, 23, 256), ("dense", 2, 24, 64),
-                                                      ("sparse", 3, 25, 128)])
+                                                      ("sparse", 3, 25, 128), ("dense", 2, 24, 0),
+                                                      ("sparse", 3, 25, 0)])
 def test_wave_search_parity(eng, kind, labels, seed, min_len):
-    """Backward-neighbour lists of min_len entries or more searched by the
-    whole wavefront (hubs of a scale-12, degree-16 graph with one to three
-    labels): index, counts, per-edge split and tuples as the oracle's."""
+    """Shared lists searched by the whole wavefront (held one per lane up to
+    64 entries, by pivots from min_len entries; min_len 0: the search turned
+    off, one binary search per lane) on hubs of a scale-12, degree-16 graph
+    with one to three labels: index, counts, per-edge split and tuples as
+    the oracle's."""
     print(f"case wave search {kind} {min_len}", flush=True)
     spec = synth_spec(12, 16, labels, seed=seed, query_size=5, query_kind=kind, query_seed=seed)
     host = HostSession(spec=spec, batch_edges=400, threads=4)
