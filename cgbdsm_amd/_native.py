@@ -42,6 +42,14 @@ class Stats(C.Structure):
                                   "last_dfs_busy")]
 
 
+class SplitStats(C.Structure):
+    """cgbdsm_split_stats (include/cgbdsm.h)."""
+    _fields_ = [("threads", C.c_int), ("ratio", C.c_double), ("batches", C.c_uint64),
+                ("last_cpu_rows", C.c_uint64), ("last_gpu_rows", C.c_uint64), ("last_cpu_ms", C.c_double),
+                ("last_gpu_ms", C.c_double), ("last_cpu_matches", C.c_uint64), ("mirror_entries", C.c_uint64),
+                ("mirror_ms", C.c_double)]
+
+
 class KTime(C.Structure):
     """cgbdsm_ktime (include/cgbdsm.h)."""
     _fields_ = [("ms", C.c_double), ("launches", C.c_uint64), ("bytes", C.c_uint64), ("items", C.c_uint64)]
@@ -187,6 +195,9 @@ def dev_lib():
         L.cgbdsm_calig_old_ids.argtypes = [P, C.c_void_p, U32, C.POINTER(U32)]
         L.cgbdsm_calig_stats.argtypes = [P, C.POINTER(C.c_double), C.c_void_p]
         L.cgbdsm_staged_keys.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
+        L.cgbdsm_set_cpu_split.argtypes = [P, I, C.c_double, C.c_double, U64]
+        L.cgbdsm_cpu_split_stats.argtypes = [P, C.POINTER(SplitStats)]
+        L.cgbdsm_cpu_mirror_export.argtypes = [P, I, C.c_void_p, U64, C.POINTER(U64)]
         _dev = L
     return _dev
 
@@ -204,5 +215,5 @@ DEV_SYMBOLS = [
     "cgbdsm_graph_info", "cgbdsm_calig_attach", "cgbdsm_calig_update", "cgbdsm_calig_match_deleted",
     "cgbdsm_calig_export_lit",
     "cgbdsm_calig_old_ids", "cgbdsm_calig_stats", "cgbdsm_staged_keys", "cgbdsm_set_dfs_steal",
-    "cgbdsm_set_dfs_search",
+    "cgbdsm_set_dfs_search", "cgbdsm_set_cpu_split", "cgbdsm_cpu_split_stats", "cgbdsm_cpu_mirror_export",
 ]

@@ -12,7 +12,8 @@
 #include "kernels.h"
 
 namespace cgd {
-struct S1Query;   // device Stage-1 state of a query (stage1.hip)
+struct S1Query;     // device Stage-1 state of a query (stage1.hip)
+struct SplitState;  // host mirror and host share of the final DFS (split.hip)
 }
 using namespace cgd;
 
@@ -123,6 +124,8 @@ struct cgbdsm_query {
     // Device Stage-1 state when the query's batches come from
     // cgbdsm_calig_update (stage1.hip), else null.
     cgd::S1Query* s1 = nullptr;
+    // CPU/GPU split of the final DFS (cgbdsm_set_cpu_split), else null.
+    cgd::SplitState* split = nullptr;
     bool count_wide = false;  // the last matching's count did not fit 64 bits
     struct Pma {
         DBuf<uint64_t> slot, shadow, lo, S, keys, keys2;
@@ -198,6 +201,20 @@ void IndexInitDevice(cgbdsm_query* q, uint64_t n, uint32_t dv_count, uint32_t* c
 // staged one (capi.hip).
 void StageDeviceKeys(cgbdsm_query* q, uint64_t n_ins, uint64_t n_del, uint64_t n_upd, const uint8_t* edge_ok,
                      uint32_t dv);
+// CPU/GPU split of the final DFS (split.hip).  The On* hooks keep the host
+// mirror in step with what the device applies; SplitPlan gives the rows the
+// host counts (0: none); SplitStart copies them and the masks to the host and
+// starts the host count; SplitFinish waits for it, adds its per-edge counts
+// to q->h_cnt (read after the GPU's share) and moves the ratio.
+void SplitFree(cgbdsm_query* q);
+void SplitInvalidate(cgbdsm_query* q);
+void SplitOnApply(cgbdsm_query* q);
+void SplitOnUpdateIndex(cgbdsm_query* q);
+void SplitOnCommit(cgbdsm_query* q);
+uint64_t SplitPlan(cgbdsm_query* q, uint64_t n);
+void SplitStart(cgbdsm_query* q, const uint32_t* d_rows, uint64_t k, uint64_t n, uint32_t D);
+void SplitJoin(cgbdsm_query* q, bool abort);
+void SplitFinish(cgbdsm_query* q, double gpu_ms);
 int KBegin(cgbdsm_query* q, int cls);
 void KEnd(cgbdsm_query* q, int idx);
 void KAdd(cgbdsm_query* q, int cls, uint64_t bytes, uint64_t items);

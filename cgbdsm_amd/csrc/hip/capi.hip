@@ -589,6 +589,7 @@ void PmaDelete(cgbdsm_query* q, const uint64_t* keys, uint64_t n) {
 }
 
 void IndexInitDevice(cgbdsm_query* q, uint64_t n, uint32_t dv_count, uint32_t* card, float* avg) {
+    SplitInvalidate(q);
     GrowDV(q, std::max<uint32_t>(dv_count, 1));
     q->dv = dv_count;
     q->W = (q->dv + 63) / 64;
@@ -754,6 +755,7 @@ cgbdsm_status cgbdsm_query_destroy(cgbdsm_query* q) {
     return Guard([&] {
         if (!q) return;
         if (q->s) (void)hipStreamSynchronize(q->s);
+        SplitFree(q);
         S1Detach(q);
         Release(q, q->m_rec);
         Release(q, q->u_rec);
@@ -1024,6 +1026,9 @@ cgbdsm_status cgbdsm_batch_stage_next(cgbdsm_query* q, int* staged) {
 cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
     return Guard([&] {
         if (!q || q->phase != 1) Throw(CGBDSM_ERR_INVALID, "transfer_update_all needs a staged batch");
+        // The host mirror takes the batch's keys before the device's sort
+        // reuses their buffers.
+        if (q->split) SplitOnApply(q);
         SetCounter(q, &Counters::bytes, 0);
         SetCounter(q, &Counters::rows_touched, 0);
         q->batch_idx_bytes = 0;
@@ -1074,6 +1079,7 @@ cgbdsm_status cgbdsm_transfer_update_all(cgbdsm_query* q) {
 cgbdsm_status cgbdsm_transfer_update_index(cgbdsm_query* q, int* nonempty) {
     return Guard([&] {
         if (!q || q->phase != 2) Throw(CGBDSM_ERR_INVALID, "transfer_update_index must follow transfer_update_all");
+        if (q->split) SplitOnUpdateIndex(q);
         LaunchClearU(q->u_rec.p, q->u_rows.p, Cnt(q, &Counters::u_rows), q->u_rows_upper, q->s);
         SetCounter(q, &Counters::u_rows, 0);
         q->u_rows_upper = 0;
@@ -1350,8 +1356,24 @@ cgbdsm_status cgbdsm_matching(cgbdsm_query* q, unsigned long long* num_matches, 
         // Step 5: final DFS count from the current level.
         ResetMatchCounters(q, q->s);
         ResetDfsCounters(q);
-        RunDfs(q, q->front[f.cur].p, f.n, f.depth, nullptr, 0);
-        CheckDfs(q, f.n);
+        const uint64_t k = q->split ? SplitPlan(q, f.n) : 0;
+        if (k) {
+            // Rows [0, k) are counted on the host while the GPU counts the rest.
+            SplitStart(q, q->front[f.cur].p, k, f.n, f.depth);
+            try {
+                CGD_HIP(hipEventRecord(q->ev[6], q->s));
+                RunDfs(q, q->front[f.cur].p + k * f.depth, f.n - k, f.depth, nullptr, 0);
+                CGD_HIP(hipEventRecord(q->ev[7], q->s));
+                CheckDfs(q, f.n - k);
+            } catch (...) {
+                SplitJoin(q, true);
+                throw;
+            }
+            SplitFinish(q, Elapsed(q->ev[6], q->ev[7]));
+        } else {
+            RunDfs(q, q->front[f.cur].p, f.n, f.depth, nullptr, 0);
+            CheckDfs(q, f.n);
+        }
         FinishMatching(q, f, q, num_matches, per_edge);
         CheckWide(&q, 1);
     });
@@ -1436,6 +1458,7 @@ cgbdsm_status cgbdsm_batch_update_global_index(cgbdsm_query* q) {
     return Guard([&] {
         if (!q || (q->phase != 5 && q->phase != 4 && q->phase != 3))
             Throw(CGBDSM_ERR_INVALID, "batch_update_global_index must follow transfer_update_index");
+        if (q->split) SplitOnCommit(q);
         if (q->n_upd) {
             LaunchUnique(q->k_usort.p, q->k_alt.p, q->n_upd, q->flags.p, q->scan_scratch.p, Cnt(q, &Counters::n_unique),
                          q->s);

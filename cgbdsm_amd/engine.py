@@ -314,6 +314,25 @@ class DeviceQuery:
     def set_deadline(self, ms):
         _check(self.L.cgbdsm_set_deadline_ms(self.h, float(ms)))
 
+    def set_cpu_split(self, threads=-1, ratio=0.0, alpha=0.3, min_rows=0):
+        """CPU/GPU split of the final DFS (cgbdsm_set_cpu_split): the host
+        counts a share of the frontier rows while the GPU counts the rest.
+        threads 0 turns it off, -1 uses hardware threads / NUMA nodes."""
+        _check(self.L.cgbdsm_set_cpu_split(self.h, int(threads), float(ratio), float(alpha), int(min_rows)))
+
+    def split_stats(self):
+        s = N.SplitStats()
+        _check(self.L.cgbdsm_cpu_split_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in N.SplitStats._fields_}
+
+    def mirror_export(self, which):
+        """The host mirror's M (0) or U (1) in the boundary key format."""
+        n = C.c_uint64()
+        _check(self.L.cgbdsm_cpu_mirror_export(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(self.L.cgbdsm_cpu_mirror_export(self.h, which, out.ctypes.data, n.value, C.byref(n)))
+        return out[: n.value]
+
     def profile(self, enable=True):
         _check(self.L.cgbdsm_profile(self.h, 1 if enable else 0))
 

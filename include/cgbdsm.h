@@ -293,6 +293,47 @@ cgbdsm_status cgbdsm_set_dfs_steal(cgbdsm_query* q, uint32_t range_min, uint32_t
  * counterpart (the reference waits unboundedly, graph/graph_gpu.cu:1457-1464). */
 cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms);
 
+/* CPU/GPU split of the final DFS of cgbdsm_matching: the host counts frontier
+ * rows [0, k) on `threads` threads while the GPU counts rows [k, n), and the
+ * two per-edge counts are added.  Replaces the reference's
+ * Step5CPUGPUSplit (graph/graph_gpu.cu:1185-1258) with its host count
+ * cpuExtendBFSAllBitCountParallel (index/cpu_dfs.cpp:173-217) and host index
+ * mirror (index/cpu_index_mirror.h:32-48, kept in sync at
+ * index/calig_helper_gamma.cu:279-302, 503-516, 558-566, 588-595).
+ *   threads   0 = off (the default; frees the mirror), < 0 = hardware
+ *             threads / NUMA nodes (index/calig.cpp:1092-1095);
+ *   ratio     the host's initial share of the rows, in [0, 0.95]; 0 = one
+ *             row per host thread on the first split batch;
+ *   alpha     weight of the newest measured balance in the moving average
+ *             of the share (graph/graph_gpu.cu:1250-1253); 0 keeps the
+ *             share fixed;
+ *   min_rows  frontiers of fewer rows stay on the GPU (0 = 8192, the
+ *             reference's default, graph/graph_gpu.h:128).
+ * The share has no lower clamp (the reference's is 0.05): a share under one
+ * row leaves the batch to the GPU.  The host mirror of M is built from the
+ * device's index at the next batch and follows every batch after it; counts
+ * are identical to an unsplit count.  Single-query counts only
+ * (cgbdsm_matching, cgbdsm_run_batch); batched launches and tuple output
+ * stay on the GPU. */
+cgbdsm_status cgbdsm_set_cpu_split(cgbdsm_query* q, int threads, double ratio, double alpha, uint64_t min_rows);
+typedef struct {
+    int threads;                /* host threads (0: split off)                      */
+    double ratio;               /* share the next split batch starts from           */
+    uint64_t batches;           /* matchings split so far                           */
+    uint64_t last_cpu_rows;     /* rows counted on the host in the last split batch */
+    uint64_t last_gpu_rows;
+    double last_cpu_ms;         /* host count, wall time                            */
+    double last_gpu_ms;         /* GPU share, HIP events on the query's stream      */
+    uint64_t last_cpu_matches;  /* matches counted on the host                      */
+    uint64_t mirror_entries;    /* entries of M in the host mirror                  */
+    double mirror_ms;           /* host time keeping the mirror in step, last batch */
+} cgbdsm_split_stats;
+cgbdsm_status cgbdsm_cpu_split_stats(cgbdsm_query* q, cgbdsm_split_stats* out);
+/* The host mirror's M (which = 0) or the batch's U (which = 1) as sorted keys
+ * in the boundary format, like cgbdsm_index_export (no reference
+ * counterpart: the mirror is internal there). */
+cgbdsm_status cgbdsm_cpu_mirror_export(cgbdsm_query* q, int which, uint64_t* out, uint64_t cap, uint64_t* n_out);
+
 /* Kernel classes timed with HIP events on the query's stream when profiling
  * is enabled (bytes = algorithmic bytes as defined per class). */
 enum {

@@ -34,7 +34,11 @@ def _masks_equal(q, idx_masks, plan, dv, edge_ok):
 
 
 def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, match_params=None,
-               dfs_options=None, layout=None, max_batches=None, log=None, wave_search=None):
+               dfs_options=None, layout=None, max_batches=None, log=None, wave_search=None, split=None):
+    """Host-Stage-1 stream through the device, checked against the oracle
+    batch by batch.  split: keyword arguments of DeviceQuery.set_cpu_split;
+    the host mirror's M and U are then also compared with the oracle's
+    after every apply and commit."""
     keys, dv, card, avg = host.initial_keys()
     q = eng.query(host.query_tables)
     if layout:
@@ -55,7 +59,16 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
     idx.init(keys)
     if check_index:
         assert np.array_equal(q.export(0), idx.export(0)), "initial M differs"
-    res = {"batches": 0, "total": 0, "mismatches": 0, "counts": [], "per_edge": [], "skipped": 0, "dfs_batched": 0}
+    res = {"batches": 0, "total": 0, "mismatches": 0, "counts": [], "per_edge": [], "skipped": 0, "dfs_batched": 0,
+           "split": []}
+    if split is not None:
+        q.set_cpu_split(**split)
+
+    def mirror_check(which, what, kb):
+        if split is not None and not np.array_equal(q.mirror_export(which), idx.export(which)):
+            res["mismatches"] += 1
+            if log:
+                log(f"batch {kb.index}: host mirror {what} differs")
     nb = host.num_batches if max_batches is None else min(max_batches, host.num_batches)
     for _ in range(nb):
         kb = host.next_batch()
@@ -69,8 +82,10 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
                 res["mismatches"] += 1
                 if log:
                     log(f"batch {kb.index}: M after apply differs")
+        mirror_check(0, "M after apply", kb)
         nonempty = q.transfer_update_index()
         idx.rebuild_update(kb.upd)
+        mirror_check(1, "U", kb)
         res["batches"] += 1
         if not nonempty:
             assert kb.upd.size == 0
@@ -89,8 +104,13 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
                 res["mismatches"] += 1
                 if log:
                     log(f"batch {kb.index}: masks differ {bad[:5]}")
+        sb = q.split_stats()["batches"] if split is not None else 0
         cnt, per = q.matching()
         res["dfs_batched"] += q.stats()["last_dfs_batched"]
+        if split is not None:
+            st = q.split_stats()
+            if st["batches"] > sb:
+                res["split"].append(st)
         if tuples_cap:
             ocnt, oper, otup, otei = idx.match(plan, kb.edge_ok, masks=om, tuples_cap=tuples_cap)
             n, tup, tei = q.tuples(tuples_cap)
@@ -110,6 +130,7 @@ def run_stream(host, eng, *, check_index=True, check_masks=True, tuples_cap=0, m
                 log(f"batch {kb.index}: count {cnt} vs oracle {ocnt}; per-edge {list(per)} vs {list(oper)}")
         q.batch_update_global_index()
         idx.insert(kb.upd)
+        mirror_check(0, "M after commit", kb)
         if check_index and not np.array_equal(q.export(0), idx.export(0)):
             res["mismatches"] += 1
             if log:

@@ -34,7 +34,7 @@ def test_host_exports_every_declared_symbol():
 def test_struct_layouts_match_the_header(tmp_path):
     """ctypes mirrors agree with the C compiler on every field offset."""
     structs = [("cgbdsm_plan", N.Plan), ("cgbdsm_stats", N.Stats), ("cgh_synth_spec", N.SynthSpec),
-               ("cgh_batch", N.Batch)]
+               ("cgh_batch", N.Batch), ("cgbdsm_split_stats", N.SplitStats)]
     rename = {"dele": "del"}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cgbdsm_host.h"', "int main(void){"]
     for cname, py in structs:
