@@ -827,6 +827,11 @@ def main():
     ap.add_argument("--wave-search", type=int, default=64,
                     help="final DFS searches shared lists with the whole wavefront: at most 64 entries held one "
                          "per lane, this many or more (a power of two >= 64) by pivots; 0: off")
+    ap.add_argument("--cpu-split", type=int, default=0,
+                    help="--single: host threads counting a share of each final-DFS frontier beside the GPU "
+                         "(0: off; -1: hardware threads / NUMA nodes)")
+    ap.add_argument("--split-ratio", type=float, default=0.0,
+                    help="--cpu-split: initial host share (0: one row per host thread, then the measured balance)")
     ap.add_argument("--layout", default="gpma", choices=["gamma", "gpma"],
                     help="main-index layout (both give identical lists; GPMA maintains them faster)")
     ap.add_argument("--serve", action="store_true",
@@ -918,6 +923,9 @@ def main():
     q.set_match_params(bfs_row_limit=args.bfs_row_limit, donate_min=args.donate_min)
     q.set_dfs_options(shortest_list=not args.no_shortest_list, product_suffix=not args.no_product)
     q.set_dfs_search(args.wave_search > 0, args.wave_search or 256)
+    if args.cpu_split:
+        q.set_cpu_split(threads=args.cpu_split, ratio=args.split_ratio, alpha=0.3, min_rows=0)
+    split_log = []
     enq_ms = []
     for kb in batches[: warm + steps]:
         t_e = time.perf_counter()
@@ -937,6 +945,7 @@ def main():
         counts.append(c)
     q.profile(True)
     q.kernel_times(reset=True)
+    split_before = q.split_stats()["batches"] if args.cpu_split else 0
     q.sync()
     t0 = time.perf_counter()
     stage_ms = {"apply": 0.0, "update_index": 0.0, "masks": 0.0, "match": 0.0, "commit": 0.0}
@@ -959,6 +968,14 @@ def main():
             f"donated {st['last_dfs_donated']}, busy {st['last_dfs_busy']:.3f}, steps {st['last_dfs_steps']}, "
             f"lanes/step {st['last_dfs_step_lanes'] / max(1, st['last_dfs_steps']):.1f}, "
             f"LDS cache {st['last_dfs_lds_cache']} u32/wave")
+        sp = q.split_stats() if args.cpu_split else None
+        if sp and sp["batches"] > (split_log[-1]["batches"] if split_log else split_before):
+            split_log.append({"batch": warm + i, **{k: (round(v, 6) if isinstance(v, float) else v)
+                                                     for k, v in sp.items()}})
+            log(f"  split: share {sp['ratio']:.6f}, host {sp['last_cpu_rows']} rows / {sp['last_cpu_ms']:.1f} ms "
+                f"({sp['last_cpu_matches']} matches) after {sp['last_wait_ms']:.1f} ms waiting for the mirror, "
+                f"GPU {sp['last_gpu_rows']} rows / {sp['last_gpu_ms']:.1f} ms, "
+                f"mirror {sp['mirror_entries']} entries, {sp['mirror_ms']:.1f} ms host upkeep")
     q.sync()
     elapsed = time.perf_counter() - t0
     kt = q.kernel_times()
@@ -1015,6 +1032,8 @@ def main():
         "kernels_gbs": {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 3) for k, v in kt.items() if v["ms"] > 0 and v["bytes"]},
         "dfs_totals": dfs,
         "roofline": roofline, "cpu_baseline": cpu, "multi_query": mq, "dfs_tuning": tuning,
+        "cpu_split": {"threads": args.cpu_split, "final_share": round(q.split_stats()["ratio"], 9),
+                      "split_batches": split_log} if args.cpu_split else None,
     }
     print(json.dumps(line), flush=True)
 

@@ -46,7 +46,8 @@ class SplitStats(C.Structure):
     """cgbdsm_split_stats (include/cgbdsm.h)."""
     _fields_ = [("threads", C.c_int), ("ratio", C.c_double), ("batches", C.c_uint64),
                 ("last_cpu_rows", C.c_uint64), ("last_gpu_rows", C.c_uint64), ("last_cpu_ms", C.c_double),
-                ("last_gpu_ms", C.c_double), ("last_cpu_matches", C.c_uint64), ("mirror_entries", C.c_uint64),
+                ("last_gpu_ms", C.c_double), ("last_wait_ms", C.c_double), ("last_cpu_matches", C.c_uint64),
+                ("mirror_entries", C.c_uint64),
                 ("mirror_ms", C.c_double)]
 
 

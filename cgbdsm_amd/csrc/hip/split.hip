@@ -22,15 +22,28 @@
 //    read with the edge tag's visibility: U for the tag's own two
 //    directions, M plus appended U for directions of earlier edges), so its
 //    per-edge counts add to the GPU's.
-//  * The ratio has no floor: the reference clamps it to [0.05, 0.95], which
+//  * The mirror's upkeep runs on a background thread of its own, in the
+//    order the device applied the key sets, overlapping the device's index
+//    maintenance, masks and its share of the DFS; the host count waits for
+//    it before reading the mirror.
+//  * The share has no floor: the reference clamps it to [0.05, 0.95], which
 //    on a host that counts hundreds of times slower than the GPU would hand
 //    it 5 % of the rows, and the batch 30 times the GPU's time, every batch.
-//    Here it only stays below 0.95; a share under one row leaves the batch
-//    to the GPU.
+//    Here it stays below 0.95, and the balance it moves toward also charges
+//    the host's wait for the mirror: with rates c (host) and g (GPU) in rows
+//    per ms and a wait of w ms, the host's k rows end with the GPU's n - k
+//    when w + k / c = (n - k) / g.  A share under one row leaves the batch
+//    to the GPU; after 16 such batches one row per host thread probes the
+//    rates again.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <exception>
+#include <functional>
+#include <memory>
 #include <dirent.h>
 #include <mutex>
 #include <thread>
@@ -45,36 +58,45 @@ struct SplitState {
     double ratio = 0, ratio0 = 0, alpha = 0.3;
     uint64_t min_rows = 8192;
     bool measured = false;
-    // Host mirror of M: row = owner * ndir + dir -> sorted targets.
+    uint32_t idle = 0;  // batches left to the GPU since the last split
+    // Host mirror of M: row = owner * ndir + dir -> sorted targets.  The
+    // flags say what the queued upkeep will have built; reading the mirror
+    // waits for the queue (Drain / WaitIdle).
     bool m_valid = false;
     std::vector<std::vector<uint32_t>> m;
-    uint64_t m_entries = 0;
-    // U of the current batch: sorted (row, target) pairs with duplicates,
-    // and each touched row's range in them.
+    std::atomic<uint64_t> m_entries{0};
+    // U of the current batch: sorted targets with duplicates, and each
+    // touched row's range in them.
     bool u_valid = false;
     std::vector<uint32_t> u_vals;
-    std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>> u_rows;
+    std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> u_rows;
+    std::vector<uint64_t> upd_keys;  // the batch's Upd keys, sorted (committed into M)
+    // Background upkeep: one thread running queued tasks in order.
+    std::thread keeper;
+    std::mutex kmu;
+    std::condition_variable kcv;
+    std::deque<std::function<void()>> tasks;
+    bool kbusy = false, kquit = false;
+    std::exception_ptr kerr;  // first failed task, reported by the next Drain
+    std::atomic<double> mirror_ms{0};  // upkeep time of the batch's tasks so far
     // Host copies for the count.
     std::vector<uint64_t> masks;
-    uint32_t* h_rows = nullptr;  // pinned
-    uint64_t h_rows_cap = 0;     // u32 words
-    std::vector<uint64_t> keys;      // key staging
-    std::vector<uint64_t> upd_keys;  // the batch's Upd keys, sorted (committed into M)
+    std::vector<uint32_t> rows;
     // Running host count.
     std::vector<std::thread> workers;
     std::atomic<uint64_t> next{0};
-    std::atomic<uint32_t> stop{0};  // 1: abort asked, 2: deadline passed
+    std::atomic<uint32_t> stop{0};  // 1: abort asked, 2: deadline passed, 3: mirror upkeep failed
     std::atomic<uint32_t> bad{0};   // a row held an out-of-range tag or vertex id
-    std::mutex mu;                  // guards per_edge and wide
+    std::mutex mu;                  // guards per_edge, wide and the times below
     uint64_t per_edge[kMaxQE] = {};
     bool wide = false;
-    std::chrono::steady_clock::time_point t0;
-    double cpu_ms = 0;
+    std::chrono::steady_clock::time_point t0, t_count, t_end;
+    bool counting = false;
     uint64_t k = 0, n = 0;
     uint32_t D = 0;
     // Statistics.
     uint64_t batches = 0, last_cpu_rows = 0, last_gpu_rows = 0, last_cpu_matches = 0;
-    double last_cpu_ms = 0, last_gpu_ms = 0, mirror_ms = 0;
+    double last_cpu_ms = 0, last_gpu_ms = 0, last_wait_ms = 0;
 };
 
 }  // namespace cgd
@@ -82,6 +104,13 @@ struct SplitState {
 namespace cgapi {
 
 namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double MsSince(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+double MsBetween(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
 
 // Threads of the reference's host rule: hardware threads / NUMA nodes
 // (index/calig.cpp:1092-1095).
@@ -112,6 +141,55 @@ void ParallelFor(uint64_t n, int threads, F&& f) {
     for (auto& x : th) x.join();
 }
 
+// ---------------------------------------------------------------- upkeep queue
+void KeeperLoop(SplitState* S) {
+    std::unique_lock<std::mutex> lk(S->kmu);
+    for (;;) {
+        S->kcv.wait(lk, [&] { return S->kquit || !S->tasks.empty(); });
+        if (S->tasks.empty()) return;  // quit once the queue is empty
+        std::function<void()> t = std::move(S->tasks.front());
+        S->tasks.pop_front();
+        S->kbusy = true;
+        lk.unlock();
+        try {
+            t();
+        } catch (...) {
+            lk.lock();
+            if (!S->kerr) S->kerr = std::current_exception();
+            lk.unlock();
+        }
+        lk.lock();
+        S->kbusy = false;
+        S->kcv.notify_all();
+    }
+}
+
+void Post(SplitState& S, std::function<void()> t) {
+    std::lock_guard<std::mutex> g(S.kmu);
+    if (!S.keeper.joinable()) S.keeper = std::thread(KeeperLoop, &S);
+    S.tasks.push_back(std::move(t));
+    S.kcv.notify_all();
+}
+
+// Waits until every queued task ran; false if one of them failed.
+bool WaitIdle(SplitState& S) {
+    std::unique_lock<std::mutex> lk(S.kmu);
+    S.kcv.wait(lk, [&] { return S.tasks.empty() && !S.kbusy; });
+    return !S.kerr;
+}
+
+// WaitIdle, rethrowing a failed task's error (the mirror is then dropped).
+void Drain(SplitState& S) {
+    if (WaitIdle(S)) return;
+    std::exception_ptr e;
+    {
+        std::lock_guard<std::mutex> g(S.kmu);
+        std::swap(e, S.kerr);
+    }
+    S.m_valid = S.u_valid = false;
+    std::rethrow_exception(e);
+}
+
 void CopyKeys(cgbdsm_query* q, const DBuf<uint64_t>& b, uint64_t n, std::vector<uint64_t>& out) {
     out.resize(n);
     if (n) {
@@ -134,9 +212,8 @@ std::vector<uint64_t> Groups(const std::vector<uint64_t>& keys) {
     return starts;
 }
 
-// M <- M | keys (remove = false) or M \ keys (remove = true), keys sorted.
-void MirrorMerge(cgbdsm_query* q, std::vector<uint64_t>& keys, bool remove) {
-    SplitState& S = *q->split;
+// M <- M | keys (remove = false) or M \ keys (remove = true).
+void MirrorMerge(const cgbdsm_query* q, SplitState& S, std::vector<uint64_t>& keys, bool remove) {
     if (keys.empty()) return;
     std::sort(keys.begin(), keys.end());
     const std::vector<uint64_t> starts = Groups(keys);
@@ -157,24 +234,24 @@ void MirrorMerge(cgbdsm_query* q, std::vector<uint64_t>& keys, bool remove) {
         delta[g] = (int64_t)out.size() - (int64_t)L.size();
         L.swap(out);
     });
-    for (int64_t d : delta) S.m_entries += d;
+    int64_t sum = 0;
+    for (int64_t d : delta) sum += d;
+    S.m_entries.fetch_add((uint64_t)sum);
 }
 
-// Mirror of M built from the device's export (the index after the last commit).
+// Mirror of M built from the device's export (the index after the last
+// commit), on the calling thread once the queue is idle.
 void MirrorBuild(cgbdsm_query* q) {
     SplitState& S = *q->split;
+    Drain(S);
     uint64_t n = 0;
     if (cgbdsm_index_export(q, 0, nullptr, 0, &n) != CGBDSM_OK) Throw(CGBDSM_ERR_DEVICE, g_err);
-    S.keys.assign(n, 0);
-    if (n && cgbdsm_index_export(q, 0, S.keys.data(), n, &n) != CGBDSM_OK) Throw(CGBDSM_ERR_DEVICE, g_err);
+    std::vector<uint64_t> keys(n);
+    if (n && cgbdsm_index_export(q, 0, keys.data(), n, &n) != CGBDSM_OK) Throw(CGBDSM_ERR_DEVICE, g_err);
     S.m.clear();
-    S.m_entries = 0;
-    MirrorMerge(q, S.keys, false);
+    S.m_entries.store(0);
+    MirrorMerge(q, S, keys, false);
     S.m_valid = true;
-}
-
-double MsSince(std::chrono::steady_clock::time_point t) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
 
 // ---------------------------------------------------------------- host count
@@ -279,6 +356,18 @@ struct HostDfs {
 
 void HostWorker(cgbdsm_query* q) {
     SplitState& S = *q->split;
+    // The mirror must hold this batch's M and U before any row is read.
+    if (!WaitIdle(S)) {
+        uint32_t z = 0;
+        S.stop.compare_exchange_strong(z, 3);
+    }
+    {
+        std::lock_guard<std::mutex> g(S.mu);
+        if (!S.counting) {
+            S.counting = true;
+            S.t_count = Clock::now();
+        }
+    }
     HostDfs H{q, &S};
     uint64_t local[kMaxQE] = {};
     const uint32_t D = S.D, ts = q->hplan.tag_shift, vm = q->hplan.v_mask;
@@ -286,7 +375,7 @@ void HostWorker(cgbdsm_query* q) {
         if (H.Stopped(&S)) break;
         const uint64_t r = S.next.fetch_add(1, std::memory_order_relaxed);
         if (r >= S.k) break;
-        const uint32_t* row = S.h_rows + r * D;
+        const uint32_t* row = S.rows.data() + r * D;
         const uint32_t ei = row[0] >> ts;
         bool good = ei < q->qe;
         H.matched[0] = row[0] & vm;
@@ -306,14 +395,32 @@ void HostWorker(cgbdsm_query* q) {
     std::lock_guard<std::mutex> g(S.mu);
     for (uint32_t e = 0; e < q->qe; e++) S.per_edge[e] = HostDfs::Add(S.per_edge[e], local[e], S.wide);
     S.wide |= H.wide;
+    const auto now = Clock::now();
+    if (now > S.t_end) S.t_end = now;
 }
 
 }  // namespace
 
+void SplitJoin(cgbdsm_query* q, bool abort) {
+    SplitState& S = *q->split;
+    if (abort) {
+        uint32_t z = 0;
+        S.stop.compare_exchange_strong(z, 1);
+    }
+    for (auto& w : S.workers) w.join();
+    S.workers.clear();
+}
+
 void SplitFree(cgbdsm_query* q) {
     if (!q->split) return;
+    SplitState& S = *q->split;
     SplitJoin(q, true);
-    if (q->split->h_rows) (void)hipHostFree(q->split->h_rows);
+    {
+        std::lock_guard<std::mutex> g(S.kmu);
+        S.kquit = true;
+        S.kcv.notify_all();
+    }
+    if (S.keeper.joinable()) S.keeper.join();
     delete q->split;
     q->split = nullptr;
 }
@@ -326,61 +433,71 @@ void SplitInvalidate(cgbdsm_query* q) {
 
 void SplitOnApply(cgbdsm_query* q) {
     SplitState& S = *q->split;
-    const auto t0 = std::chrono::steady_clock::now();
     S.u_valid = false;
     if (!S.m_valid) MirrorBuild(q);
-    CopyKeys(q, q->k_ins, q->n_ins, S.keys);
-    MirrorMerge(q, S.keys, false);
-    CopyKeys(q, q->k_del, q->n_del, S.keys);
-    MirrorMerge(q, S.keys, true);
-    S.mirror_ms = MsSince(t0);
+    auto ins = std::make_shared<std::vector<uint64_t>>();
+    auto del = std::make_shared<std::vector<uint64_t>>();
+    CopyKeys(q, q->k_ins, q->n_ins, *ins);
+    CopyKeys(q, q->k_del, q->n_del, *del);
+    Post(S, [q, ins, del] {
+        SplitState& St = *q->split;
+        const auto t0 = Clock::now();
+        MirrorMerge(q, St, *ins, false);
+        MirrorMerge(q, St, *del, true);
+        St.mirror_ms.store(MsSince(t0));
+    });
 }
 
 void SplitOnUpdateIndex(cgbdsm_query* q) {
     SplitState& S = *q->split;
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<uint64_t>& K = S.upd_keys;
-    CopyKeys(q, q->k_upd, q->n_upd, K);
-    std::sort(K.begin(), K.end());
-    S.u_vals.resize(K.size());
-    S.u_rows.clear();
-    const std::vector<uint64_t> starts = Groups(K);
-    for (uint64_t g = 0; g + 1 < starts.size(); g++)
-        S.u_rows[RowOf(q, K[starts[g]])] = {(uint32_t)starts[g], (uint32_t)(starts[g + 1] - starts[g])};
-    for (uint64_t i = 0; i < K.size(); i++) S.u_vals[i] = (uint32_t)K[i];
+    auto upd = std::make_shared<std::vector<uint64_t>>();
+    CopyKeys(q, q->k_upd, q->n_upd, *upd);
+    Post(S, [q, upd] {
+        SplitState& St = *q->split;
+        const auto t0 = Clock::now();
+        std::vector<uint64_t>& K = *upd;
+        std::sort(K.begin(), K.end());
+        St.u_vals.resize(K.size());
+        St.u_rows.clear();
+        const std::vector<uint64_t> starts = Groups(K);
+        for (uint64_t g = 0; g + 1 < starts.size(); g++)
+            St.u_rows[RowOf(q, K[starts[g]])] = {starts[g], (uint32_t)(starts[g + 1] - starts[g])};
+        for (uint64_t i = 0; i < K.size(); i++) St.u_vals[i] = (uint32_t)K[i];
+        St.upd_keys.swap(K);
+        St.mirror_ms.store(St.mirror_ms.load() + MsSince(t0));
+    });
     S.u_valid = S.m_valid;
-    S.mirror_ms += MsSince(t0);
 }
 
 void SplitOnCommit(cgbdsm_query* q) {
     SplitState& S = *q->split;
-    const auto t0 = std::chrono::steady_clock::now();
-    // The Upd keys saved by SplitOnUpdateIndex (commit follows it).
-    if (S.m_valid && S.u_valid) MirrorMerge(q, S.upd_keys, false);
-    else S.m_valid = false;
+    // M takes the Upd keys the update-index task saved (it ran before).
+    if (S.m_valid && S.u_valid) {
+        Post(S, [q] {
+            SplitState& St = *q->split;
+            const auto t0 = Clock::now();
+            MirrorMerge(q, St, St.upd_keys, false);
+            St.mirror_ms.store(St.mirror_ms.load() + MsSince(t0));
+        });
+    } else {
+        S.m_valid = false;
+    }
     S.u_valid = false;
-    S.mirror_ms += MsSince(t0);
 }
 
 uint64_t SplitPlan(cgbdsm_query* q, uint64_t n) {
     SplitState& S = *q->split;
     if (!S.threads || !S.m_valid || !S.u_valid || n < std::max<uint64_t>(S.min_rows, 2)) return 0;
     uint64_t k = S.measured || S.ratio0 > 0 ? (uint64_t)(S.ratio * (double)n) : (uint64_t)S.threads;
+    if (!k && ++S.idle >= 16) k = (uint64_t)S.threads;  // probe the rates again
+    if (k) S.idle = 0;
     return std::min(k, n - 1);
 }
 
 void SplitStart(cgbdsm_query* q, const uint32_t* d_rows, uint64_t k, uint64_t n, uint32_t D) {
     SplitState& S = *q->split;
-    const uint64_t words = k * D;
-    if (words > S.h_rows_cap) {
-        if (S.h_rows) CGD_HIP(hipHostFree(S.h_rows));
-        S.h_rows = nullptr;
-        S.h_rows_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(words + words / 2, 1024);
-        CGD_HIP(hipHostMalloc((void**)&S.h_rows, cap * 4, hipHostMallocDefault));
-        S.h_rows_cap = cap;
-    }
-    CGD_HIP(hipMemcpyAsync(S.h_rows, d_rows, words * 4, hipMemcpyDeviceToHost, q->s));
+    S.rows.resize(k * D);
+    CGD_HIP(hipMemcpyAsync(S.rows.data(), d_rows, k * D * 4, hipMemcpyDeviceToHost, q->s));
     S.masks.resize((uint64_t)q->qe * kMaxQV * q->W);
     if (!S.masks.empty())
         CGD_HIP(hipMemcpyAsync(S.masks.data(), q->masks.p, S.masks.size() * 8, hipMemcpyDeviceToHost, q->s));
@@ -393,26 +510,18 @@ void SplitStart(cgbdsm_query* q, const uint32_t* d_rows, uint64_t k, uint64_t n,
     S.bad.store(0);
     memset(S.per_edge, 0, sizeof(S.per_edge));
     S.wide = false;
-    S.t0 = std::chrono::steady_clock::now();
+    S.counting = false;
+    S.t0 = S.t_count = S.t_end = Clock::now();
     const int t = (int)std::min<uint64_t>((uint64_t)S.threads, k);
     for (int i = 0; i < t; i++) S.workers.emplace_back(HostWorker, q);
-}
-
-void SplitJoin(cgbdsm_query* q, bool abort) {
-    SplitState& S = *q->split;
-    if (abort) {
-        uint32_t z = 0;
-        S.stop.compare_exchange_strong(z, 1);
-    }
-    for (auto& w : S.workers) w.join();
-    S.workers.clear();
-    S.cpu_ms = MsSince(S.t0);
 }
 
 void SplitFinish(cgbdsm_query* q, double gpu_ms) {
     SplitState& S = *q->split;
     SplitJoin(q, false);
-    if (S.bad.load()) Throw(CGBDSM_ERR_DEVICE, "host share of the final DFS read a frontier row with an out-of-range tag or vertex id");
+    Drain(S);
+    if (S.bad.load())
+        Throw(CGBDSM_ERR_DEVICE, "host share of the final DFS read a frontier row with an out-of-range tag or vertex id");
     if (S.stop.load() == 2)
         Throw(CGBDSM_ERR_OVERFLOW, "host share of the final DFS exceeded the deadline of " +
                                        std::to_string(q->deadline_ms) + " ms; count incomplete");
@@ -425,10 +534,14 @@ void SplitFinish(cgbdsm_query* q, double gpu_ms) {
         total += S.per_edge[e];
     }
     if (wide) q->h_cnt->count_ovf = 1;
+    const double wait_ms = MsBetween(S.t0, S.t_count), cpu_ms = MsBetween(S.t_count, S.t_end);
     const uint64_t gpu_rows = S.n - S.k;
-    if (S.alpha > 0 && S.cpu_ms > 1e-6 && gpu_ms > 1e-6 && gpu_rows) {
-        const double cpu_tp = (double)S.k / S.cpu_ms, gpu_tp = (double)gpu_rows / gpu_ms;
-        const double rt = cpu_tp / (cpu_tp + gpu_tp);
+    if (S.alpha > 0 && cpu_ms > 1e-6 && gpu_ms > 1e-6 && gpu_rows) {
+        // Rows per ms of each side; the host's k rows end with the GPU's
+        // n - k when wait + k / c = (n - k) / g.
+        const double c = (double)S.k / cpu_ms, g = (double)gpu_rows / gpu_ms, nn = (double)S.n;
+        const double kb = std::max(0.0, (nn / g - wait_ms) / (1.0 / c + 1.0 / g));
+        const double rt = kb / nn;
         S.ratio = S.measured || S.ratio0 > 0 ? S.alpha * rt + (1.0 - S.alpha) * S.ratio : rt;
         S.ratio = std::min(S.ratio, 0.95);
         S.measured = true;
@@ -436,7 +549,8 @@ void SplitFinish(cgbdsm_query* q, double gpu_ms) {
     S.batches++;
     S.last_cpu_rows = S.k;
     S.last_gpu_rows = gpu_rows;
-    S.last_cpu_ms = S.cpu_ms;
+    S.last_cpu_ms = cpu_ms;
+    S.last_wait_ms = wait_ms;
     S.last_gpu_ms = gpu_ms;
     S.last_cpu_matches = total;
 }
@@ -458,6 +572,7 @@ cgbdsm_status cgbdsm_set_cpu_split(cgbdsm_query* q, int threads, double ratio, d
         }
         if (!q->split) q->split = new SplitState();
         SplitState& S = *q->split;
+        Drain(S);
         S.threads = threads < 0 ? DefaultThreads() : threads;
         S.ratio = S.ratio0 = ratio;
         S.alpha = alpha;
@@ -479,9 +594,10 @@ cgbdsm_status cgbdsm_cpu_split_stats(cgbdsm_query* q, cgbdsm_split_stats* out) {
         out->last_gpu_rows = S.last_gpu_rows;
         out->last_cpu_ms = S.last_cpu_ms;
         out->last_gpu_ms = S.last_gpu_ms;
+        out->last_wait_ms = S.last_wait_ms;
         out->last_cpu_matches = S.last_cpu_matches;
-        out->mirror_entries = S.m_valid ? S.m_entries : 0;
-        out->mirror_ms = S.mirror_ms;
+        out->mirror_entries = S.m_valid ? S.m_entries.load() : 0;
+        out->mirror_ms = S.mirror_ms.load();
     });
 }
 
@@ -489,7 +605,8 @@ cgbdsm_status cgbdsm_cpu_mirror_export(cgbdsm_query* q, int which, uint64_t* out
     return Guard([&] {
         if (!q || (which != 0 && which != 1)) Throw(CGBDSM_ERR_INVALID, "bad query or index selector");
         if (!q->split) Throw(CGBDSM_ERR_INVALID, "the CPU split is off (cgbdsm_set_cpu_split)");
-        const SplitState& S = *q->split;
+        SplitState& S = *q->split;
+        Drain(S);
         if (which == 0 ? !S.m_valid : !S.u_valid) Throw(CGBDSM_ERR_INVALID, "the host mirror holds no such index yet");
         std::vector<uint64_t> h;
         auto emit = [&](uint64_t row, const uint32_t* v, uint64_t n) {

@@ -324,9 +324,11 @@ typedef struct {
     uint64_t last_gpu_rows;
     double last_cpu_ms;         /* host count, wall time                            */
     double last_gpu_ms;         /* GPU share, HIP events on the query's stream      */
+    double last_wait_ms;        /* host count's wait for the mirror upkeep          */
     uint64_t last_cpu_matches;  /* matches counted on the host                      */
     uint64_t mirror_entries;    /* entries of M in the host mirror                  */
-    double mirror_ms;           /* host time keeping the mirror in step, last batch */
+    double mirror_ms;           /* mirror upkeep of the last batch so far (host
+                                   time on its background thread)                  */
 } cgbdsm_split_stats;
 cgbdsm_status cgbdsm_cpu_split_stats(cgbdsm_query* q, cgbdsm_split_stats* out);
 /* The host mirror's M (which = 0) or the batch's U (which = 1) as sorted keys
