@@ -162,9 +162,11 @@ __device__ void FillPlanLdsWave(const Tables& T, PlanLds* L) {
     WaveSync();
 }
 
-// Uniform view of the plan for one edge tag.
+// Uniform view of the plan for one edge tag.  The plan tables are addressed
+// as LDS (a 32-bit address, not a generic pointer).
+typedef const __attribute__((address_space(3))) PlanLds LdsPlan;
 struct Ctx {
-    const PlanLds* L;
+    LdsPlan* L;
     uint64_t vs, cls, nbn;
     uint32_t qv, ei, sa, sb, ones;
     uint64_t app0, app1;
@@ -172,7 +174,7 @@ struct Ctx {
 
 __device__ __forceinline__ Ctx MakeCtx(const Tables& T, const PlanLds* L) {
     Ctx X;
-    X.L = L;
+    X.L = (LdsPlan*)L;
     X.vs = Uniform64(L->vs);
     X.cls = Uniform64(L->cls);
     X.nbn = Uniform64(L->nbn);
@@ -579,6 +581,18 @@ __device__ __forceinline__ void PutAgent(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ uint32_t GetAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Statistics words of one final-DFS wave, in LDS: added to without waiting
+// (lane 0) and read once at the end.  Held in scalar registers they were
+// spilled to memory by the pressure of the candidate loops, a round trip on
+// every step.
+struct WaveStats {
+    unsigned long long steps, step_lanes, batched, donated, scanned, probes;
+};
+typedef __attribute__((address_space(3))) WaveStats LdsStats;
+__device__ __forceinline__ void StatAdd(__attribute__((address_space(3))) unsigned long long& f, unsigned long long v) {
+    if (LaneId() == 0) __hip_atomic_fetch_add(&f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 __device__ __forceinline__ void PublishItem(StealQueue& Q, uint64_t slot, unsigned long long ticket) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
@@ -1244,11 +1258,14 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     __shared__ PlanLds s_plan[kWpb];
     __shared__ uint32_t s_csize[kWpb][kMaxQV];
     __shared__ BxLds s_bx[kWpb];
+    __shared__ WaveStats s_stats[kWpb];
     // Dynamic LDS, per wave: survivor stacks of surv_levels levels (64 u32
     // each; every job of the launch has qv - 1 <= surv_levels) and the
     // stable-intersection cache of stab_cap u32.
     extern __shared__ uint32_t s_dyn[];
-    const uint32_t wv = threadIdx.x / kWave, lane = LaneId();
+    // The wave's index, read first lane: the per-wave LDS pointers below are
+    // then scalar values, not per-lane copies.
+    const uint32_t wv = Uniform(threadIdx.x / kWave), lane = LaneId();
     uint32_t* surv = s_dyn + wv * (surv_levels * kWave + stab_cap);
     uint32_t* stab = surv + surv_levels * kWave;
     uint32_t* matched = s_matched[wv];
@@ -1308,14 +1325,17 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     // wall-clock poll (a lane cannot reach 2^32 within one poll period) and
     // at exit.
     uint32_t my_scanned = 0, my_probes = 0;
-    unsigned long long w_scanned = 0, w_probes = 0;
+    LdsStats* ws = (LdsStats*)&s_stats[wv];
+    if (lane == 0) ws->steps = ws->step_lanes = ws->batched = ws->donated = ws->scanned = ws->probes = 0;
+    WaveSync();
     auto fold = [&]() {
-        w_scanned += Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_scanned), kWave - 1, kWave));
-        w_probes += Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_probes), kWave - 1, kWave));
+        const unsigned long long a = Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_scanned), kWave - 1, kWave));
+        const unsigned long long b = Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_probes), kWave - 1, kWave));
+        StatAdd(ws->scanned, a);
+        StatAdd(ws->probes, b);
         my_scanned = 0;
         my_probes = 0;
     };
-    unsigned long long donated = 0, steps = 0, step_lanes = 0, batched = 0;
     uint64_t r_next = 0, r_end = 0, r_first = 0, ticket = 0;
     bool primary = true, idle = false, stop = false, have_ticket = false, from_row = false;
     uint32_t iter = 0, backoff = 1;
@@ -1347,9 +1367,11 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 ab = 1;
             }
         }
-        idl_hint = __shfl(idl, 0, kWave);
-        fl = __shfl(fl, 0, kWave);
-        return __shfl(ab, 0, kWave) || (fl & kFlagAborted);
+        // Uniform values (read first lane): branches on them stay scalar, so
+        // the loops' carried state can stay in scalar registers.
+        idl_hint = Uniform64(__shfl(idl, 0, kWave));
+        fl = Uniform64(__shfl(fl, 0, kWave));
+        return Uniform((uint32_t)__shfl(ab, 0, kWave)) || (fl & kFlagAborted);
     };
     while (!stop) {
         // ---------------- obtain a work item
@@ -1414,7 +1436,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             const uint64_t slot = ticket & (Q.cap - 1);
             unsigned long long sq = 0;
             if (lane == 0) sq = LoadAgent(&Q.seq[slot]);
-            sq = __shfl(sq, 0, kWave);
+            sq = Uniform64(__shfl(sq, 0, kWave));
             if (sq == ticket + 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 StealItem* it = &Q.items[slot];
@@ -1435,7 +1457,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             } else {
                 unsigned long long pend = 0;
                 if (lane == 0) pend = LoadAgent(&cnt->q_pending.v);
-                pend = __shfl(pend, 0, kWave);
+                pend = Uniform64(__shfl(pend, 0, kWave));
                 if (pend == 0) break;
                 if (poll()) break;
                 for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(63);
@@ -1450,7 +1472,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         sc.state = 0;
         WaveSync();
         unsigned long long count = 0;
-        uint32_t d = D0;
+        uint32_t d = Uniform(D0);
         for (;;) {
             // (every iteration at the batched level, whose steps are long:
             // donors must see idle wavefronts early)
@@ -1481,8 +1503,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                             h = LoadAgent(&cnt->q_head.v);
                             t = LoadAgent(&cnt->q_tail.v);
                         }
-                        h = __shfl(h, 0, kWave);
-                        t = __shfl(t, 0, kWave);
+                        h = Uniform64(__shfl(h, 0, kWave));
+                        t = Uniform64(__shfl(t, 0, kWave));
                         uint32_t k = ccount - pos - 1;
                         const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
                         k = k < want ? k : want;
@@ -1504,7 +1526,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                                 PutAgent(&it->hi, kFull);
                                 PublishItem(Q, slot, t0 + lane);
                             }
-                            donated += k;
+                            StatAdd(ws->donated, k);
                             ccount -= k;
                             idl_hint = idl_hint > k ? idl_hint - k : 0;
                             if (lane == 0) lvl[d].cnt = ccount;
@@ -1553,8 +1575,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                     h = LoadAgent(&cnt->q_head.v);
                     t = LoadAgent(&cnt->q_tail.v);
                 }
-                h = __shfl(h, 0, kWave);
-                t = __shfl(t, 0, kWave);
+                h = Uniform64(__shfl(h, 0, kWave));
+                t = Uniform64(__shfl(t, 0, kWave));
                 const uint32_t rem = limit - off;
                 uint32_t parts = (uint32_t)(idl_hint < 15 ? idl_hint + 1 : 16);
                 if (parts > rem / rmin) parts = rem / rmin;
@@ -1578,7 +1600,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                         PutAgent(&it->hi, plo + piece < limit ? plo + piece : limit);
                         PublishItem(Q, slot, t0 + lane);
                     }
-                    donated += k;
+                    StatAdd(ws->donated, k);
                     idl_hint = idl_hint > k ? idl_hint - k : 0;
                     limit = off + piece;
                     if (lane == 0) lvl[d].end = limit;
@@ -1588,8 +1610,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             const uint32_t i = off + lane;
             uint32_t c = 0;
             bool ok = false;
-            steps++;
-            step_lanes += limit - off < kWave ? limit - off : kWave;
+            StatAdd(ws->steps, 1);
+            StatAdd(ws->step_lanes, limit - off < kWave ? limit - off : kWave);
             if (wsearch) {
                 if (i < limit) {
                     c = At(L, i);
@@ -1608,7 +1630,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 if (m) {
                     count = SatAdd(count, BatchSuffix(X, T, d, matched, m, c, s_csize[wv], cvalid, sc, s_bx[wv],
                                                       my_scanned, my_probes, wsearch ? wmin : 0u));
-                    batched++;
+                    StatAdd(ws->batched, 1);
                 }
                 if (lane == 0) lvl[d].off = off + kWave;
                 WaveSync();
@@ -1656,13 +1678,14 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         }
     }
     fold();
+    WaveSync();
     if (lane == 0) {
-        atomicAdd(&cnt->scanned, w_scanned);
-        atomicAdd(&cnt->probes, w_probes);
-        atomicAdd(&cnt->steps, steps);
-        atomicAdd(&cnt->step_lanes, step_lanes);
-        if (batched) atomicAdd(&cnt->batched, batched);
-        if (donated) atomicAdd(&cnt->donated, donated);
+        atomicAdd(&cnt->scanned, (unsigned long long)ws->scanned);
+        atomicAdd(&cnt->probes, (unsigned long long)ws->probes);
+        atomicAdd(&cnt->steps, (unsigned long long)ws->steps);
+        atomicAdd(&cnt->step_lanes, (unsigned long long)ws->step_lanes);
+        if (ws->batched) atomicAdd(&cnt->batched, (unsigned long long)ws->batched);
+        if (ws->donated) atomicAdd(&cnt->donated, (unsigned long long)ws->donated);
     }
     flush_pe();
 }
