@@ -1203,7 +1203,7 @@ cgbdsm_status cgbdsm_set_deadline_ms(cgbdsm_query* q, double ms) {
 // The last kernel (the tag grouping) may still run on q's stream on return;
 // q->fr_ev marks the frontier's completion there, for a batched DFS launch
 // on another query's stream to wait on.
-const Frontier& BuildFrontier(cgbdsm_query* q) {
+static const Frontier& BuildFrontier(cgbdsm_query* q) {
     if (q->fr_ready) return q->fr;
     const EiList L = ActiveEdges(q);
     SetCounters(q, {{&Counters::rows_out, 0}, {&Counters::flags, 0}, {&Counters::tuples, 0}}, q->s);
@@ -1278,20 +1278,20 @@ const Frontier& BuildFrontier(cgbdsm_query* q) {
 
 // Zeroes the per-edge counts (and the frontier/tuple flags) of `q` on the
 // stream `s` that will run the final DFS.
-void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
+static void ResetMatchCounters(cgbdsm_query* q, hipStream_t s) {
     static_assert(offsetof(Counters, per_edge) == offsetof(Counters, count_ovf) + 8, "count_ovf precedes per_edge");
     FillU64((uint64_t*)&q->d_cnt->count_ovf, 0, 1 + kMaxQE, s);
     FillU64((uint64_t*)&q->d_cnt->tuples, 0, 1, s);
 }
 
-void ResetDfsCounters(cgbdsm_query* q) {
+static void ResetDfsCounters(cgbdsm_query* q) {
     SetCounters(q, {{&Counters::scanned, 0}, {&Counters::probes, 0}, {&Counters::donated, 0},
                     {&Counters::wave_ticks, 0}, {&Counters::idle_ticks, 0}, {&Counters::steps, 0},
                     {&Counters::step_lanes, 0}, {&Counters::batched, 0}}, q->s);
 }
 
 // Statistics of the final-DFS launch whose queue lived in `leader`.
-void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
+static void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
     ReadCounters(leader);
     const Counters* c = leader->h_cnt;
     if (c->flags & kFlagBadRow)
@@ -1305,7 +1305,7 @@ void CheckDfs(cgbdsm_query* leader, uint64_t rows) {
 // the launch's work queue (the query itself for a single-query launch).
 // A query whose batch count did not fit 64 bits (the count is then wrong:
 // the call fails instead of returning it).
-void CheckWide(cgbdsm_query* const* qs, int nq) {
+static void CheckWide(cgbdsm_query* const* qs, int nq) {
     std::string names;
     for (int i = 0; i < nq; i++)
         if (qs[i]->count_wide) names += (names.empty() ? "" : ", ") + std::to_string(i);
@@ -1313,8 +1313,8 @@ void CheckWide(cgbdsm_query* const* qs, int nq) {
         Throw(CGBDSM_ERR_OVERFLOW, "the batch's match count does not fit 64 bits (query " + names + " of the call)");
 }
 
-void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* launch, unsigned long long* num_matches,
-                    uint64_t* per_edge) {
+static void FinishMatching(cgbdsm_query* q, const Frontier& f, const cgbdsm_query* launch,
+                           unsigned long long* num_matches, uint64_t* per_edge) {
     const Counters* c = launch->h_cnt;
     q->st.last_dfs_scanned = c->scanned;
     q->st.last_dfs_probes = c->probes;

@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
             lu = ur.len;
         }
         const uint32_t incl = WaveInclusiveScan(lm + lu);
-        const uint32_t total = Uniform(__shfl(incl, kWave - 1, kWave));
+        const uint32_t total = ReadLane(incl, kWave - 1);
         if (!total) continue;
         incl_l[lane] = incl;
         s_lenm[wv][lane] = lm;
@@ -363,11 +363,11 @@ __global__ void __launch_bounds__(kThreads) KSeed(Tables T, EiList EL, uint32_t*
             }
             const uint32_t nr = (uint32_t)__popcll(ok);
             const uint32_t incl2 = WaveInclusiveScan(nr);
-            const uint32_t tot2 = Uniform(__shfl(incl2, kWave - 1, kWave));
+            const uint32_t tot2 = ReadLane(incl2, kWave - 1);
             if (!tot2) continue;
             unsigned long long base = 0;
             if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)tot2);
-            base = Uniform64(__shfl(base, 0, kWave));
+            base = ReadLane64(base, 0);
             if (base + tot2 > cap_rows) {
                 if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
                 continue;
@@ -425,11 +425,11 @@ __global__ void __launch_bounds__(kThreads) KSeed1(Tables T, EiList EL, uint32_t
         if ((m0 & use_u) && LdURec(T.u_rec + r).len) emit |= m0 & use_u;
         const uint32_t nr = (uint32_t)__popcll(emit);
         const uint32_t incl = WaveInclusiveScan(nr);
-        const uint32_t tot = Uniform(__shfl(incl, kWave - 1, kWave));
+        const uint32_t tot = ReadLane(incl, kWave - 1);
         if (!tot) continue;
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(&cnt->rows_out, (unsigned long long)tot);
-        base = Uniform64(__shfl(base, 0, kWave));
+        base = ReadLane64(base, 0);
         if (base + tot > cap_rows) {
             if (lane == 0) atomicOr(&cnt->flags, kFlagFrontier);
             continue;
@@ -671,7 +671,7 @@ __device__ __forceinline__ bool InListWave(const List& L, uint32_t c, bool act, 
             const uint32_t pv = __shfl(piv, (int)(pos + step - 1), kWave);
             if (pv < c) pos += step;
         }
-        const uint32_t top = __shfl(piv, kWave - 1, kWave);
+        const uint32_t top = ReadLane(piv, kWave - 1);
         if (act) {
             probes++;
             if (c <= top) {
@@ -765,11 +765,7 @@ __device__ __forceinline__ uint32_t ChooseList(const Ctx& X, const Tables& T, ui
         const uint32_t n = L.bl + L.al;
         key = ((n < (1u << 27) ? n : (1u << 27) - 1) << 4) | lane;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t other = __shfl_xor(key, o, kWave);
-        key = other < key ? other : key;
-    }
-    return Uniform(key) & 15u;
+    return WaveMin(key) & 15u;
 }
 
 // The cached stable intersection takes the LDS that the other per-wave
@@ -1156,7 +1152,7 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
             }
             const uint32_t ne = prod ? (from_src ? nmin : len) : 0u;
             const uint32_t incl = WaveInclusiveScan(ne);
-            const uint32_t total = Uniform(__shfl(incl, kWave - 1, kWave));
+            const uint32_t total = ReadLane(incl, kWave - 1);
             B.base[lane] = Ls.base;
             B.app[lane] = Ls.app;
             B.bl[lane] = Ls.bl;
@@ -1224,9 +1220,15 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
         const long long val = (long long)raw - (long long)hits;
         prod = val > 0 ? SatMul(prod, (unsigned long long)val) : 0ull;
     }
+    // Saturating sum over the lanes (a DPP scan; lane 63 holds the total).
     unsigned long long sum = prod;
-    for (int o = 32; o > 0; o >>= 1) sum = SatAdd(sum, __shfl_xor(sum, o, kWave));
-    return Uniform64(sum);
+    sum = SatAdd(sum, DppMove64<kDppShr1>(sum));
+    sum = SatAdd(sum, DppMove64<kDppShr2>(sum));
+    sum = SatAdd(sum, DppMove64<kDppShr4>(sum));
+    sum = SatAdd(sum, DppMove64<kDppShr8>(sum));
+    sum = SatAdd(sum, DppMove64<kDppBcast15, 0xa>(sum));
+    sum = SatAdd(sum, DppMove64<kDppBcast31, 0xc>(sum));
+    return ReadLane64(sum, kWave - 1);
 }
 
 // Final DFS.  Work items are frontier rows (taken `grab` at a time from a
@@ -1329,8 +1331,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     if (lane == 0) ws->steps = ws->step_lanes = ws->batched = ws->donated = ws->scanned = ws->probes = 0;
     WaveSync();
     auto fold = [&]() {
-        const unsigned long long a = Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_scanned), kWave - 1, kWave));
-        const unsigned long long b = Uniform64(__shfl(WaveInclusiveScan((unsigned long long)my_probes), kWave - 1, kWave));
+        const unsigned long long a = ReadLane64(WaveInclusiveScan((unsigned long long)my_scanned), kWave - 1);
+        const unsigned long long b = ReadLane64(WaveInclusiveScan((unsigned long long)my_probes), kWave - 1);
         StatAdd(ws->scanned, a);
         StatAdd(ws->probes, b);
         my_scanned = 0;
@@ -1369,9 +1371,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         }
         // Uniform values (read first lane): branches on them stay scalar, so
         // the loops' carried state can stay in scalar registers.
-        idl_hint = Uniform64(__shfl(idl, 0, kWave));
-        fl = Uniform64(__shfl(fl, 0, kWave));
-        return Uniform((uint32_t)__shfl(ab, 0, kWave)) || (fl & kFlagAborted);
+        idl_hint = ReadLane64(idl, 0);
+        fl = ReadLane64(fl, 0);
+        return ReadLane((uint32_t)ab, 0) || (fl & kFlagAborted);
     };
     while (!stop) {
         // ---------------- obtain a work item
@@ -1381,7 +1383,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             if (r_next >= r_end) {
                 unsigned long long r0 = 0;
                 if (lane == 0) r0 = atomicAdd(&cnt->q_rows.v, (unsigned long long)grab);
-                r0 = Uniform64(__shfl(r0, 0, kWave));
+                r0 = ReadLane64(r0, 0);
                 r_next = r0;
                 r_end = r0 + grab < n_rows ? r0 + grab : n_rows;
                 r_first = r0;
@@ -1428,7 +1430,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                 if (lane == 0) {
                     tk = atomicAdd(&cnt->q_head.v, 1ull);
                 }
-                ticket = Uniform64(__shfl(tk, 0, kWave));
+                ticket = ReadLane64(tk, 0);
                 have_ticket = true;
                 idle = true;
                 t_idle0 = (long long)Uniform64((unsigned long long)wall_clock64());
@@ -1436,7 +1438,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             const uint64_t slot = ticket & (Q.cap - 1);
             unsigned long long sq = 0;
             if (lane == 0) sq = LoadAgent(&Q.seq[slot]);
-            sq = Uniform64(__shfl(sq, 0, kWave));
+            sq = ReadLane64(sq, 0);
             if (sq == ticket + 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 StealItem* it = &Q.items[slot];
@@ -1457,7 +1459,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
             } else {
                 unsigned long long pend = 0;
                 if (lane == 0) pend = LoadAgent(&cnt->q_pending.v);
-                pend = Uniform64(__shfl(pend, 0, kWave));
+                pend = ReadLane64(pend, 0);
                 if (pend == 0) break;
                 if (poll()) break;
                 for (uint32_t b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(63);
@@ -1503,8 +1505,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                             h = LoadAgent(&cnt->q_head.v);
                             t = LoadAgent(&cnt->q_tail.v);
                         }
-                        h = Uniform64(__shfl(h, 0, kWave));
-                        t = Uniform64(__shfl(t, 0, kWave));
+                        h = ReadLane64(h, 0);
+                        t = ReadLane64(t, 0);
                         uint32_t k = ccount - pos - 1;
                         const uint32_t want = (uint32_t)(idl_hint < 8 ? 2 * idl_hint : 16);
                         k = k < want ? k : want;
@@ -1514,7 +1516,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                                 atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
                                 t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
                             }
-                            t0 = Uniform64(__shfl(t0, 0, kWave));
+                            t0 = ReadLane64(t0, 0);
                             if (lane < k) {
                                 const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                                 StealItem* it = &Q.items[slot];
@@ -1575,8 +1577,8 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                     h = LoadAgent(&cnt->q_head.v);
                     t = LoadAgent(&cnt->q_tail.v);
                 }
-                h = Uniform64(__shfl(h, 0, kWave));
-                t = Uniform64(__shfl(t, 0, kWave));
+                h = ReadLane64(h, 0);
+                t = ReadLane64(t, 0);
                 const uint32_t rem = limit - off;
                 uint32_t parts = (uint32_t)(idl_hint < 15 ? idl_hint + 1 : 16);
                 if (parts > rem / rmin) parts = rem / rmin;
@@ -1588,7 +1590,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                         atomicAdd(&cnt->q_pending.v, (unsigned long long)k);
                         t0 = atomicAdd(&cnt->q_tail.v, (unsigned long long)k);
                     }
-                    t0 = Uniform64(__shfl(t0, 0, kWave));
+                    t0 = ReadLane64(t0, 0);
                     if (lane < k) {
                         const uint64_t slot = (t0 + lane) & (Q.cap - 1);
                         StealItem* it = &Q.items[slot];

@@ -84,7 +84,7 @@ struct SView {
 };
 
 struct alignas(128) S1Cnt {
-    unsigned long long n_list[4];   // work-list sizes (see S1Query)
+    unsigned long long n_list[6];   // work-list sizes, one per work list L[0..5] (see S1Query)
     unsigned long long n_died, n_big, n_keys, n_upd, ovf, items, scan_total, newv;
     unsigned long long dirmask[2];  // directed query edges that received an update key
     unsigned long long pairs;       // label-compatible pairs (attach)

@@ -1,5 +1,5 @@
-// wave64 helpers for gfx950 (CDNA4): lane masks, DPP-backed scans through
-// __shfl_up, and an intra-wave LDS visibility fence.
+// wave64 helpers for gfx950 (CDNA4): lane masks, DPP scans and reductions,
+// lane reads, and an intra-wave LDS visibility fence.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,15 +24,65 @@ __device__ __forceinline__ void WaveSync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-template <class T>
-__device__ __forceinline__ T WaveInclusiveScan(T v) {
-    const uint32_t lane = LaneId();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T n = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += n;
-    }
+// DPP lane moves (GFX9 wave64 controls, VALU only: no LDS-pipe round trip
+// as a shuffle takes).  kDppShr<n> moves lane i - n of the same row of 16 to
+// lane i; kDppBcast15 (rows 1 and 3 enabled) moves lane 15 of the previous
+// row, kDppBcast31 (rows 2 and 3) moves lane 31.  A lane whose source lies
+// outside its row, or whose row is not enabled, reads `fill`.  Every lane of
+// the wavefront must be active.
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
+constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ uint32_t DppMove(uint32_t v, uint32_t fill = 0u) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, Ctrl, RowMask, 0xf, false);
+}
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ uint64_t DppMove64(uint64_t v) {
+    return ((uint64_t)DppMove<Ctrl, RowMask>((uint32_t)(v >> 32)) << 32) | DppMove<Ctrl, RowMask>((uint32_t)v);
+}
+
+// Inclusive prefix sums over the 64 lanes (rows of 16 first, then the row
+// totals carried by the two broadcasts).  Every lane must be active.
+__device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
+    v += DppMove<kDppShr1>(v);
+    v += DppMove<kDppShr2>(v);
+    v += DppMove<kDppShr4>(v);
+    v += DppMove<kDppShr8>(v);
+    v += DppMove<kDppBcast15, 0xa>(v);
+    v += DppMove<kDppBcast31, 0xc>(v);
     return v;
+}
+__device__ __forceinline__ unsigned long long WaveInclusiveScan(unsigned long long v) {
+    v += DppMove64<kDppShr1>(v);
+    v += DppMove64<kDppShr2>(v);
+    v += DppMove64<kDppShr4>(v);
+    v += DppMove64<kDppShr8>(v);
+    v += DppMove64<kDppBcast15, 0xa>(v);
+    v += DppMove64<kDppBcast31, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long WaveInclusiveScan(unsigned long v) {
+    return (unsigned long)WaveInclusiveScan((unsigned long long)v);
+}
+
+// Smallest value over the 64 lanes, in every lane's result (a min-scan,
+// lane 63 read back).  Every lane must be active.
+__device__ __forceinline__ uint32_t WaveMin(uint32_t v) {
+    v = min(v, DppMove<kDppShr1>(v, ~0u));
+    v = min(v, DppMove<kDppShr2>(v, ~0u));
+    v = min(v, DppMove<kDppShr4>(v, ~0u));
+    v = min(v, DppMove<kDppShr8>(v, ~0u));
+    v = min(v, DppMove<kDppBcast15, 0xa>(v, ~0u));
+    v = min(v, DppMove<kDppBcast31, 0xc>(v, ~0u));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Lane l's value as a scalar (v_readlane: read whatever the lane's
+// register holds, active or not).
+__device__ __forceinline__ uint32_t ReadLane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ uint64_t ReadLane64(uint64_t v, int l) {
+    return ((uint64_t)ReadLane((uint32_t)(v >> 32), l) << 32) | ReadLane((uint32_t)v, l);
 }
 
 template <class T>
