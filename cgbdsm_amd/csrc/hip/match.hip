@@ -1036,6 +1036,7 @@ struct BxLds {
     const uint32_t* base[kWave];
     const uint32_t* app[kWave];
     uint32_t bl[kWave], al[kWave], incl[kWave], cnt[kWave];  // al bit 31: scan the source, probe the list
+    uint32_t own[kWave];  // chunk position -> sibling whose segment starts there
 };
 
 // Product-suffix counts of all survivors of one 64-candidate step at the
@@ -1159,16 +1160,26 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
             B.al[lane] = from_src ? (Ls.al | 0x80000000u) : Ls.al;
             B.incl[lane] = incl;
             B.cnt[lane] = 0;
+            B.own[lane] = 0;
             WaveSync();
+            const uint32_t st = incl - ne;  // this sibling's first element
             for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
                 const uint32_t e = c0 + lane;
+                // Owner of element e: each non-empty segment marks its first
+                // element in this chunk (the one covering c0 marks lane 0), and
+                // a prefix max carries the marks forward.  Marks left from
+                // earlier chunks of this position name earlier siblings than
+                // the one covering c0, so they never win.
+                if (ne && st < c0 + kWave && incl > c0) B.own[st > c0 ? st - c0 : 0u] = lane;
+                WaveSync();
+                uint32_t lo = B.own[lane];
+                lo = max(lo, DppMove<kDppShr1>(lo));
+                lo = max(lo, DppMove<kDppShr2>(lo));
+                lo = max(lo, DppMove<kDppShr4>(lo));
+                lo = max(lo, DppMove<kDppShr8>(lo));
+                lo = max(lo, DppMove<kDppBcast15, 0xa>(lo));
+                lo = max(lo, DppMove<kDppBcast31, 0xc>(lo));
                 if (e < total) {
-                    uint32_t lo = 0, hi = kWave - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (B.incl[mid] > e) hi = mid;
-                        else lo = mid + 1;
-                    }
                     const uint32_t o = e - (lo ? B.incl[lo - 1] : 0u);
                     const uint32_t alw = B.al[lo];
                     scanned++;
