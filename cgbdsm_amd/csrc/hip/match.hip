@@ -1080,12 +1080,22 @@ __device__ unsigned long long BatchSuffix(const Ctx& X, const Tables& T, uint32_
         const uint32_t cls = Cls(X, j), qj = Vs(X, j);
         const bool same = pcls == cls;  // x_s itself may fall in C_j
         if (kp == kNoSkip) {
+            // Size and hits are both sibling-independent; the hits (earlier
+            // same-class positions inside C_j) hold while positions 0..p-1
+            // stay (bits 16.. of cvalid, dropped at every descent above p),
+            // so a level's steps after its first read both from LDS.
             const bool cached = (cvalid >> j) & 1u;
+            const bool hcached = cached && ((cvalid >> (16 + j)) & 1u);
             uint32_t raw = cached ? Uniform(csize[j]) : 0u, hits = 0;
-            SetSize(X, T, 1u << j, p, matched, !cached, raw, hits, sc, scanned, probes);
-            if (!cached) {
-                if (lane == 0) csize[j] = raw;
-                cvalid |= 1u << j;
+            if (hcached) {
+                hits = Uniform(csize[kMaxQV + j]);
+            } else {
+                SetSize(X, T, 1u << j, p, matched, !cached, raw, hits, sc, scanned, probes);
+                if (lane == 0) {
+                    csize[j] = raw;
+                    csize[kMaxQV + j] = hits;
+                }
+                cvalid |= (1u << j) | (1u << (16 + j));
             }
             long long val = (long long)raw - (long long)hits;
             // Is the sibling itself in C_j?  C_j's lists do not depend on
@@ -1269,7 +1279,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
     __shared__ uint32_t s_matched[kWpb][kMaxQV];
     __shared__ DfsLevel s_lvl[kWpb][kMaxQV];
     __shared__ PlanLds s_plan[kWpb];
-    __shared__ uint32_t s_csize[kWpb][kMaxQV];
+    __shared__ uint32_t s_csize[kWpb][2 * kMaxQV];  // cached raw sizes, then cached hits
     __shared__ BxLds s_bx[kWpb];
     __shared__ WaveStats s_stats[kWpb];
     // Dynamic LDS, per wave: survivor stacks of surv_levels levels (64 u32
@@ -1332,7 +1342,9 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
         wmin = (opts >> kJobWaveShift) & 31u ? 1u << ((opts >> kJobWaveShift) & 31u) : kWaveSearchMin;
         wmin = wmin < (uint32_t)kWave ? (uint32_t)kWave : wmin;
     };
-    uint32_t cvalid = 0;  // valid cached singleton sizes (bit per position)
+    // Valid cached singleton sizes (bit per position) and, from bit 16, valid
+    // cached hits of the batched suffix (dropped whenever a position changes).
+    uint32_t cvalid = 0;
     // Candidates scanned and binary-search probes: 32-bit per lane (two
     // fewer registers each), folded into the wave's 64-bit totals at every
     // wall-clock poll (a lane cannot reach 2^32 within one poll period) and
@@ -1552,7 +1564,7 @@ __global__ void __launch_bounds__(kThreads) CGD_DFS_ATTR KDfs(const DfsJob* __re
                         matched[d] = x;
                         lvl[d + 1] = DfsLevel{0, 0, 0, kFull, kNoSel};
                     }
-                    cvalid &= ~Uniform(s_plan[wv].dep_mask[d]);
+                    cvalid &= ~Uniform(s_plan[wv].dep_mask[d]) & 0xFFFFu;
                     if ((sc.deps >> d) & 1u) sc.state = 0;
                     WaveSync();
                     d++;
