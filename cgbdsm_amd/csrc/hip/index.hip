@@ -96,14 +96,14 @@ __device__ __forceinline__ void PushLong(bool push, uint64_t s, uint32_t* __rest
     const uint64_t m = __ballot(push);
     unsigned long long base = 0;
     if (m && LaneId() == 0) base = atomicAdd(n_long, (unsigned long long)__popcll(m));
-    base = __shfl(base, 0, kWave);
+    base = ReadLane64(base, 0);
     if (push) long_segs[base + RankBelow(m)] = (uint32_t)s;
 }
 
 __device__ __forceinline__ void FoldCounters(unsigned long long bytes, unsigned long long touched,
                                              Counters* __restrict__ cnt) {
-    bytes = __shfl(WaveInclusiveScan(bytes), 63, kWave);
-    touched = __shfl(WaveInclusiveScan(touched), 63, kWave);
+    bytes = ReadLane64(WaveInclusiveScan(bytes), 63);
+    touched = ReadLane64(WaveInclusiveScan(touched), 63);
     if (LaneId() == 0) {
         if (bytes) atomicAdd(&cnt->bytes, bytes);
         if (touched) atomicAdd(&cnt->rows_touched, touched);
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
             const uint32_t ncap = CapFor(newlen);
             unsigned long long off = 0;
             if (lane == 0) off = atomicAdd(&cnt->pool_top, (unsigned long long)ncap);
-            off = Uniform64(__shfl(off, 0, kWave));
+            off = ReadLane64(off, 0);
             if (off + ncap > pool_cap) {
                 if (lane == 0) atomicOr(&cnt->flags, kFlagPool);
                 continue;
@@ -586,7 +586,7 @@ __global__ void KCountDups(const uint64_t* __restrict__ k, uint64_t n, Counters*
     unsigned long long mine = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         mine += k[i] == k[i - 1];
-    const unsigned long long tot = __shfl(WaveInclusiveScan(mine), 63, kWave);
+    const unsigned long long tot = ReadLane64(WaveInclusiveScan(mine), 63);
     if (LaneId() == 0 && tot) atomicAdd(&cnt->u_dups, tot);
 }
 

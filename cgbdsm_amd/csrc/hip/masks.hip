@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) KMaskDepth(Tables T, uint32_t d, EiList L
         const uint64_t word = __ballot(ok);
         if (lane == 0) out[w] = word;
     }
-    const unsigned long long tot = __shfl(WaveInclusiveScan(reads), 63, kWave);
+    const unsigned long long tot = ReadLane64(WaveInclusiveScan(reads), 63);
     if (lane == 0 && tot) atomicAdd(&cnt->bytes, 4ull * tot);
 }
 

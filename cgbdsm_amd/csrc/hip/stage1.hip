@@ -119,10 +119,10 @@ __device__ __forceinline__ void DeadSet(uint32_t* dead, uint32_t v, uint32_t ui,
 // Per-lane slots on a shared counter for `cnt` items (all lanes call).
 __device__ __forceinline__ uint64_t WaveReserve(uint32_t cnt, unsigned long long* counter) {
     const uint32_t incl = WaveInclusiveScan(cnt);
-    const uint32_t total = __shfl(incl, 63, kWave);
+    const uint32_t total = ReadLane(incl, 63);
     unsigned long long base = 0;
     if (total && LaneId() == 63) base = atomicAdd(counter, (unsigned long long)total);
-    base = __shfl(base, 63, kWave);
+    base = ReadLane64(base, 63);
     return base + incl - cnt;
 }
 
@@ -160,7 +160,7 @@ __device__ __forceinline__ LbItem LbLocate(const uint64_t* __restrict__ offs, ui
         const uint64_t ee = __shfl(e, (int)(pos + step - 1), kWave);
         if (ee <= it) pos += step;
     }
-    const uint64_t last = __shfl(e, 63, kWave);
+    const uint64_t last = ReadLane64(e, 63);
     const uint64_t prev = __shfl(e, (int)(pos ? pos - 1 : 0), kWave);
     LbItem r;
     r.valid = it < total;
@@ -255,7 +255,7 @@ __global__ void KS1Codes(const int32_t* __restrict__ labels, uint32_t n, QTab Q,
         vstate[v] = (c << 12) | (light_all ? bc : 0u);
         mine += (unsigned long long)__popc(bc);
     }
-    const unsigned long long tot = __shfl(WaveInclusiveScan(mine), 63, kWave);
+    const unsigned long long tot = ReadLane64(WaveInclusiveScan(mine), 63);
     if (LaneId() == 0 && tot) atomicAdd(pairs, tot);
 }
 

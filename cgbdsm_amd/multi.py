@@ -18,6 +18,7 @@ Stage 2 run.  Two ways to run a set of queries over a batch:
 Counts are identical either way (every call is exact); only the schedule
 differs.
 """
+import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
@@ -158,9 +159,13 @@ class QuerySet:
     def run_served(self, tune=None, profile_lead=False):
         """The batch for every query: Stage 1 and the frontiers concurrently,
         one batched final-DFS launch, commits concurrently.  Returns the
-        counts and the lead query's DFS kernel time when profiled."""
+        counts and the lead query's DFS kernel time when profiled; the wall
+        time of the three phases is kept in `last_phases` (seconds:
+        prepare, matching, commit)."""
         ms = self.members
+        t0 = time.perf_counter()
         live = list(self.pool.map(self._prepare, ms))
+        t1 = t2 = time.perf_counter()
         ks = [k for k in range(len(ms)) if live[k]]
         out = [0] * len(ms)
         dfs = None
@@ -178,9 +183,11 @@ class QuerySet:
                 kt = lead.kernel_times()
                 lead.profile(False)
                 dfs = kt["dfs"]
+            t2 = time.perf_counter()
             list(self.pool.map(lambda k: ms[k].q.batch_update_global_index(), ks))
         for m in ms:
             m.q.sync()
+        self.last_phases = (t1 - t0, t2 - t1, time.perf_counter() - t2)
         return out, dfs
 
     def close(self):
