@@ -660,7 +660,9 @@ def cpu_baseline_config4(args, kept, w, log):
         kb = host.next_batch()
         s1 = time.perf_counter() - t1
         log(f"cpu baseline {sp.name}: setup {setup:.1f}s, [S1] {1e3 * s1:.0f} ms")
-        s2, frac, _ = cpu_s2_sampled(keys, dv, plan, kb, threads, args.cpu_seconds / 6, log)
+        # (half the CPU budget: the sampled keys' costs are heavy-tailed, and
+        # smaller samples gave estimates 3x apart between runs)
+        s2, frac, _ = cpu_s2_sampled(keys, dv, plan, kb, threads, args.cpu_seconds / 2, log)
         sample.append(f"{sp.name} (setup {setup:.1f}s untimed, [S2] from {100 * frac:.3g} % of the update keys)")
         per_q.append(s1 + s2)
         s1_tot += s1
