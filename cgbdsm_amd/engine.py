@@ -258,6 +258,7 @@ class DeviceQuery:
 
     def set_match_params(self, bfs_row_limit=65536, frontier_cap=0, donate_min=4):
         _check(self.L.cgbdsm_set_match_params(self.h, bfs_row_limit, frontier_cap, donate_min))
+        self.match_params = (bfs_row_limit, frontier_cap, donate_min)
 
     def set_layout(self, layout):
         """'gamma' (per-list merges) or 'gpma' (packed-memory array); before index_init."""

@@ -63,31 +63,38 @@ def autotune(engine, queries, grabs=GRABS, blocks_per_cu=BLOCKS_PER_CU, repeats=
 # wavefront splits off for idle ones (the reference's fixed STEAL_CHUNK /
 # TASK_SHARE_BOUND, kernels/enumeration_balance.cu:11-17); donate_min the
 # survivors left at which it hands work to the queue; cache_cap the u32 per
-# wavefront of the LDS stable-intersection cache (0: all the LDS left).
+# wavefront of the LDS stable-intersection cache (0: all the LDS left);
+# wave_search the shortest list (a power of two >= 64) searched by the whole
+# wavefront's pivots instead of one lane's binary search (0: never).
 RANGE_MIN = (256, 1024, 4096)
 DONATE_MIN = (1, 4, 16)
 CACHE_CAP = (0, 1024, 256)
+WAVE_SEARCH = (64, 256, 1024, 0)
 
 
 def autotune_full(engine, queries, grabs=GRABS, blocks_per_cu=BLOCKS_PER_CU, range_min=RANGE_MIN,
-                  donate_min=DONATE_MIN, cache_cap=CACHE_CAP, log=None):
-    """autotune, then the stealing knobs one after another.  Returns
-    (knobs, table, counts): knobs = {rows_per_grab, blocks_per_cu, range_min,
-    donate_min, cache_cap}, table = {knob: {value: DFS ms}}; every candidate
-    must give identical counts, and the fastest setting is left applied."""
+                  donate_min=DONATE_MIN, cache_cap=CACHE_CAP, wave_search=WAVE_SEARCH, log=None):
+    """autotune, then the stealing and search knobs one after another.
+    Returns (knobs, table, counts): knobs = {rows_per_grab, blocks_per_cu,
+    range_min, donate_min, cache_cap, wave_search}, table = {knob: {value: DFS
+    ms}}; every candidate must give identical counts, and the fastest setting
+    is left applied."""
     best, shape_table, ref = autotune(engine, queries, grabs=grabs, blocks_per_cu=blocks_per_cu, log=log)
     knobs = {"rows_per_grab": best[0], "blocks_per_cu": best[1], "range_min": 1024, "donate_min": 4,
-             "cache_cap": 0}
+             "cache_cap": 0, "wave_search": 64}
     table = {"shape": {f"{g}/{b}": ms for (g, b), ms in shape_table.items()}}
     lead = queries[0]
 
     def apply(k):
         for q in queries:
             q.set_dfs_steal(k["range_min"], k["cache_cap"])
-            q.set_match_params(bfs_row_limit=getattr(q, "bfs_row_limit", 65536), donate_min=k["donate_min"])
+            limit, cap, _ = getattr(q, "match_params", (65536, 0, 4))
+            q.set_match_params(bfs_row_limit=limit, frontier_cap=cap, donate_min=k["donate_min"])
+            q.set_dfs_search(k["wave_search"] > 0, k["wave_search"] or 64)
 
     lead.profile(True)
-    for name, values in (("range_min", range_min), ("donate_min", donate_min), ("cache_cap", cache_cap)):
+    for name, values in (("range_min", range_min), ("donate_min", donate_min), ("cache_cap", cache_cap),
+                         ("wave_search", wave_search)):
         table[name] = {}
         for v in values:
             trial = dict(knobs, **{name: v})

@@ -133,10 +133,11 @@ def test_matching_batch_rejects(eng):
 
 
 def test_dfs_tuning_keeps_counts(eng):
-    """Every final-DFS launch shape the autotuner tries counts the same batch
-    identically (checked by the tuner itself) and equal to the oracle; a
-    second cgbdsm_matching on the same batch counts it again."""
-    from cgbdsm_amd.tune import autotune
+    """Every final-DFS launch shape the autotuner tries, and every stealing
+    and search knob the full tuner tries, counts the same batch identically
+    (checked by the tuners themselves) and equal to the oracle; a second
+    cgbdsm_matching on the same batch counts it again."""
+    from cgbdsm_amd.tune import autotune, autotune_full
 
     print("case dfs tuning", flush=True)
     sess = _open(eng, [("dense", 5, 2), ("sparse", 6, 4), ("tree", 5, 3)], scale=11, batch_edges=300)
@@ -160,8 +161,17 @@ def test_dfs_tuning_keeps_counts(eng):
         qs = [sess[k][1] for k, _ in live]
         # one query alone (b == 0), then all of them in one launch
         sel = qs[:1] if b == 0 else qs
-        best, table, got = autotune(eng, sel, grabs=(1, 4, 16, 64), blocks_per_cu=(0, 1))
-        assert len(table) == 8 and best in table
+        if b == 0:
+            best, table, got = autotune(eng, sel, grabs=(1, 4, 16, 64), blocks_per_cu=(0, 1))
+            assert len(table) == 8 and best in table
+        else:
+            # the full tuner: every stealing and search candidate must give
+            # the counts of the first, which are checked against the oracle
+            cand = dict(range_min=(256, 4096), donate_min=(1, 16), cache_cap=(0, 256), wave_search=(64, 1024, 0))
+            knobs, table, got = autotune_full(eng, sel, grabs=(1, 16), blocks_per_cu=(0,), **cand)
+            assert len(table["shape"]) == 2
+            for name, values in cand.items():
+                assert set(table[name]) == set(values) and knobs[name] in values
         tuned += 1
         for (k, kb), (cnt, per) in zip(live, got):
             host, q, idx, plan = sess[k]
