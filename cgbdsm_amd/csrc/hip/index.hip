@@ -89,6 +89,9 @@ __device__ __forceinline__ uint32_t LowerBoundKeys(const uint64_t* __restrict__ 
 }
 
 constexpr uint32_t kShortWork = 48;
+// Chunks of 64 entries a long-list merge loads per step (independent loads in
+// flight per lane; a hub's list is thousands of chunks long).
+constexpr uint32_t kMergeUnroll = 4;
 
 // Appends segment s (lanes with push set) to the queue of long lists.
 __device__ __forceinline__ void PushLong(bool push, uint64_t s, uint32_t* __restrict__ long_segs,
@@ -292,11 +295,19 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
                 continue;
             }
             C = T.pool + off;
-            for (uint32_t k = 0; k < na; k += kWave) {
-                const uint32_t i = k + lane;
-                const uint32_t a = i < na ? A[i] : 0u;
-                const uint32_t r = in_lanes ? RankInLanes(held, a) : (i < na ? LowerBound(Bv, nbp, a) : 0u);
-                if (i < na) C[i + r] = a;
+            for (uint32_t k = 0; k < na; k += kWave * kMergeUnroll) {
+                uint32_t a[kMergeUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                    const uint32_t i = k + u * kWave + lane;
+                    a[u] = i < na ? A[i] : 0u;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                    const uint32_t i = k + u * kWave + lane;
+                    const uint32_t r = in_lanes ? RankInLanes(held, a[u]) : (i < na ? LowerBound(Bv, nbp, a[u]) : 0u);
+                    if (i < na) C[i + r] = a[u];
+                }
             }
             bytes += 4ull * na + 4ull * na;
             if (lane == 0) {
@@ -304,16 +315,33 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
                 rec.cap = ncap;
             }
         } else {
-            // In place: only A[i0..na) moves, back to front.
+            // In place: only A[i0..na) moves, back to front, kMergeUnroll
+            // chunks of 64 per step, all loaded before any is stored.  An
+            // entry moves up by the new values below it, which never exceeds
+            // the shift of the entries above it, so a store lands at or above
+            // its own source (read this step) and below every entry moved in
+            // earlier steps.
             const uint32_t i0 = Bp[0];
             for (uint32_t hi = na; hi > i0;) {
-                const uint32_t lo = hi > i0 + kWave ? hi - kWave : i0;
-                const uint32_t i = hi - 1 - lane;
-                const bool valid = lane < hi - lo;
-                const uint32_t a = valid ? A[i] : 0u;
-                const uint32_t r = in_lanes ? RankInLanes(held, a) : (valid ? LowerBound(Bv, nbp, a) : 0u);
+                const uint32_t lo = hi > i0 + kWave * kMergeUnroll ? hi - kWave * kMergeUnroll : i0;
+                uint32_t a[kMergeUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                    const uint32_t d = u * kWave + lane;  // distance below hi - 1
+                    a[u] = d < hi - lo ? A[hi - 1 - d] : 0u;
+                }
+                uint32_t r[kMergeUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                    const bool valid = u * kWave + lane < hi - lo;
+                    r[u] = in_lanes ? RankInLanes(held, a[u]) : (valid ? LowerBound(Bv, nbp, a[u]) : 0u);
+                }
                 WaveSync();
-                if (valid) A[i + r] = a;
+#pragma unroll
+                for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                    const uint32_t d = u * kWave + lane;
+                    if (d < hi - lo) A[hi - 1 - d + r[u]] = a[u];
+                }
                 WaveSync();
                 hi = lo;
             }
@@ -363,26 +391,40 @@ __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_
         // Fewer than 64 removed values: held one per lane (see KMergeInsert).
         const bool in_lanes = nd < (uint32_t)kWave;
         const uint32_t held = in_lanes && lane < nd ? (uint32_t)(D[lane] & vmask) : ~0u;
-        for (uint32_t k = i0; k < na; k += kWave) {
-            const uint32_t i = k + lane;
-            const bool valid = i < na;
-            const uint32_t a = valid ? A[i] : 0u;
-            bool del = false;
-            if (in_lanes) {
-                // every lane shuffles (a lane left out would not supply its value)
-                const uint32_t p = RankInLanes(held, a);
-                const uint32_t at = __shfl(held, (int)(p < (uint32_t)kWave ? p : kWave - 1), kWave);
-                del = valid && p < nd && at == a;
-            } else if (valid) {
-                const uint32_t p = LowerBoundKeys(D, nd, a, vmask);
-                del = p < nd && (uint32_t)(D[p] & vmask) == a;
+        // kMergeUnroll chunks of 64 are loaded before any of them is stored:
+        // the kept entries move down to w <= k, so a store never reaches an
+        // entry of a later step, and every entry of this step is in registers.
+        for (uint32_t k = i0; k < na; k += kWave * kMergeUnroll) {
+            uint32_t a[kMergeUnroll];
+            bool keep[kMergeUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                const uint32_t i = k + u * kWave + lane;
+                keep[u] = i < na;
+                a[u] = keep[u] ? A[i] : 0u;
             }
-            const bool keep = valid && !del;
-            const uint64_t m = __ballot(keep);
+#pragma unroll
+            for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                bool del = false;
+                if (in_lanes) {
+                    // every lane shuffles (a lane left out would not supply its value)
+                    const uint32_t p = RankInLanes(held, a[u]);
+                    const uint32_t at = __shfl(held, (int)(p < (uint32_t)kWave ? p : kWave - 1), kWave);
+                    del = keep[u] && p < nd && at == a[u];
+                } else if (keep[u]) {
+                    const uint32_t p = LowerBoundKeys(D, nd, a[u], vmask);
+                    del = p < nd && (uint32_t)(D[p] & vmask) == a[u];
+                }
+                keep[u] = keep[u] && !del;
+            }
             WaveSync();
-            if (keep) A[w + RankBelow(m)] = a;
+#pragma unroll
+            for (uint32_t u = 0; u < kMergeUnroll; u++) {
+                const uint64_t m = __ballot(keep[u]);
+                if (keep[u]) A[w + RankBelow(m)] = a[u];
+                w += (uint32_t)__popcll(m);
+            }
             WaveSync();
-            w += (uint32_t)__popcll(m);
         }
         bytes += 4ull * (na - i0) + 4ull * (w - i0);
         if (w != na) touched++;
