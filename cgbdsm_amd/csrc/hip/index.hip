@@ -88,6 +88,48 @@ __device__ __forceinline__ uint32_t LowerBoundKeys(const uint64_t* __restrict__ 
     return lo;
 }
 
+// Lower bound of a wave-uniform v in sorted a[0, n): each level loads 64
+// evenly spaced pivots (one per lane) and keeps the bucket that holds the
+// answer, so a list of n entries costs ceil(log64 n) dependent round trips
+// instead of log2 n.  Every lane must call it, with the same v.
+__device__ __forceinline__ uint32_t WaveLowerBound(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    const uint32_t lane = LaneId();
+    uint32_t lo = 0;
+    while (n > (uint32_t)kWave) {
+        const uint32_t step = (n + kWave - 1) / kWave;
+        const bool below = lane * step < n && a[lo + lane * step] < v;
+        const uint32_t c = (uint32_t)__popcll(__ballot(below));
+        if (c == 0) return lo;
+        // the answer lies after the last pivot below v, up to the next pivot
+        const uint32_t first = lo + (c - 1) * step + 1;
+        const uint32_t end = c * step < n ? lo + c * step : lo + n;
+        lo = first;
+        n = end - first;
+    }
+    const bool below = lane < n && a[lo + lane] < v;
+    return lo + (uint32_t)__popcll(__ballot(below));
+}
+
+// Lower bound of each lane's own v in sorted a[0, n), given the wave's 64
+// pivots held = a[lane * step] (~0 past the end, step = ceil(n / 64)): the
+// pivots below v name a bucket of step - 1 entries, searched alone.  The
+// shuffle search counts at most 63 pivots; lane 63's is checked apart.
+// Every lane must call it.
+__device__ __forceinline__ uint32_t LowerBoundPiv(const uint32_t* __restrict__ a, uint32_t n, uint32_t step,
+                                                  uint32_t held, uint32_t v) {
+    const uint32_t c = RankInLanes(held, v) + (ReadLane(held, kWave - 1) < v ? 1u : 0u);
+    if (c == 0) return 0;
+    const uint32_t first = (c - 1) * step + 1;
+    const uint32_t end = c * step < n ? c * step : n;
+    return first + LowerBound(a + first, end - first, v);
+}
+
+// The pivots of LowerBoundPiv for sorted a[0, n): a[lane * step].
+__device__ __forceinline__ uint32_t Pivots(const uint32_t* __restrict__ a, uint32_t n, uint32_t step) {
+    const uint32_t p = LaneId() * step;
+    return p < n ? a[p] : ~0u;
+}
+
 constexpr uint32_t kShortWork = 48;
 // Chunks of 64 entries a long-list merge loads per step (independent loads in
 // flight per lane; a hub's list is thousands of chunks long).
@@ -258,13 +300,17 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
         MRec rec = T.m_rec[row];
         uint32_t* A = T.pool + rec.off;
         const uint32_t na = rec.len;
-        // 1. B' = B \ A with final positions.
+        // 1. B' = B \ A with final positions (lists of more than 128
+        // entries searched from 64 pivots).
+        const bool a_piv = na > 2u * kWave;
+        const uint32_t astep = (na + kWave - 1) / kWave;
+        const uint32_t apiv = a_piv ? Pivots(A, na, astep) : 0u;
         uint32_t nbp = 0;
         for (uint32_t k = 0; k < nb; k += kWave) {
             const uint32_t j = k + lane;
             const bool valid = j < nb;
             const uint32_t b = valid ? (uint32_t)(keys[bs + j] & vmask) : 0u;
-            const uint32_t p = valid ? LowerBound(A, na, b) : 0u;
+            const uint32_t p = a_piv ? LowerBoundPiv(A, na, astep, apiv, b) : (valid ? LowerBound(A, na, b) : 0u);
             const bool keep = valid && !(p < na && A[p] == b);
             const uint64_t m = __ballot(keep);
             const uint32_t r = RankBelow(m);
@@ -281,9 +327,11 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
         const uint32_t* Bv = sc_val + bs;
         const uint32_t* Bp = sc_pos + bs;
         // Fewer than 64 new values: held one per lane, so a moved entry's
-        // shift (new values below it) is found by shuffles.
+        // shift (new values below it) is found by shuffles; more: 64 pivots
+        // held one per lane narrow each search to one bucket.
         const bool in_lanes = nbp < (uint32_t)kWave;
-        const uint32_t held = in_lanes && lane < nbp ? Bv[lane] : ~0u;
+        const uint32_t bstep = (nbp + kWave - 1) / kWave;
+        const uint32_t held = Pivots(Bv, nbp, bstep);
         uint32_t* C = A;
         if (newlen > rec.cap) {
             const uint32_t ncap = CapFor(newlen);
@@ -305,7 +353,7 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
 #pragma unroll
                 for (uint32_t u = 0; u < kMergeUnroll; u++) {
                     const uint32_t i = k + u * kWave + lane;
-                    const uint32_t r = in_lanes ? RankInLanes(held, a[u]) : (i < na ? LowerBound(Bv, nbp, a[u]) : 0u);
+                    const uint32_t r = in_lanes ? RankInLanes(held, a[u]) : LowerBoundPiv(Bv, nbp, bstep, held, a[u]);
                     if (i < na) C[i + r] = a[u];
                 }
             }
@@ -332,10 +380,8 @@ __global__ void __launch_bounds__(kThreads) KMergeInsert(Tables T, const uint64_
                 }
                 uint32_t r[kMergeUnroll];
 #pragma unroll
-                for (uint32_t u = 0; u < kMergeUnroll; u++) {
-                    const bool valid = u * kWave + lane < hi - lo;
-                    r[u] = in_lanes ? RankInLanes(held, a[u]) : (valid ? LowerBound(Bv, nbp, a[u]) : 0u);
-                }
+                for (uint32_t u = 0; u < kMergeUnroll; u++)
+                    r[u] = in_lanes ? RankInLanes(held, a[u]) : LowerBoundPiv(Bv, nbp, bstep, held, a[u]);
                 WaveSync();
 #pragma unroll
                 for (uint32_t u = 0; u < kMergeUnroll; u++) {
@@ -386,7 +432,7 @@ __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_
         uint32_t* A = T.pool + T.m_rec[row].off;
         bytes += 8ull * nd;
         if (na == 0) continue;
-        const uint32_t i0 = LowerBound(A, na, (uint32_t)(D[0] & vmask));
+        const uint32_t i0 = WaveLowerBound(A, na, (uint32_t)(D[0] & vmask));
         uint32_t w = i0;
         // Fewer than 64 removed values: held one per lane (see KMergeInsert).
         const bool in_lanes = nd < (uint32_t)kWave;
