@@ -124,6 +124,17 @@ __device__ __forceinline__ uint32_t LowerBoundPiv(const uint32_t* __restrict__ a
     return first + LowerBound(a + first, end - first, v);
 }
 
+// LowerBoundPiv over the values of sorted keys k[0, n) (value = key & vmask),
+// given held = the value of k[lane * step] (~0 past the end).
+__device__ __forceinline__ uint32_t LowerBoundKeysPiv(const uint64_t* __restrict__ k, uint32_t n, uint32_t step,
+                                                      uint32_t held, uint32_t v, uint64_t vmask) {
+    const uint32_t c = RankInLanes(held, v) + (ReadLane(held, kWave - 1) < v ? 1u : 0u);
+    if (c == 0) return 0;
+    const uint32_t first = (c - 1) * step + 1;
+    const uint32_t end = c * step < n ? c * step : n;
+    return first + LowerBoundKeys(k + first, end - first, v, vmask);
+}
+
 // The pivots of LowerBoundPiv for sorted a[0, n): a[lane * step].
 __device__ __forceinline__ uint32_t Pivots(const uint32_t* __restrict__ a, uint32_t n, uint32_t step) {
     const uint32_t p = LaneId() * step;
@@ -434,9 +445,11 @@ __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_
         if (na == 0) continue;
         const uint32_t i0 = WaveLowerBound(A, na, (uint32_t)(D[0] & vmask));
         uint32_t w = i0;
-        // Fewer than 64 removed values: held one per lane (see KMergeInsert).
+        // Fewer than 64 removed values: held one per lane (see KMergeInsert);
+        // more: 64 pivots held one per lane narrow each search to a bucket.
         const bool in_lanes = nd < (uint32_t)kWave;
-        const uint32_t held = in_lanes && lane < nd ? (uint32_t)(D[lane] & vmask) : ~0u;
+        const uint32_t dstep = (nd + kWave - 1) / kWave;
+        const uint32_t held = lane * dstep < nd ? (uint32_t)(D[lane * dstep] & vmask) : ~0u;
         // kMergeUnroll chunks of 64 are loaded before any of them is stored:
         // the kept entries move down to w <= k, so a store never reaches an
         // entry of a later step, and every entry of this step is in registers.
@@ -457,9 +470,9 @@ __global__ void __launch_bounds__(kThreads) KMergeRemove(Tables T, const uint64_
                     const uint32_t p = RankInLanes(held, a[u]);
                     const uint32_t at = __shfl(held, (int)(p < (uint32_t)kWave ? p : kWave - 1), kWave);
                     del = keep[u] && p < nd && at == a[u];
-                } else if (keep[u]) {
-                    const uint32_t p = LowerBoundKeys(D, nd, a[u], vmask);
-                    del = p < nd && (uint32_t)(D[p] & vmask) == a[u];
+                } else {
+                    const uint32_t p = LowerBoundKeysPiv(D, nd, dstep, held, a[u], vmask);
+                    del = keep[u] && p < nd && (uint32_t)(D[p] & vmask) == a[u];
                 }
                 keep[u] = keep[u] && !del;
             }
